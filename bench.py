@@ -1,0 +1,430 @@
+"""Benchmark: effective HBM bandwidth of the 30-qubit gate sweep on one MI355X.
+
+One step applies the sweep to a 30-qubit complex128 state of seeded complex
+Gaussian amplitudes resident in HBM: H, X, Z, RZ, CNOT, SWAP, fSim, CU3 and a
+seeded 3-qubit Haar unitary, each on target qubits 0, 15 and 29 (bit
+strides 2^29, 2^14, 1) -- 27 kernel launches.  `value` is the sweep's
+algorithmic bytes divided by the step time (GB/s).
+
+Algorithmic bytes of a gate: 2 x 128 B x (number of 128-byte lines of the
+state holding an amplitude the gate can change) -- read + write of every
+touched line, counted from the gate's index set, not measured.
+
+Also reported (not `value`): the same sweep in complex64, per-gate GB/s, the
+dominant kernel's roofline, QFT on 32 qubits complex64 (seconds), collapse /
+probabilities / 10^6-shot sampling on qubits {0, 7, 29}, a 33-qubit
+complex128 capacity run, and the CPU baseline: the C/OpenMP restatement of
+the reference's numba kernels on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu]
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--nqubits", type=int, default=30)
+    p.add_argument("--no-extras", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--only-sweep", action="store_true", help="sweep only (for profiler runs)")
+    return p.parse_args()
+
+
+# ------------------------------------------------------------ sweep spec --
+def sweep_gates(n, rng):
+    """(label, gate, selecting qubits, predicate on their bits) for the sweep."""
+    from qibojit_amd._qibo import gates
+
+    haar = []
+    for _ in range(3):
+        z = (rng.standard_normal((8, 8)) + 1j * rng.standard_normal((8, 8))) / np.sqrt(2)
+        q, r = np.linalg.qr(z)
+        haar.append(q * (np.diag(r) / np.abs(np.diag(r))))
+    out = []
+    theta = 0.731
+    for i, t in enumerate((0, n // 2, n - 1)):
+        p2 = (t, t + 1) if t < n - 1 else (t - 1, t)
+        ctrl = t + 1 if t < n - 1 else t - 1
+        h3 = (t, t + 1, t + 2) if t + 2 < n else (n - 3, n - 2, n - 1)
+        always = lambda bits: True  # noqa: E731
+        out += [
+            (f"H({t})", gates.H(t), (t,), always),
+            (f"X({t})", gates.X(t), (t,), always),
+            (f"Z({t})", gates.Z(t), (t,), lambda b, t=t: b[t] == 1),
+            (f"RZ({t})", gates.RZ(t, theta), (t,), always),
+            (f"CNOT({ctrl},{t})", gates.CNOT(ctrl, t), (ctrl, t), lambda b, c=ctrl: b[c] == 1),
+            (f"SWAP{p2}", gates.SWAP(*p2), p2, lambda b, p=p2: b[p[0]] != b[p[1]]),
+            (f"fSim{p2}", gates.fSim(*p2, 0.3, 0.4), p2, lambda b, p=p2: b[p[0]] == 1 or b[p[1]] == 1),
+            (f"CU3({ctrl},{t})", gates.CU3(ctrl, t, 0.1, 0.2, 0.3), (ctrl, t), lambda b, c=ctrl: b[c] == 1),
+            (f"Haar3{h3}", gates.Unitary(haar[i], *h3), h3, always),
+        ]
+    return out
+
+
+def changed_fraction(n, qubits, pred, amp_bytes):
+    """(line fraction, element fraction) of amplitudes the gate can change."""
+    line_bits = int(math.log2(128 // amp_bytes))
+    bits = {q: n - 1 - q for q in qubits}
+    high = [q for q in qubits if bits[q] >= line_bits]
+    low = [q for q in qubits if bits[q] < line_bits]
+    lines = 0
+    for hv in range(2 ** len(high)):
+        assign = {q: (hv >> i) & 1 for i, q in enumerate(high)}
+        for lv in range(2 ** len(low)):
+            assign.update({q: (lv >> i) & 1 for i, q in enumerate(low)})
+            if pred(assign):
+                lines += 1
+                break
+    elems = 0
+    for v in range(2 ** len(qubits)):
+        if pred({q: (v >> i) & 1 for i, q in enumerate(qubits)}):
+            elems += 1
+    return lines / 2 ** len(high), elems / 2 ** len(qubits)
+
+
+def kernel_name(backend, gate, n, dtype):
+    """Kernel template instance the dispatch selects (for grouping launches)."""
+    from qibojit_amd.backends.hip import GATE_OPS
+
+    r = "float" if dtype == "complex64" else "double"
+    qubits = backend._create_qubits_tensor(gate, n)  # pylint: disable=protected-access
+    v = 2 if int(qubits.min()) >= 1 else 1
+    k = len(gate.target_qubits)
+    name = gate.__class__.__name__
+    if k == 1:
+        kind = {"apply_x": 1, "apply_y": 2, "apply_z": 3, "apply_z_pow": 4}.get(GATE_OPS.get(name), 0)
+        return f"k_one_qubit<{r},{kind},{v}>"
+    if k == 2:
+        kind = {"apply_swap": 1, "apply_fsim": 2}.get(GATE_OPS.get(name), 0)
+        return f"k_two_qubit<{r},{kind},{v}>"
+    return f"k_multi_reg<{r},{k}>"
+
+
+# ----------------------------------------------------------------- sweep --
+def run_sweep(backend, n, dtype, steps, warmup, seed=1234):
+    import torch
+
+    rng = np.random.default_rng(seed)
+    spec = sweep_gates(n, rng)
+    amp = 8 if dtype == "complex64" else 16
+    state_bytes = amp * 2**n
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    tdt = torch.complex64 if dtype == "complex64" else torch.complex128
+    state = torch.randn(2**n, dtype=tdt, device="cuda", generator=gen)
+    state /= torch.linalg.vector_norm(state)
+    backend.set_dtype(dtype)
+    torch.cuda.synchronize()
+
+    for _ in range(warmup):
+        for _, g, _, _ in spec:
+            state = backend.apply_gate(g, state, n)
+    torch.cuda.synchronize()
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps * (len(spec) + 1))]
+    t0 = time.perf_counter()
+    k = 0
+    for _ in range(steps):
+        ev[k].record()
+        k += 1
+        for _, g, _, _ in spec:
+            state = backend.apply_gate(g, state, n)
+            ev[k].record()
+            k += 1
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    per_gate = {lbl: [] for lbl, _, _, _ in spec}
+    step_ms = []
+    k = 0
+    for _ in range(steps):
+        start = k
+        for lbl, _, _, _ in spec:
+            per_gate[lbl].append(ev[k].elapsed_time(ev[k + 1]))
+            k += 1
+        step_ms.append(ev[start].elapsed_time(ev[k]))
+        k += 1
+
+    rows = []
+    total_bytes = 0.0
+    for lbl, g, qs, pred in spec:
+        fl, fe = changed_fraction(n, qs, pred, amp)
+        b = 2 * state_bytes * fl
+        total_bytes += b
+        ms = float(np.median(per_gate[lbl]))
+        rows.append({
+            "gate": lbl,
+            "kernel": kernel_name(backend, g, n, dtype),
+            "bytes": b,
+            "elem_bytes": 2 * state_bytes * fe,
+            "ms_median": ms,
+            "ms_mean": float(np.mean(per_gate[lbl])),
+            "GBps": b / (ms * 1e-3) / 1e9,
+        })
+    ms_per_step = float(np.mean(step_ms))
+    norm = backend.norm_squared(state)
+    del state
+    torch.cuda.empty_cache()
+    return {
+        "rows": rows,
+        "bytes_per_step": total_bytes,
+        "ms_per_step": ms_per_step,
+        "GBps": total_bytes / (ms_per_step * 1e-3) / 1e9,
+        "wall_s": wall,
+        "final_norm2": norm,
+    }
+
+
+def dominant_kernel(rows):
+    by = {}
+    for r in rows:
+        d = by.setdefault(r["kernel"], {"ms": 0.0, "bytes": 0.0, "launches": 0})
+        d["ms"] += r["ms_mean"]
+        d["bytes"] += r["bytes"]
+        d["launches"] += 1
+    name = max(by, key=lambda k: by[k]["ms"])
+    d = by[name]
+    avg_ms = d["ms"] / d["launches"]
+    avg_bytes = d["bytes"] / d["launches"]
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    return name, avg_ms, avg_bytes, achieved, d["launches"]
+
+
+# ---------------------------------------------------------------- extras --
+def run_qft(backend, n, dtype):
+    """QFT (H + controlled-phase ladder + final swaps), unfused, from a seeded state."""
+    import torch
+
+    from qibojit_amd._qibo import Circuit, gates
+
+    c = Circuit(n)
+    for i in range(n):
+        c.add(gates.H(i))
+        for j in range(i + 1, n):
+            c.add(gates.CU1(j, i, np.pi / 2 ** (j - i)))
+    for i in range(n // 2):
+        c.add(gates.SWAP(i, n - i - 1))
+    backend.set_dtype(dtype)
+    tdt = torch.complex64 if dtype == "complex64" else torch.complex128
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    state = torch.randn(2**n, dtype=tdt, device="cuda", generator=gen)
+    state /= torch.linalg.vector_norm(state)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for g in c.queue:
+        state = backend.apply_gate(g, state, n)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3
+    amp = 8 if dtype == "complex64" else 16
+    # algorithmic bytes: H full pass; CU1 touches control=1 & target=1 (1/4);
+    # SWAP touches bit1 != bit2 (1/2); line rounding for low bits.
+    total = 0.0
+    for g in c.queue:
+        name = g.__class__.__name__
+        if name == "H":
+            fl = 1.0
+        elif name == "CU1":
+            fl, _ = changed_fraction(n, g.qubits, lambda b, q=g.qubits: b[q[0]] == 1 and b[q[1]] == 1, amp)
+        else:
+            fl, _ = changed_fraction(n, g.qubits, lambda b, q=g.qubits: b[q[0]] != b[q[1]], amp)
+        total += 2 * amp * 2**n * fl
+    norm = backend.norm_squared(state)
+    del state
+    torch.cuda.empty_cache()
+    return {"nqubits": n, "dtype": dtype, "gates": len(c.queue), "seconds": sec,
+            "algorithmic_GB": total / 1e9, "GBps": total / sec / 1e9, "final_norm2": norm}
+
+
+def run_measurement_ops(backend, n, dtype):
+    import torch
+
+    backend.set_dtype(dtype)
+    tdt = torch.complex64 if dtype == "complex64" else torch.complex128
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    state = torch.randn(2**n, dtype=tdt, device="cuda", generator=gen)
+    state /= torch.linalg.vector_norm(state)
+    qubits = [0, 7, n - 1]
+    amp = 8 if dtype == "complex64" else 16
+    out = {}
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    probs = backend.calculate_probabilities(state, qubits, n)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    out["probabilities"] = {"ms": dt * 1e3, "GBps": amp * 2**n / dt / 1e9}
+    t = time.perf_counter()
+    freqs = backend.measure_frequencies_op(
+        torch.zeros(8, dtype=torch.int64, device="cuda"), probs, 10**6, 3, 1234, backend.nthreads
+    )
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    out["metropolis_1e6_shots"] = {"ms": dt * 1e3, "chains": backend.nthreads, "total": int(freqs.sum())}
+    t = time.perf_counter()
+    backend.collapse_state(state, qubits, 5, n)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    b = amp * 2**n * (7 / 8 + 2 / 8)
+    out["collapse_normalized"] = {"ms": dt * 1e3, "GBps_at_1.125x": amp * 2**n * 1.125 / dt / 1e9,
+                                  "GBps_issued": b / dt / 1e9}
+    del state
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_capacity(backend, n=33):
+    """One dense 1-qubit gate on an n-qubit complex128 state (137 GB at n=33)."""
+    import torch
+
+    from qibojit_amd._qibo import gates
+
+    backend.set_dtype("complex128")
+    try:
+        state = backend.zero_state(n)  # values do not affect the timing
+    except (torch.cuda.OutOfMemoryError, MemoryError, RuntimeError) as exc:
+        return {"nqubits": n, "error": str(exc)[:200]}
+    g = gates.H(n // 2)
+    state = backend.apply_gate(g, state, n)  # warm
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    state = backend.apply_gate(g, state, n)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    b = 2 * 16 * 2**n
+    del state
+    torch.cuda.empty_cache()
+    return {"nqubits": n, "state_GB": 16 * 2**n / 1e9, "gate": f"H({n // 2})", "ms": ms, "GBps": b / ms * 1e3 / 1e9}
+
+
+def cpu_baseline(n):
+    """The reference's kernels restated in C/OpenMP, on this host's cores.
+
+    Sample: the 9 sweep gates on the mid-stride target (qubit n/2) of an
+    n-qubit complex128 state, one pass each; GB/s = algorithmic bytes / time.
+    """
+    from oracle import port
+
+    threads = len(os.sched_getaffinity(0))
+    pb = port.PortBackend(nthreads=threads)
+    rng = np.random.default_rng(1234)
+    spec = sweep_gates(n, rng)[9:18]  # the nine gates of the mid-stride target
+    state = np.empty(2**n, dtype=np.complex128)
+    state.fill(1 / math.sqrt(2**n))  # values do not affect the timing
+    total_b, total_t = 0.0, 0.0
+    for lbl, g, qs, pred in spec:
+        fl, _ = changed_fraction(n, qs, pred, 16)
+        t = time.perf_counter()
+        state = pb.apply_gate(g, state, n)
+        total_t += time.perf_counter() - t
+        total_b += 2 * 16 * 2**n * fl
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo", encoding="utf-8") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": total_b / total_t / 1e9,
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{len(spec)} sweep gates ({', '.join(s[0] for s in spec)}) on a {n}-qubit complex128 state, "
+                  f"one pass each, {total_t:.1f} s; C/OpenMP restatement of the numba kernels; CPU: {cpu}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: every box here has a single MI355X (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import torch
+
+    from qibojit_amd.backends.hip import HipBackend
+
+    backend = HipBackend()
+    n = args.nqubits
+    print(f"# sweep complex128 n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+    main_res = run_sweep(backend, n, "complex128", args.steps, args.warmup)
+    extras = {}
+    if not args.only_sweep:
+        print("# sweep complex64", file=sys.stderr, flush=True)
+        c64 = run_sweep(backend, n, "complex64", args.steps, args.warmup)
+        extras["sweep_complex64"] = {"GBps": c64["GBps"], "ms_per_step": c64["ms_per_step"],
+                                     "frac": c64["GBps"] / HBM_PEAK_GBS, "rows": c64["rows"]}
+    if not args.no_extras and not args.only_sweep:
+        print("# measurement ops", file=sys.stderr, flush=True)
+        extras["measurement_complex128"] = run_measurement_ops(backend, n, "complex128")
+        print("# qft 32q complex64", file=sys.stderr, flush=True)
+        extras["qft_32q_complex64"] = run_qft(backend, 32, "complex64")
+        print("# capacity 33q complex128", file=sys.stderr, flush=True)
+        extras["capacity_33q_complex128"] = run_capacity(backend, 33)
+
+    name, avg_ms, avg_bytes, achieved, launches = dominant_kernel(main_res["rows"])
+    roofline = {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "kernel": name,
+        "launches_per_step": launches,
+        "avg_launch_ms": avg_ms,
+        "algorithmic_bytes_per_launch": avg_bytes,
+    }
+    cpu = None
+    if not args.no_cpu and not args.only_sweep:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(n)
+
+    line = {
+        "metric": "eff. HBM GB/s (30q gate sweep); QFT circuit s; adjoint grads/s vs plain @1e-6",
+        "value": main_res["GBps"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex128",
+        "data": "synthetic (seeded complex Gaussian state, normalised)",
+        "config": {
+            "workload": f"{n}-qubit gate sweep: H, X, Z, RZ, CNOT, SWAP, fSim, CU3, 3q Haar on targets 0, {n // 2}, {n - 1}",
+            "nqubits": n,
+            "gates_per_step": len(main_res["rows"]),
+            "algorithmic_GB_per_step": main_res["bytes_per_step"] / 1e9,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "per_gate": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in main_res["rows"]],
+        "extras": extras,
+        "device": torch.cuda.get_device_name(0),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,192 @@
+/*
+ * qjhip.h -- C-ABI of the MI355X (gfx950) state-vector kernels.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns an int status (QJ_OK == 0).  On failure the thread-local message of
+ * qj_last_error() says what went wrong; the Python host maps QJ_EVALUE to
+ * ValueError, QJ_EMEMORY to MemoryError and QJ_EHIP to RuntimeError.
+ *
+ * State vectors live in device memory owned by the caller (any allocator:
+ * qj_malloc below, or a framework tensor's data pointer).  All calls that take
+ * a `stream` (a hipStream_t, NULL = the legacy default stream) are
+ * stream-ordered and do not synchronise the host, unless documented otherwise
+ * (functions returning a host value synchronise that stream).
+ *
+ * Bit convention (reference: src/qibojit/backends/cpu.py:565-569, :610):
+ * qubit q of an n-qubit register is bit (n - 1 - q) of the amplitude index,
+ * so qubit 0 is the most significant bit.
+ *
+ * Gate matrices are host pointers to row-major complex numbers of the state's
+ * precision (complex64 = 2 x float, complex128 = 2 x double).  They are copied
+ * into kernel arguments or a stream-ordered staging ring; the caller may reuse
+ * the host buffer as soon as the call returns.
+ *
+ * Concurrency: the library keeps per-device scratch (matrix staging ring,
+ * reduction partials).  Calls on one device must be issued on one stream at a
+ * time; calls issued concurrently on different streams of one device race.
+ */
+#ifndef QJHIP_H
+#define QJHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define QJ_OK 0
+#define QJ_EVALUE 1  /* bad argument (shape, qubit index, too many targets) */
+#define QJ_EMEMORY 2 /* device or pinned-host allocation failed */
+#define QJ_EHIP 3    /* HIP runtime error (launch failure, no device, ...) */
+
+/* ---- element types ---- */
+#define QJ_COMPLEX64 0
+#define QJ_COMPLEX128 1
+#define QJ_FLOAT32 2
+#define QJ_FLOAT64 3
+
+/* ---- one-qubit kernel kinds (reference names: GATE_OPS, cpu.py:23-36) ---- */
+#define QJ_APPLY_GATE 0  /* dense 2x2 */
+#define QJ_APPLY_X 1     /* swap of the pair */
+#define QJ_APPLY_Y 2     /* (a, b) -> (-i b, i a) */
+#define QJ_APPLY_Z 3     /* b -> -b */
+#define QJ_APPLY_Z_POW 4 /* b -> gate[0] * b */
+
+/* ---- two-qubit kernel kinds ---- */
+#define QJ_APPLY_TWO_QUBIT_GATE 0 /* dense 4x4 */
+#define QJ_APPLY_SWAP 1           /* |01> <-> |10> */
+#define QJ_APPLY_FSIM 2           /* 2x2 on {|01>,|10>} (gate[0..3]), phase gate[4] on |11> */
+
+/* ---- runtime ---- */
+
+/* Thread-local text of the last error ("" when none). */
+const char* qj_last_error(void);
+/* Version of this ABI; bumped on any signature change. */
+int qj_abi_version(void);
+/* Number of visible HIP devices (0 without a GPU; no error). */
+int qj_device_count(int* count);
+/* Make `device` current for the calling host thread. */
+int qj_set_device(int device);
+/* Device allocation of `nbytes` (QJ_EMEMORY on failure). */
+int qj_malloc(void** ptr, uint64_t nbytes);
+int qj_free(void* ptr);
+/* Stream-ordered copies host <-> device.  Pageable host memory makes these
+ * effectively synchronous for the host. */
+int qj_memcpy_htod(void* dst, const void* src, uint64_t nbytes, void* stream);
+int qj_memcpy_dtoh(void* dst, const void* src, uint64_t nbytes, void* stream);
+int qj_memcpy_dtod(void* dst, const void* src, uint64_t nbytes, void* stream);
+int qj_stream_synchronize(void* stream);
+
+/* ---- state preparation ----
+ * Replaces ops.initial_state_vector (src/qibojit/custom_operators/ops.py:10-18)
+ * and the CupyBackend zeros + initial_state_kernel pair
+ * (src/qibojit/backends/gpu.py:584-600, raw_kernels.py:548-558).
+ * Writes |0...0> into `nelements` amplitudes (a density matrix of dimension d
+ * is the d*d vector with element 0 set: ops.py:21-30). */
+int qj_initial_state(void* state, int dtype, uint64_t nelements, void* stream);
+
+/* ---- gate application ----
+ * `nqubits` is the length exponent of the vector (2*n for a density matrix).
+ * `qubits` is the reference's "qubits tensor" (cpu.py:565-569): the bit
+ * positions of controls and targets, sorted ascending, `nqubits_sorted` long
+ * (pass NULL/0 for a gate without controls: the targets are then implied).
+ * Bit positions of controls are the entries of `qubits` that are not targets.
+ */
+
+/* One-qubit gate on bit `m` (= nqubits - target - 1).
+ * Replaces NumbaBackend._one_qubit_base (cpu.py:606-616) with the kernels
+ * apply_gate/x/y/z/z_pow_kernel and their multicontrol_ variants
+ * (src/qibojit/custom_operators/gates.py:15-114; raw_kernels.py:128-221,
+ * 293-396).  `gate`: 4 complex (dense, row-major) for QJ_APPLY_GATE, 1 complex
+ * for QJ_APPLY_Z_POW, ignored (may be NULL) for X/Y/Z. */
+int qj_apply_one_qubit(void* state, int dtype, int nqubits, int kind, int m,
+                       const int32_t* qubits, int nqubits_sorted,
+                       const void* gate, void* stream);
+
+/* Two-qubit gate.  `bit1`/`bit2` are the bit positions of the FIRST and
+ * SECOND listed target: matrix row index = 2*bit(target1) + bit(target2).
+ * Replaces NumbaBackend._two_qubit_base (cpu.py:618-635), which encodes the
+ * same pair as (m1 < m2, swap_targets), and the kernels
+ * apply_two_qubit_gate/swap/fsim_kernel + multicontrol_ variants
+ * (gates.py:117-254; raw_kernels.py:224-290, 399-477).  `gate`: 16 complex
+ * (dense), 5 complex (fsim: gates.py:230-234), ignored for swap. */
+int qj_apply_two_qubit(void* state, int dtype, int nqubits, int kind, int bit1,
+                       int bit2, const int32_t* qubits, int nqubits_sorted,
+                       const void* gate, void* stream);
+
+/* Dense k-qubit gate, 1 <= k <= 12.  `target_bits[u]` is the bit position of
+ * the u-th listed target; target 0 is the most significant bit of the matrix
+ * index (equivalent to the reference's reversed int64 strides,
+ * cpu.py:594-596 and gates.py:257-262).  `gate`: 4^k complex, row-major.
+ * Replaces NumbaBackend._multi_qubit_base (cpu.py:581-604) with
+ * apply_three/four/five_qubit_gate_kernel and apply_multi_qubit_gate_kernel
+ * (gates.py:265-424; raw_kernels.py:480-518). */
+int qj_apply_multi_qubit(void* state, int dtype, int nqubits,
+                         const int32_t* target_bits, int ntargets,
+                         const int32_t* qubits, int nqubits_sorted,
+                         const void* gate, void* stream);
+
+/* ---- measurement ---- */
+
+/* Collapse onto outcome `result` of the measured bits `qubits_kernel`
+ * (= [n - q - 1 for q in reversed(measured qubits)], cpu.py:550-552); bit j of
+ * `result` is the value of bit qubits_kernel[j] (ops.py:33-40).  Zeroes every
+ * other amplitude and, when `normalize`, divides the kept ones by the FP64
+ * norm sqrt(sum |kept|^2).  Replaces ops.collapse_state /
+ * collapse_state_normalized (ops.py:43-79) and CupyBackend.collapse_state
+ * (gpu.py:616-644).  `norm_out` (host, may be NULL) receives that norm; when
+ * it is non-NULL the call synchronises `stream`. */
+int qj_collapse_state(void* state, int dtype, int nqubits,
+                      const int32_t* qubits_kernel, int nmeasured,
+                      int64_t result, int normalize, double* norm_out,
+                      void* stream);
+
+/* Marginal probabilities of `qubits` (qubit ids, in list order; the first is
+ * the most significant bit of the output index), accumulated in FP64 and
+ * written to `probs` (device, 2^nmeasured elements of `prob_dtype`
+ * QJ_FLOAT32/QJ_FLOAT64).  Replaces the qibo calculate_probabilities used by
+ * CupyBackend.calculate_probabilities (gpu.py:751-768). */
+int qj_calculate_probabilities(const void* state, int dtype, int nqubits,
+                               const int32_t* qubits, int nmeasured,
+                               void* probs, int prob_dtype, void* stream);
+
+/* Metropolis frequency sampler (ops.measure_frequencies, ops.py:82-108):
+ * `nchains` chains; chain c re-seeds an MT19937 generator with
+ * `chain_seeds[c]` (host array; the reference draws them with
+ * randint(0, 1e8) after seed(seed)), starts at argmax(probs), and for each of
+ * its shots proposes (shot + randint(0, nstates)) % nstates with NumPy-legacy
+ * masked randint, accepts when probs[new]/probs[shot] (computed in the probs
+ * type, compared in double) > random(), then counts the current shot.  Chain
+ * c runs nshots/nchains shots; the last chain also runs the remainder.
+ * `frequencies` (device int64, nstates) is ADDED to, as in the reference. */
+int qj_measure_frequencies(int64_t* frequencies, const void* probs,
+                           int prob_dtype, int64_t nstates, int64_t nshots,
+                           const uint32_t* chain_seeds, int nchains,
+                           void* stream);
+
+/* ---- reductions (FP64 accumulation; synchronise `stream`) ---- */
+
+/* sum |state_i|^2 over `nelements` amplitudes. */
+int qj_norm_squared(const void* state, int dtype, uint64_t nelements,
+                    double* out, void* stream);
+/* <a|b> = sum conj(a_i) b_i ; out = {re, im}. */
+int qj_vdot(const void* a, const void* b, int dtype, uint64_t nelements,
+            double* out, void* stream);
+/* <psi| H |psi> for the open-chain transverse-field Ising model
+ * H = -sum_{i<n-1} Z_i Z_{i+1} - g sum_i X_i. */
+int qj_expectation_tfim(const void* state, int dtype, int nqubits, double g,
+                        double* out, void* stream);
+
+/* ---- elementwise helpers (stream-ordered, no host sync) ---- */
+
+/* y = alpha * x + beta * y  (alpha, beta complex: {re, im}); with beta == 0
+ * y is write-only (it may hold uninitialised memory). */
+int qj_axpby(const void* x, void* y, int dtype, uint64_t nelements,
+             const double* alpha, const double* beta, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QJHIP_H */

@@ -1,0 +1,589 @@
+"""HipBackend: qibo's state-vector backend interface on MI355X (gfx950).
+
+The host side mirrors qibojit's NumbaBackend/CupyBackend dispatch
+(src/qibojit/backends/cpu.py:369-635, gpu.py:584-1076): a gate becomes a
+kernel family (GATE_OPS), a kernel-facing matrix (CustomMatrices), a sorted
+"qubits tensor" of bit positions and a target count, and then one call into
+``libqjhip.so``.  Unlike CupyBackend, gate matrices and qubit tensors stay on
+the host (they travel as kernel arguments or through the library's staging
+ring) and no call synchronises the host per gate.
+
+States are torch tensors on the current HIP device: torch provides device
+memory and the stream, every state-vector computation is a kernel of the
+library.  There is no CPU fallback; without the library or a device the
+constructor raises ImportError, as CupyBackend does without CUDA
+(gpu.py:85-91).
+"""
+
+import ctypes
+import math
+import os
+from collections import Counter
+
+import numpy as np
+import torch
+
+from qibojit_amd import _native
+from qibojit_amd._native import ONE_QUBIT_KINDS, TWO_QUBIT_KINDS, call, i32_array
+from qibojit_amd._qibo import (
+    SHOT_METROPOLIS_THRESHOLD,
+    Backend,
+    FusedGate,
+    NumpyMatrices,
+    ParametrizedGate,
+    QuantumState,
+    raise_error,
+)
+from qibojit_amd.backends.matrices import CustomMatrices
+from qibojit_amd.fusion import GATE_OPS, fused_matrix
+
+
+_TORCH_DTYPES = {
+    "complex64": torch.complex64,
+    "complex128": torch.complex128,
+    "float32": torch.float32,
+    "float64": torch.float64,
+    "int8": torch.int8,
+    "int32": torch.int32,
+    "int64": torch.int64,
+}
+_NATIVE_CODES = {
+    torch.complex64: _native.QJ_COMPLEX64,
+    torch.complex128: _native.QJ_COMPLEX128,
+    torch.float32: _native.QJ_FLOAT32,
+    torch.float64: _native.QJ_FLOAT64,
+}
+_REAL_OF = {torch.complex64: torch.float32, torch.complex128: torch.float64}
+
+
+def _dtype_name(dtype):
+    if isinstance(dtype, torch.dtype):
+        return str(dtype).replace("torch.", "")
+    return str(np.dtype(dtype))
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class HipBackend(Backend):
+    """qibo Backend whose state-vector operations run as gfx950 HIP kernels."""
+
+    MAX_NUM_TARGETS = 12
+
+    def __init__(self):
+        super().__init__()
+        from qibojit_amd import __version__  # pylint: disable=import-outside-toplevel
+
+        try:
+            _native.lib()
+        except OSError as exc:
+            raise ImportError(f"Could not load libqjhip.so: {exc}") from exc
+        if not torch.cuda.is_available() or _native.device_count() == 0:
+            raise ImportError("Could not detect HIP compatible devices.")
+        self.name = "qibojit"
+        self.platform = "hip"
+        self.engine = torch
+        self.np = np
+        self.dtype = "complex128"
+        self.device = "/GPU:0"
+        self._torch_device = torch.device("cuda", 0)
+        self.matrices = NumpyMatrices(self.dtype)
+        self.custom_matrices = CustomMatrices(self.dtype)
+        self.tensor_types = (np.ndarray, torch.Tensor)
+        self.supports_multigpu = False
+        self.oom_error = (MemoryError, torch.cuda.OutOfMemoryError)
+        self.versions.update(
+            {"qibojit_amd": __version__, "torch": torch.__version__, "hip": str(torch.version.hip)}
+        )
+        self.measure_frequencies_op = self.measure_frequencies
+        # Metropolis chain count, as NumbaBackend (cpu.py:86-89).
+        self.nthreads = len(os.sched_getaffinity(0))
+
+    # ------------------------------------------------------------ config --
+    def set_device(self, device):
+        if "GPU" not in device:
+            raise_error(ValueError, f"Device {device} is not available for {self} backend.")
+        idx = int(device.split(":")[-1]) if ":" in device else 0
+        if idx >= torch.cuda.device_count():
+            raise_error(ValueError, f"Device {device} is not available for {self} backend.")
+        self.device = device
+        self._torch_device = torch.device("cuda", idx)
+        torch.cuda.set_device(self._torch_device)
+        call("qj_set_device", idx)
+
+    def set_dtype(self, dtype):
+        dtype = _dtype_name(dtype)
+        if dtype != self.dtype:
+            super().set_dtype(dtype)
+            self.matrices = NumpyMatrices(self.dtype)
+            self.custom_matrices = CustomMatrices(self.dtype)
+
+    def set_seed(self, seed):
+        super().set_seed(seed)
+
+    def set_threads(self, nthreads):
+        self.nthreads = int(nthreads)
+
+    # ------------------------------------------------------------ arrays --
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self._torch_device).cuda_stream)
+
+    def cast(self, array, dtype=None, copy=False):
+        if dtype is None:
+            dtype = self.dtype
+        tdt = _TORCH_DTYPES[_dtype_name(dtype)]
+        if isinstance(array, torch.Tensor):
+            out = array.to(device=self._torch_device, dtype=tdt)
+            if copy and out.data_ptr() == array.data_ptr():
+                out = out.clone()
+            return out.contiguous()
+        arr = np.asarray(array)
+        if arr.dtype.kind == "c" and tdt in (torch.float32, torch.float64):
+            arr = arr.real
+        arr = np.ascontiguousarray(arr, dtype=_dtype_name(tdt))
+        return torch.from_numpy(arr).to(self._torch_device)
+
+    def to_numpy(self, array):
+        if isinstance(array, torch.Tensor):
+            return array.detach().cpu().numpy()
+        if isinstance(array, QuantumState):
+            return self.to_numpy(array.state())
+        return np.asarray(array)
+
+    def is_sparse(self, array):  # pylint: disable=unused-argument
+        return False
+
+    def nqubits_of(self, state):
+        n = state.shape[0]
+        return int(round(math.log2(n)))
+
+    def _as_state(self, state):
+        """Contiguous complex device tensor holding ``state`` (no copy if it is one)."""
+        if (
+            isinstance(state, torch.Tensor)
+            and state.device == self._torch_device
+            and state.dtype in (torch.complex64, torch.complex128)
+            and state.is_contiguous()
+        ):
+            return state
+        dtype = _dtype_name(state.dtype) if hasattr(state, "dtype") else self.dtype
+        if dtype not in ("complex64", "complex128"):
+            dtype = self.dtype
+        return self.cast(state, dtype=dtype)
+
+    @staticmethod
+    def _host_matrix(matrix, tdtype):
+        """Kernel-facing matrix as a contiguous host array of the state's precision."""
+        if isinstance(matrix, torch.Tensor):
+            matrix = matrix.detach().cpu().numpy()
+        npdt = np.complex64 if tdtype == torch.complex64 else np.complex128
+        return np.ascontiguousarray(np.asarray(matrix), dtype=npdt)
+
+    # -------------------------------------------------------- state prep --
+    def zero_state(self, nqubits, density_matrix=False, dtype=None):
+        if dtype is None:
+            dtype = self.dtype
+        tdt = _TORCH_DTYPES[_dtype_name(dtype)]
+        n = 1 << nqubits
+        numel = n * n if density_matrix else n
+        state = torch.empty(numel, dtype=tdt, device=self._torch_device)
+        call("qj_initial_state", _ptr(state), _NATIVE_CODES[tdt], numel, self._stream())
+        return state.reshape(n, n) if density_matrix else state
+
+    # ---------------------------------------------------- gate dispatch --
+    def apply_gate(self, gate, state, nqubits, inverse=False):
+        density_matrix = len(tuple(state.shape)) == 2
+        if density_matrix:
+            return self._apply_gate_density_matrix(gate, state, nqubits, inverse)
+        return self._apply_gate(gate, state, nqubits)
+
+    def apply_gate_density_matrix(self, gate, state, nqubits, inverse=False):
+        return self._apply_gate_density_matrix(gate, state, nqubits, inverse)
+
+    def apply_gate_half_density_matrix(self, gate, state, nqubits):
+        """rho -> U rho: the row pass of the density-matrix update only."""
+        if len(gate.control_qubits) > 0:
+            raise_error(
+                NotImplementedError,
+                "Gate density matrix half call is not implemented for ``controlled_by`` gates.",
+            )
+        state = self._as_state(state)
+        shape = state.shape
+        matrix = self._as_custom_matrix(gate)
+        qubits = self._create_qubits_tensor(gate, nqubits)
+        targets = gate.target_qubits
+        flat = state.reshape(-1)
+        name = gate.__class__.__name__
+        if len(targets) == 1:
+            op = GATE_OPS.get(name, "apply_gate")
+            flat = self._one_qubit_base(flat, 2 * nqubits, *targets, op, matrix, qubits + nqubits)
+        elif len(targets) == 2:
+            op = GATE_OPS.get(name, "apply_two_qubit_gate")
+            flat = self._two_qubit_base(flat, 2 * nqubits, *targets, op, matrix, qubits + nqubits)
+        else:
+            flat = self._multi_qubit_base(flat, 2 * nqubits, targets, matrix, qubits + nqubits)
+        return flat.reshape(shape)
+
+    def apply_channel(self, channel, state, nqubits):
+        density_matrix = len(tuple(state.shape)) == 2
+        if density_matrix:
+            return self._apply_channel_density_matrix(channel, state, nqubits)
+        raise_error(NotImplementedError, "Channels on state vectors are sampled by qibo, not by this backend.")
+
+    def _apply_channel_density_matrix(self, channel, state, nqubits):
+        """sum_i p_i U_i rho U_i^+ + (1 - sum p_i) rho (cpu.py:400-415)."""
+        state = self._as_state(state)
+        if not channel._all_unitary_operators:  # pylint: disable=protected-access
+            state_copy = state.clone()
+        new_state = torch.empty_like(state)
+        self._axpby(state, new_state, 1 - channel.coefficient_sum, 0.0)
+        for coeff, gate in zip(channel.coefficients, channel.gates):
+            state = self.apply_gate(gate, state, nqubits)
+            self._axpby(state, new_state, coeff, 1.0)
+            if not channel._all_unitary_operators:  # pylint: disable=protected-access
+                state = state_copy.clone()
+            else:
+                state = self.apply_gate(gate, state, nqubits, inverse=True)
+        return new_state
+
+    def _axpby(self, x, y, alpha, beta):
+        a = (ctypes.c_double * 2)(float(np.real(alpha)), float(np.imag(alpha)))
+        b = (ctypes.c_double * 2)(float(np.real(beta)), float(np.imag(beta)))
+        call("qj_axpby", _ptr(x), _ptr(y), _NATIVE_CODES[x.dtype], x.numel(), a, b, self._stream())
+
+    def _apply_fanout_gate(self, gate, state, nqubits):
+        """FanOut as a CNOT per target (cpu.py:417-431)."""
+        control, targets = gate.control_qubits[0], gate.target_qubits
+        for target in targets:
+            qubits = np.array(sorted([nqubits - control - 1, nqubits - target - 1]), dtype=np.int32)
+            state = self._one_qubit_base(state, nqubits, target, GATE_OPS["CNOT"], None, qubits)
+        return state
+
+    def _apply_gate(self, gate, state, nqubits):
+        """cpu.py:433-450."""
+        state = self._as_state(state)
+        if gate.name == "fanout":
+            return self._apply_fanout_gate(gate, state, nqubits)
+        matrix = self._as_custom_matrix(gate)
+        qubits = self._create_qubits_tensor(gate, nqubits)
+        targets = gate.target_qubits
+        if len(targets) == 1:
+            op = GATE_OPS.get(gate.__class__.__name__, "apply_gate")
+            return self._one_qubit_base(state, nqubits, *targets, op, matrix, qubits)
+        if len(targets) == 2:
+            op = GATE_OPS.get(gate.__class__.__name__, "apply_two_qubit_gate")
+            return self._two_qubit_base(state, nqubits, *targets, op, matrix, qubits)
+        return self._multi_qubit_base(state, nqubits, targets, matrix, qubits)
+
+    def _apply_gate_density_matrix(self, gate, state, nqubits, inverse=False):
+        """rho -> U rho U^+ as two passes over the 2n-qubit vector (cpu.py:452-498)."""
+        name = gate.__class__.__name__
+        if name in ["Y", "CY"]:
+            return self._apply_ygate_density_matrix(gate, state, nqubits)
+        if inverse:
+            matrix = np.linalg.inv(np.asarray(self.to_numpy(gate.matrix(self)), dtype=np.complex128))
+        else:
+            matrix = self._as_custom_matrix(gate)
+        qubits = self._create_qubits_tensor(gate, nqubits)
+        qubits_dm = qubits + nqubits
+        targets = gate.target_qubits
+        targets_dm = tuple(q + nqubits for q in targets)
+        state = self._as_state(state)
+        shape = state.shape
+        flat = state.reshape(-1)
+        cmatrix = np.conj(matrix) if matrix is not None else None
+        if len(targets) == 1:
+            op = GATE_OPS.get(name, "apply_gate")
+            flat = self._one_qubit_base(flat, 2 * nqubits, *targets, op, matrix, qubits_dm)
+            flat = self._one_qubit_base(flat, 2 * nqubits, *targets_dm, op, cmatrix, qubits)
+        elif len(targets) == 2:
+            op = GATE_OPS.get(name, "apply_two_qubit_gate")
+            flat = self._two_qubit_base(flat, 2 * nqubits, *targets, op, matrix, qubits_dm)
+            flat = self._two_qubit_base(flat, 2 * nqubits, *targets_dm, op, cmatrix, qubits)
+        else:
+            flat = self._multi_qubit_base(flat, 2 * nqubits, targets, matrix, qubits_dm)
+            flat = self._multi_qubit_base(flat, 2 * nqubits, targets_dm, cmatrix, qubits)
+        return flat.reshape(shape)
+
+    def _apply_ygate_density_matrix(self, gate, state, nqubits):
+        """Y needs the generic kernel on the column pass to conjugate (cpu.py:500-517)."""
+        matrix = self._as_custom_matrix(gate)
+        qubits = self._create_qubits_tensor(gate, nqubits)
+        qubits_dm = qubits + nqubits
+        targets = gate.target_qubits
+        targets_dm = tuple(q + nqubits for q in targets)
+        state = self._as_state(state)
+        shape = state.shape
+        flat = state.reshape(-1)
+        flat = self._one_qubit_base(flat, 2 * nqubits, *targets, "apply_y", matrix, qubits_dm)
+        flat = self._one_qubit_base(flat, 2 * nqubits, *targets_dm, "apply_gate", np.conj(matrix), qubits)
+        return flat.reshape(shape)
+
+    def _as_custom_matrix(self, gate):
+        """Kernel-facing encoding of ``gate`` (cpu.py:519-539)."""
+        name = gate.__class__.__name__
+        if isinstance(gate, FusedGate):
+            return self.matrix_fused(gate)
+        _matrix = getattr(self.custom_matrices, name)
+        if name == "FanOut":
+            return _matrix(*gate.init_args)
+        if isinstance(gate, ParametrizedGate):
+            if name == "GeneralizedRBS":
+                theta = gate.init_kwargs["theta"]
+                phi = gate.init_kwargs["phi"]
+                return _matrix(gate.init_args[0], gate.init_args[1], theta, phi)
+            params = [self.to_numpy(p) if isinstance(p, torch.Tensor) else p for p in gate.parameters]
+            return _matrix(*params)
+        return _matrix(2 ** len(gate.target_qubits)) if callable(_matrix) else _matrix
+
+    @staticmethod
+    def _create_qubits_tensor(gate, nqubits):
+        """Sorted int32 bit positions of controls and targets (cpu.py:565-569)."""
+        qubits = [nqubits - q - 1 for q in gate.control_qubits]
+        qubits.extend(nqubits - q - 1 for q in gate.target_qubits)
+        return np.array(sorted(qubits), dtype=np.int32)
+
+    def _one_qubit_base(self, state, nqubits, target, kernel, gate, qubits):
+        """cpu.py:606-616 -> qj_apply_one_qubit."""
+        state = self._as_state(state)
+        kind = ONE_QUBIT_KINDS[kernel]
+        m = nqubits - target - 1
+        mat = None
+        if kind in (ONE_QUBIT_KINDS["apply_gate"], ONE_QUBIT_KINDS["apply_z_pow"]):
+            mat = self._host_matrix(gate, state.dtype)
+        qptr, keep, nq = (None, None, 0) if qubits is None else i32_array(self._seq(qubits))
+        call(
+            "qj_apply_one_qubit",
+            _ptr(state),
+            _NATIVE_CODES[state.dtype],
+            nqubits,
+            kind,
+            m,
+            qptr,
+            nq,
+            None if mat is None else mat.ctypes.data_as(ctypes.c_void_p),
+            self._stream(),
+        )
+        del keep
+        return state
+
+    def _two_qubit_base(self, state, nqubits, target1, target2, kernel, gate, qubits):
+        """cpu.py:618-635 -> qj_apply_two_qubit (first target = matrix MSB)."""
+        state = self._as_state(state)
+        kind = TWO_QUBIT_KINDS[kernel]
+        mat = None
+        if kind != TWO_QUBIT_KINDS["apply_swap"]:
+            mat = self._host_matrix(gate, state.dtype)
+        qptr, keep, nq = (None, None, 0) if qubits is None else i32_array(self._seq(qubits))
+        call(
+            "qj_apply_two_qubit",
+            _ptr(state),
+            _NATIVE_CODES[state.dtype],
+            nqubits,
+            kind,
+            nqubits - target1 - 1,
+            nqubits - target2 - 1,
+            qptr,
+            nq,
+            None if mat is None else mat.ctypes.data_as(ctypes.c_void_p),
+            self._stream(),
+        )
+        del keep
+        return state
+
+    def _multi_qubit_base(self, state, nqubits, targets, gate, qubits):
+        """cpu.py:581-604 -> qj_apply_multi_qubit."""
+        state = self._as_state(state)
+        targets = list(targets)
+        if len(targets) > self.MAX_NUM_TARGETS:
+            raise ValueError(
+                f"Number of target qubits must be <= {self.MAX_NUM_TARGETS} but is {len(targets)}."
+            )
+        if qubits is None:
+            qubits = sorted(nqubits - q - 1 for q in targets)
+        mat = self._host_matrix(gate, state.dtype)
+        tptr, tkeep, nt = i32_array([nqubits - t - 1 for t in targets])
+        qptr, qkeep, nq = i32_array(self._seq(qubits))
+        call(
+            "qj_apply_multi_qubit",
+            _ptr(state),
+            _NATIVE_CODES[state.dtype],
+            nqubits,
+            tptr,
+            nt,
+            qptr,
+            nq,
+            mat.ctypes.data_as(ctypes.c_void_p),
+            self._stream(),
+        )
+        del tkeep, qkeep
+        return state
+
+    @staticmethod
+    def _seq(qubits):
+        if isinstance(qubits, torch.Tensor):
+            qubits = qubits.detach().cpu().numpy()
+        return [int(q) for q in np.asarray(qubits).reshape(-1)]
+
+    # ------------------------------------------------------------ fusion --
+    def matrix_fused(self, fgate):
+        """Dense matrix of a FusedGate over its sorted qubits (host, complex128)."""
+        return fused_matrix(self._as_custom_matrix, fgate).astype(self.dtype)
+
+    # ------------------------------------------------------- measurement --
+    def collapse_state(self, state, qubits, shot, nqubits, normalize=True, density_matrix=False):
+        """Project onto outcome ``shot`` of ``qubits`` (gpu.py:616-644, cpu.py:541-563)."""
+        if density_matrix or len(tuple(state.shape)) == 2:
+            raise_error(NotImplementedError, "Density-matrix collapse is not implemented by the HIP backend yet.")
+        if isinstance(shot, torch.Tensor):
+            shot = int(shot.reshape(-1)[0])
+        elif hasattr(shot, "shape"):
+            shot = int(shot.flat[0])
+        elif hasattr(shot, "__len__"):
+            shot = int(shot[0])
+        else:
+            shot = int(shot)
+        state = self._as_state(state)
+        qptr, keep, nq = i32_array([nqubits - q - 1 for q in reversed(list(qubits))])
+        call(
+            "qj_collapse_state",
+            _ptr(state),
+            _NATIVE_CODES[state.dtype],
+            nqubits,
+            qptr,
+            nq,
+            shot,
+            1 if normalize else 0,
+            None,
+            self._stream(),
+        )
+        del keep
+        return state
+
+    def _collapse_statevector(self, state, qubits, shot, nqubits, normalize=True):
+        return self.collapse_state(state, qubits, shot, nqubits, normalize)
+
+    def calculate_probabilities(self, state, qubits, nqubits, density_matrix=False):
+        """Marginal |psi|^2 over ``qubits`` in list order, FP64 accumulation."""
+        if density_matrix or len(tuple(state.shape)) == 2:
+            raise_error(NotImplementedError, "Density-matrix probabilities are not implemented by the HIP backend yet.")
+        state = self._as_state(state)
+        rdt = _REAL_OF[state.dtype]
+        probs = torch.empty(1 << len(qubits), dtype=rdt, device=self._torch_device)
+        qptr, keep, nq = i32_array(list(qubits))
+        call(
+            "qj_calculate_probabilities",
+            _ptr(state),
+            _NATIVE_CODES[state.dtype],
+            nqubits,
+            qptr,
+            nq,
+            _ptr(probs),
+            _NATIVE_CODES[rdt],
+            self._stream(),
+        )
+        del keep
+        return probs
+
+    def sample_frequencies(self, probabilities, nshots):
+        """cpu.py:383-394: Metropolis chains from SHOT_METROPOLIS_THRESHOLD shots on."""
+        if nshots < SHOT_METROPOLIS_THRESHOLD:
+            return super().sample_frequencies(probabilities, nshots)
+        seed = self.random_integers(0, int(1e8), dtype=np.int64)[0]
+        nqubits = int(math.log2(tuple(probabilities.shape)[0]))
+        frequencies = torch.zeros(2**nqubits, dtype=torch.int64, device=self._torch_device)
+        frequencies = self.measure_frequencies_op(
+            frequencies, probabilities, nshots, nqubits, seed, self.nthreads
+        )
+        freqs = self.to_numpy(frequencies)
+        return Counter({i: int(f) for i, f in enumerate(freqs) if f > 0})
+
+    def measure_frequencies(self, frequencies, probs, nshots, nqubits, seed, nthreads):
+        """Metropolis frequency sampler on the device (ops.py:82-108).
+
+        The chain seeds are drawn on the host exactly as the reference does
+        (seed(seed); randint(0, int(1e8)) per chain, NumPy legacy stream); the
+        chains themselves, MT19937 included, run in the HIP kernel.  Inputs
+        may be host arrays or device tensors; the result has the container
+        type and dtype of ``frequencies`` and includes its initial counts.
+        """
+        del nqubits
+        rs = np.random.RandomState(int(seed))
+        seeds = np.array([rs.randint(0, int(1e8)) for _ in range(int(nthreads))], dtype=np.uint32)
+        host_out = not isinstance(frequencies, torch.Tensor)
+        out_dtype = frequencies.dtype
+        pdt = _dtype_name(probs.dtype)
+        if pdt not in ("float32", "float64"):
+            pdt = "float64"
+        dprobs = self.cast(probs, dtype=pdt)
+        dfreq = self.cast(frequencies, dtype="int64")
+        sptr = seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        call(
+            "qj_measure_frequencies",
+            _ptr(dfreq),
+            _ptr(dprobs),
+            _NATIVE_CODES[dprobs.dtype],
+            dprobs.numel(),
+            int(nshots),
+            sptr,
+            int(nthreads),
+            self._stream(),
+        )
+        if host_out:
+            frequencies[...] = self.to_numpy(dfreq).astype(out_dtype)
+            return frequencies
+        if dfreq.data_ptr() != frequencies.data_ptr():
+            frequencies.copy_(dfreq.to(out_dtype))
+        return frequencies
+
+    # ----------------------------------------------------- reductions --
+    def norm_squared(self, state):
+        """sum |psi_i|^2 in FP64."""
+        state = self._as_state(state)
+        out = ctypes.c_double(0.0)
+        call("qj_norm_squared", _ptr(state), _NATIVE_CODES[state.dtype], state.numel(), ctypes.byref(out), self._stream())
+        return out.value
+
+    def vdot(self, a, b):
+        """<a|b> in FP64."""
+        a, b = self._as_state(a), self._as_state(b)
+        if a.dtype != b.dtype or a.numel() != b.numel():
+            raise ValueError("vdot needs two states of equal size and dtype.")
+        out = (ctypes.c_double * 2)()
+        call("qj_vdot", _ptr(a), _ptr(b), _NATIVE_CODES[a.dtype], a.numel(), out, self._stream())
+        return complex(out[0], out[1])
+
+    def expectation_tfim(self, state, nqubits, g=1.0):
+        """<psi| -sum Z_i Z_{i+1} - g sum X_i |psi> (open chain), FP64."""
+        state = self._as_state(state)
+        out = ctypes.c_double(0.0)
+        call("qj_expectation_tfim", _ptr(state), _NATIVE_CODES[state.dtype], nqubits, float(g), ctypes.byref(out), self._stream())
+        return out.value
+
+    # ------------------------------------------------------- execution --
+    def execute_circuit(self, circuit, initial_state=None, nshots=1000):
+        """Apply the circuit's queue to |0...0> or ``initial_state``."""
+        del nshots
+        nqubits = circuit.nqubits
+        density_matrix = bool(getattr(circuit, "density_matrix", False))
+        if initial_state is None:
+            state = self.zero_state(nqubits, density_matrix=density_matrix)
+        else:
+            if isinstance(initial_state, QuantumState):
+                initial_state = initial_state.state()
+            dtype = _dtype_name(initial_state.dtype) if hasattr(initial_state, "dtype") else self.dtype
+            state = self.cast(initial_state, dtype=dtype, copy=True)
+        for gate in circuit.queue:
+            if density_matrix:
+                state = gate.apply_density_matrix(self, state, nqubits)
+            else:
+                state = gate.apply(self, state, nqubits)
+        return QuantumState(state, self)
+
+    def assert_allclose(self, value, target, rtol=1e-7, atol=0.0):
+        if isinstance(value, QuantumState):
+            value = value.state()
+        if isinstance(target, QuantumState):
+            target = target.state()
+        np.testing.assert_allclose(self.to_numpy(value), self.to_numpy(target), rtol=rtol, atol=atol)

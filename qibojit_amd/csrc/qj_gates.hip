@@ -1,0 +1,446 @@
+// Gate-application kernels for gfx950 and their C-ABI entry points.
+//
+// Every kernel is one in-place pass over the amplitudes a gate can change:
+// the free index g (the bits that are neither targets nor controls) is mapped
+// to a base index by inserting the gate's bit positions (insert_bits), and
+// each work item owns the 2^k amplitudes base + offset(r) -- a bijection, so
+// work items never overlap and no synchronisation is needed
+// (custom_operators/gates.py:5-12 is the same bijection).
+//
+// Memory shape: consecutive lanes take consecutive g, so the loads of a wave
+// are contiguous runs of the state for every target position; when every
+// gate bit is >= 1 the lowest free bit is bit 0 and a lane moves V = 2
+// consecutive amplitudes per access (16 B for complex64, 32 B for
+// complex128).  Z-like kernels (z, z_pow) read and write only the half they
+// change; swap only touches |01>,|10>; fsim skips |00>.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "qj_common.h"
+
+namespace qj {
+
+template <typename R>
+struct M1 {
+  cplx<R> m[4];
+};
+template <typename R>
+struct M2 {
+  cplx<R> m[16];
+};
+
+constexpr int kIlp = 2;  // independent work items per thread per iteration
+
+// --------------------------------------------------------------- 1 qubit --
+template <typename R, int KIND, int V>
+__global__ void __launch_bounds__(kBlock)
+    k_one_qubit(cplx<R>* __restrict__ st, u64 ngroups, u64 tmask, Ins ins, M1<R> g) {
+  const u64 stride = (u64)gridDim.x * (kBlock * kIlp);
+  for (u64 b = (u64)blockIdx.x * (kBlock * kIlp) + threadIdx.x; b < ngroups; b += stride) {
+    u64 i0[kIlp];
+    bool ok[kIlp];
+    cvec<R, V> a[kIlp], c[kIlp];
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      const u64 gv = b + (u64)u * kBlock;
+      ok[u] = gv < ngroups;
+      i0[u] = ok[u] ? insert_bits(gv * V, ins) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      if (!ok[u]) continue;
+      if (KIND != QJ_APPLY_Z && KIND != QJ_APPLY_Z_POW) a[u] = ldv<R, V>(st + i0[u]);
+      c[u] = ldv<R, V>(st + i0[u] + tmask);
+    }
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      if (!ok[u]) continue;
+      cvec<R, V> na, nc;
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        const cplx<R> x = a[u].v[e], y = c[u].v[e];
+        if (KIND == QJ_APPLY_GATE) {
+          na.v[e] = mac2(g.m[0], x, g.m[1], y);
+          nc.v[e] = mac2(g.m[2], x, g.m[3], y);
+        } else if (KIND == QJ_APPLY_X) {
+          na.v[e] = y;
+          nc.v[e] = x;
+        } else if (KIND == QJ_APPLY_Y) {
+          na.v[e] = mk(y.im, -y.re);   // -i * y
+          nc.v[e] = mk(-x.im, x.re);   //  i * x
+        } else if (KIND == QJ_APPLY_Z) {
+          nc.v[e] = mk(-y.re, -y.im);
+        } else {  // QJ_APPLY_Z_POW
+          nc.v[e] = cmul(g.m[0], y);
+        }
+      }
+      if (KIND != QJ_APPLY_Z && KIND != QJ_APPLY_Z_POW) stv<R, V>(st + i0[u], na);
+      stv<R, V>(st + i0[u] + tmask, nc);
+    }
+  }
+}
+
+// --------------------------------------------------------------- 2 qubits --
+// Amplitude r of a quad sits at base + (r>>1)*q1 + (r&1)*q2, q1/q2 = the
+// strides of the first/second listed target (matrix index 2*t1 + t2).
+template <typename R, int KIND, int V>
+__global__ void __launch_bounds__(kBlock)
+    k_two_qubit(cplx<R>* __restrict__ st, u64 ngroups, u64 q1, u64 q2, Ins ins, M2<R> g) {
+  const u64 stride = (u64)gridDim.x * (kBlock * kIlp);
+  for (u64 b = (u64)blockIdx.x * (kBlock * kIlp) + threadIdx.x; b < ngroups; b += stride) {
+    u64 i0[kIlp];
+    bool ok[kIlp];
+    cvec<R, V> a0[kIlp], a1[kIlp], a2[kIlp], a3[kIlp];
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      const u64 gv = b + (u64)u * kBlock;
+      ok[u] = gv < ngroups;
+      i0[u] = ok[u] ? insert_bits(gv * V, ins) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      if (!ok[u]) continue;
+      if (KIND == QJ_APPLY_TWO_QUBIT_GATE) a0[u] = ldv<R, V>(st + i0[u]);
+      a1[u] = ldv<R, V>(st + i0[u] + q2);
+      a2[u] = ldv<R, V>(st + i0[u] + q1);
+      if (KIND != QJ_APPLY_SWAP) a3[u] = ldv<R, V>(st + i0[u] + q1 + q2);
+    }
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      if (!ok[u]) continue;
+      cvec<R, V> n0, n1, n2, n3;
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        if (KIND == QJ_APPLY_TWO_QUBIT_GATE) {
+          const cplx<R> x0 = a0[u].v[e], x1 = a1[u].v[e], x2 = a2[u].v[e], x3 = a3[u].v[e];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            cplx<R> acc = cadd(cadd(cmul(g.m[4 * r + 0], x0), cmul(g.m[4 * r + 1], x1)),
+                               cadd(cmul(g.m[4 * r + 2], x2), cmul(g.m[4 * r + 3], x3)));
+            if (r == 0) n0.v[e] = acc;
+            if (r == 1) n1.v[e] = acc;
+            if (r == 2) n2.v[e] = acc;
+            if (r == 3) n3.v[e] = acc;
+          }
+        } else if (KIND == QJ_APPLY_SWAP) {
+          n1.v[e] = a2[u].v[e];
+          n2.v[e] = a1[u].v[e];
+        } else {  // QJ_APPLY_FSIM
+          const cplx<R> x1 = a1[u].v[e], x2 = a2[u].v[e];
+          n1.v[e] = mac2(g.m[0], x1, g.m[1], x2);
+          n2.v[e] = mac2(g.m[2], x1, g.m[3], x2);
+          n3.v[e] = cmul(g.m[4], a3[u].v[e]);
+        }
+      }
+      if (KIND == QJ_APPLY_TWO_QUBIT_GATE) stv<R, V>(st + i0[u], n0);
+      stv<R, V>(st + i0[u] + q2, n1);
+      stv<R, V>(st + i0[u] + q1, n2);
+      if (KIND != QJ_APPLY_SWAP) stv<R, V>(st + i0[u] + q1 + q2, n3);
+    }
+  }
+}
+
+// ------------------------------------------------------ 3..5 qubits (regs) --
+// The 2^K amplitudes of a group are held in registers; the matrix is staged
+// once per workgroup in LDS and read as wave-uniform broadcasts.
+template <int K>
+struct Offs {
+  u64 o[1 << K];
+};
+
+template <typename R, int K>
+__global__ void __launch_bounds__(kBlock)
+    k_multi_reg(cplx<R>* __restrict__ st, u64 ngroups, Ins ins, Offs<K> off,
+                const cplx<R>* __restrict__ mat) {
+  constexpr int N = 1 << K;
+  __shared__ cplx<R> sm[N * N];
+  for (int t = threadIdx.x; t < N * N; t += kBlock) sm[t] = mat[t];
+  __syncthreads();
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 g = (u64)blockIdx.x * kBlock + threadIdx.x; g < ngroups; g += stride) {
+    const u64 base = insert_bits(g, ins);
+    cplx<R> x[N];
+#pragma unroll
+    for (int c = 0; c < N; ++c) x[c] = st[base + off.o[c]];
+#pragma unroll 1
+    for (int r = 0; r < N; ++r) {
+      cplx<R> acc = cmul(sm[r * N], x[0]);
+#pragma unroll
+      for (int c = 1; c < N; ++c) acc = cadd(acc, cmul(sm[r * N + c], x[c]));
+      st[base + off.o[r]] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------- any k (LDS-staged) --
+// GP groups per iteration are gathered into LDS, then each thread produces
+// output rows; used for k > 5 (and as the reference path for any k).
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_multi_lds(cplx<R>* __restrict__ st, u64 ngroups, Ins ins, int K,
+                const u64* __restrict__ offs, const cplx<R>* __restrict__ mat) {
+  extern __shared__ char smem_raw[];
+  cplx<R>* xs = reinterpret_cast<cplx<R>*>(smem_raw);
+  const int N = 1 << K;
+  const int GP = N >= kBlock ? 1 : kBlock / N;
+  for (u64 g0 = (u64)blockIdx.x * GP; g0 < ngroups; g0 += (u64)gridDim.x * GP) {
+    for (int t = threadIdx.x; t < GP * N; t += kBlock) {
+      const int gi = t / N, c = t % N;
+      const u64 g = g0 + gi;
+      if (g < ngroups) xs[t] = st[insert_bits(g, ins) + offs[c]];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < GP * N; t += kBlock) {
+      const int gi = t / N, r = t % N;
+      const u64 g = g0 + gi;
+      if (g < ngroups) {
+        const cplx<R>* row = mat + (size_t)r * N;
+        const cplx<R>* xv = xs + gi * N;
+        cplx<R> acc = cmul(row[0], xv[0]);
+        for (int c = 1; c < N; ++c) acc = cadd(acc, cmul(row[c], xv[c]));
+        st[insert_bits(g, ins) + offs[r]] = acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ host side --
+
+// Validates the sorted qubit list and builds the insertion spec: every
+// position gets a 0 inserted, control positions (those not in `targets`) are
+// then set to 1.
+static int build_spec(int nqubits, const int* targets, int ntargets, const int32_t* qubits,
+                      int nq, Ins* ins, const char* who) {
+  if (nqubits < 1 || nqubits > 62)
+    return set_error(QJ_EVALUE, "%s: nqubits=%d out of range", who, nqubits);
+  std::vector<int> pos;
+  if (qubits && nq > 0) {
+    pos.assign(qubits, qubits + nq);
+  } else {
+    pos.assign(targets, targets + ntargets);
+    std::sort(pos.begin(), pos.end());
+  }
+  if ((int)pos.size() > kMaxPos || (int)pos.size() > nqubits)
+    return set_error(QJ_EVALUE, "%s: %zu gate qubits for a %d-qubit vector", who, pos.size(), nqubits);
+  for (size_t j = 0; j < pos.size(); ++j) {
+    if (pos[j] < 0 || pos[j] >= nqubits)
+      return set_error(QJ_EVALUE, "%s: bit position %d out of range [0, %d)", who, pos[j], nqubits);
+    if (j && pos[j] <= pos[j - 1])
+      return set_error(QJ_EVALUE, "%s: qubits tensor must be sorted and unique", who);
+  }
+  for (int t = 0; t < ntargets; ++t) {
+    if (targets[t] < 0 || targets[t] >= nqubits)
+      return set_error(QJ_EVALUE, "%s: target bit %d out of range", who, targets[t]);
+    for (int s = 0; s < t; ++s)
+      if (targets[s] == targets[t]) return set_error(QJ_EVALUE, "%s: repeated target", who);
+    if (std::find(pos.begin(), pos.end(), targets[t]) == pos.end())
+      return set_error(QJ_EVALUE, "%s: target bit %d missing from qubits tensor", who, targets[t]);
+  }
+  ins->npos = (int)pos.size();
+  ins->ones = 0;
+  for (size_t j = 0; j < pos.size(); ++j) {
+    ins->pos[j] = pos[j];
+    bool is_target = false;
+    for (int t = 0; t < ntargets; ++t) is_target |= (targets[t] == pos[j]);
+    if (!is_target) ins->ones |= u64(1) << pos[j];
+  }
+  for (int j = ins->npos; j < kMaxPos; ++j) ins->pos[j] = 0;
+  return QJ_OK;
+}
+
+template <typename R, int KIND, int V>
+static void launch_one(cplx<R>* st, u64 ngroups, u64 tmask, const Ins& ins, const M1<R>& g,
+                       hipStream_t s) {
+  const u64 grid = grid_for(ngroups, (u64)kBlock * kIlp);
+  k_one_qubit<R, KIND, V><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, ngroups, tmask, ins, g);
+}
+
+template <typename R>
+static int one_qubit_impl(void* state, int nqubits, int kind, int m, const int32_t* qubits,
+                          int nq, const void* gate, hipStream_t s) {
+  Ins ins;
+  int rc = build_spec(nqubits, &m, 1, qubits, nq, &ins, "apply_one_qubit");
+  if (rc) return rc;
+  M1<R> g;
+  std::memset(&g, 0, sizeof(g));
+  if (kind == QJ_APPLY_GATE || kind == QJ_APPLY_Z_POW) {
+    if (!gate) return set_error(QJ_EVALUE, "apply_one_qubit: gate matrix required");
+    std::memcpy(g.m, gate, (kind == QJ_APPLY_GATE ? 4 : 1) * sizeof(cplx<R>));
+  } else if (kind != QJ_APPLY_X && kind != QJ_APPLY_Y && kind != QJ_APPLY_Z) {
+    return set_error(QJ_EVALUE, "apply_one_qubit: unknown kind %d", kind);
+  }
+  const u64 npairs = u64(1) << (nqubits - ins.npos);
+  const bool vec = ins.pos[0] >= 1;  // lowest gate bit above bit 0
+  const u64 ngroups = vec ? npairs / 2 : npairs;
+  const u64 tmask = u64(1) << m;
+  cplx<R>* st = static_cast<cplx<R>*>(state);
+#define QJ_ONE(K)                                                  \
+  if (vec) launch_one<R, K, 2>(st, ngroups, tmask, ins, g, s);    \
+  else launch_one<R, K, 1>(st, ngroups, tmask, ins, g, s);
+  switch (kind) {
+    case QJ_APPLY_GATE: QJ_ONE(QJ_APPLY_GATE); break;
+    case QJ_APPLY_X: QJ_ONE(QJ_APPLY_X); break;
+    case QJ_APPLY_Y: QJ_ONE(QJ_APPLY_Y); break;
+    case QJ_APPLY_Z: QJ_ONE(QJ_APPLY_Z); break;
+    default: QJ_ONE(QJ_APPLY_Z_POW); break;
+  }
+#undef QJ_ONE
+  return check_launch("apply_one_qubit launch");
+}
+
+template <typename R, int KIND, int V>
+static void launch_two(cplx<R>* st, u64 ngroups, u64 q1, u64 q2, const Ins& ins, const M2<R>& g,
+                       hipStream_t s) {
+  const u64 grid = grid_for(ngroups, (u64)kBlock * kIlp);
+  k_two_qubit<R, KIND, V><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, ngroups, q1, q2, ins, g);
+}
+
+template <typename R>
+static int two_qubit_impl(void* state, int nqubits, int kind, int bit1, int bit2,
+                          const int32_t* qubits, int nq, const void* gate, hipStream_t s) {
+  int tb[2] = {bit1, bit2};
+  Ins ins;
+  int rc = build_spec(nqubits, tb, 2, qubits, nq, &ins, "apply_two_qubit");
+  if (rc) return rc;
+  M2<R> g;
+  std::memset(&g, 0, sizeof(g));
+  if (kind == QJ_APPLY_TWO_QUBIT_GATE || kind == QJ_APPLY_FSIM) {
+    if (!gate) return set_error(QJ_EVALUE, "apply_two_qubit: gate matrix required");
+    std::memcpy(g.m, gate, (kind == QJ_APPLY_FSIM ? 5 : 16) * sizeof(cplx<R>));
+  } else if (kind != QJ_APPLY_SWAP) {
+    return set_error(QJ_EVALUE, "apply_two_qubit: unknown kind %d", kind);
+  }
+  const u64 nquads = u64(1) << (nqubits - ins.npos);
+  const bool vec = ins.pos[0] >= 1;
+  const u64 ngroups = vec ? nquads / 2 : nquads;
+  const u64 q1 = u64(1) << bit1, q2 = u64(1) << bit2;
+  cplx<R>* st = static_cast<cplx<R>*>(state);
+#define QJ_TWO(K)                                                     \
+  if (vec) launch_two<R, K, 2>(st, ngroups, q1, q2, ins, g, s);      \
+  else launch_two<R, K, 1>(st, ngroups, q1, q2, ins, g, s);
+  switch (kind) {
+    case QJ_APPLY_TWO_QUBIT_GATE: QJ_TWO(QJ_APPLY_TWO_QUBIT_GATE); break;
+    case QJ_APPLY_SWAP: QJ_TWO(QJ_APPLY_SWAP); break;
+    default: QJ_TWO(QJ_APPLY_FSIM); break;
+  }
+#undef QJ_TWO
+  return check_launch("apply_two_qubit launch");
+}
+
+template <typename R, int K>
+static void launch_multi_reg(cplx<R>* st, u64 ngroups, const Ins& ins, const int* tb,
+                             const cplx<R>* dmat, hipStream_t s) {
+  Offs<K> off;
+  for (int r = 0; r < (1 << K); ++r) {
+    u64 o = 0;
+    for (int u = 0; u < K; ++u)
+      if ((r >> (K - 1 - u)) & 1) o |= u64(1) << tb[u];
+    off.o[r] = o;
+  }
+  const u64 grid = grid_for(ngroups, kBlock);
+  k_multi_reg<R, K><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, ngroups, ins, off, dmat);
+}
+
+template <typename R>
+static int multi_qubit_impl(void* state, int nqubits, const int32_t* target_bits, int ntargets,
+                            const int32_t* qubits, int nq, const void* gate, hipStream_t s) {
+  if (ntargets < 1 || ntargets > 12)
+    return set_error(QJ_EVALUE, "Number of target qubits must be <= 12 but is %d.", ntargets);
+  if (!gate) return set_error(QJ_EVALUE, "apply_multi_qubit: gate matrix required");
+  std::vector<int> tb(target_bits, target_bits + ntargets);
+  if (ntargets == 1)
+    return one_qubit_impl<R>(state, nqubits, QJ_APPLY_GATE, tb[0], qubits, nq, gate, s);
+  if (ntargets == 2)
+    return two_qubit_impl<R>(state, nqubits, QJ_APPLY_TWO_QUBIT_GATE, tb[0], tb[1], qubits, nq,
+                             gate, s);
+  Ins ins;
+  int rc = build_spec(nqubits, tb.data(), ntargets, qubits, nq, &ins, "apply_multi_qubit");
+  if (rc) return rc;
+  const int K = ntargets, N = 1 << K;
+  const u64 ngroups = u64(1) << (nqubits - ins.npos);
+  cplx<R>* st = static_cast<cplx<R>*>(state);
+  const size_t mat_bytes = (size_t)N * N * sizeof(cplx<R>);
+  if (K <= 5) {
+    void* dmat = nullptr;
+    int ticket = -1;
+    rc = ring_upload(gate, mat_bytes, s, &dmat, &ticket);
+    if (rc) return rc;
+    const cplx<R>* dm = static_cast<const cplx<R>*>(dmat);
+    if (K == 3) launch_multi_reg<R, 3>(st, ngroups, ins, tb.data(), dm, s);
+    else if (K == 4) launch_multi_reg<R, 4>(st, ngroups, ins, tb.data(), dm, s);
+    else launch_multi_reg<R, 5>(st, ngroups, ins, tb.data(), dm, s);
+    rc = check_launch("apply_multi_qubit launch");
+    ring_release(ticket, s);
+    return rc;
+  }
+  // k > 5: matrix followed by the 2^k offsets in one upload.
+  std::vector<char> buf(mat_bytes + (size_t)N * sizeof(u64));
+  std::memcpy(buf.data(), gate, mat_bytes);
+  u64* offs = reinterpret_cast<u64*>(buf.data() + mat_bytes);
+  for (int r = 0; r < N; ++r) {
+    u64 o = 0;
+    for (int u = 0; u < K; ++u)
+      if ((r >> (K - 1 - u)) & 1) o |= u64(1) << tb[u];
+    offs[r] = o;
+  }
+  void* dbuf = nullptr;
+  int ticket = -1;
+  rc = ring_upload(buf.data(), buf.size(), s, &dbuf, &ticket);
+  if (rc) return rc;
+  const cplx<R>* dm = static_cast<const cplx<R>*>(dbuf);
+  const u64* doffs = reinterpret_cast<const u64*>(static_cast<char*>(dbuf) + mat_bytes);
+  const int GP = N >= kBlock ? 1 : kBlock / N;
+  const size_t lds = (size_t)GP * N * sizeof(cplx<R>);
+  const u64 grid = grid_for(ngroups, GP);
+  k_multi_lds<R><<<dim3((unsigned)grid), dim3(kBlock), lds, s>>>(st, ngroups, ins, K, doffs, dm);
+  rc = check_launch("apply_multi_qubit launch");
+  ring_release(ticket, s);
+  return rc;
+}
+
+}  // namespace qj
+
+using namespace qj;
+
+extern "C" {
+
+int qj_apply_one_qubit(void* state, int dtype, int nqubits, int kind, int m,
+                       const int32_t* qubits, int nqubits_sorted, const void* gate, void* stream) {
+  if (!state) return set_error(QJ_EVALUE, "apply_one_qubit: null state");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == QJ_COMPLEX64)
+    return one_qubit_impl<float>(state, nqubits, kind, m, qubits, nqubits_sorted, gate, s);
+  if (dtype == QJ_COMPLEX128)
+    return one_qubit_impl<double>(state, nqubits, kind, m, qubits, nqubits_sorted, gate, s);
+  return set_error(QJ_EVALUE, "apply_one_qubit: state dtype must be complex64/complex128");
+}
+
+int qj_apply_two_qubit(void* state, int dtype, int nqubits, int kind, int bit1, int bit2,
+                       const int32_t* qubits, int nqubits_sorted, const void* gate, void* stream) {
+  if (!state) return set_error(QJ_EVALUE, "apply_two_qubit: null state");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == QJ_COMPLEX64)
+    return two_qubit_impl<float>(state, nqubits, kind, bit1, bit2, qubits, nqubits_sorted, gate, s);
+  if (dtype == QJ_COMPLEX128)
+    return two_qubit_impl<double>(state, nqubits, kind, bit1, bit2, qubits, nqubits_sorted, gate, s);
+  return set_error(QJ_EVALUE, "apply_two_qubit: state dtype must be complex64/complex128");
+}
+
+int qj_apply_multi_qubit(void* state, int dtype, int nqubits, const int32_t* target_bits,
+                         int ntargets, const int32_t* qubits, int nqubits_sorted, const void* gate,
+                         void* stream) {
+  if (!state) return set_error(QJ_EVALUE, "apply_multi_qubit: null state");
+  if (!target_bits) return set_error(QJ_EVALUE, "apply_multi_qubit: null targets");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == QJ_COMPLEX64)
+    return multi_qubit_impl<float>(state, nqubits, target_bits, ntargets, qubits, nqubits_sorted,
+                                   gate, s);
+  if (dtype == QJ_COMPLEX128)
+    return multi_qubit_impl<double>(state, nqubits, target_bits, ntargets, qubits, nqubits_sorted,
+                                    gate, s);
+  return set_error(QJ_EVALUE, "apply_multi_qubit: state dtype must be complex64/complex128");
+}
+
+}  // extern "C"

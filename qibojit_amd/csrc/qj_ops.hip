@@ -1,0 +1,763 @@
+// State preparation, collapse, marginal probabilities, Metropolis sampling
+// and FP64 reductions for gfx950, with their C-ABI entry points.
+//
+// Reductions are deterministic: each workgroup reduces in a fixed order
+// (wave64 butterfly, then the 4 waves) into a partials array, and a second
+// kernel sums the partials in a fixed order.  Nothing is accumulated with
+// floating-point atomics, so repeated calls give bit-identical results.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "qj_common.h"
+
+namespace qj {
+
+constexpr int kRedBlocks = 1024;  // workgroups of a grid-stride reduction
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum of v over the workgroup, valid on thread 0.
+__device__ __forceinline__ double block_sum(double v, double* wsum) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) wsum[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBlock / 64; ++i) t += wsum[i];
+  __syncthreads();
+  return t;
+}
+
+template <typename R>
+__device__ __forceinline__ double abs2(cplx<R> x) {
+  const double re = (double)x.re, im = (double)x.im;
+  return re * re + im * im;
+}
+
+// out[v] = (T) sum_b partials[v*sv + b*sb], b < np; one workgroup per value.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_finalize(const double* __restrict__ partials, u64 np, u64 sv, u64 sb, T* __restrict__ out) {
+  __shared__ double wsum[kBlock / 64];
+  const u64 v = blockIdx.x;
+  double acc = 0.0;
+  for (u64 b = threadIdx.x; b < np; b += kBlock) acc += partials[v * sv + b * sb];
+  acc = block_sum(acc, wsum);
+  if (threadIdx.x == 0) out[v] = (T)acc;
+}
+
+// ---------------------------------------------------------- initial state --
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_initial_state(cplx<R>* __restrict__ st, u64 n) {
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    st[i] = mk<R>(i == 0 ? R(1) : R(0), R(0));
+}
+
+// --------------------------------------------------------------- collapse --
+// Every amplitude i splits into (free bits, measured bits); it is kept iff
+// (i & mask) == keep.  Pass 1 zeroes the rest (write-only) and accumulates
+// |kept|^2; pass 2 divides the kept amplitudes by the norm read from device
+// memory (no host round trip).
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_collapse_zero(cplx<R>* __restrict__ st, u64 n, u64 mask, u64 keep, int want_norm,
+                    double* __restrict__ partials) {
+  __shared__ double wsum[kBlock / 64];
+  double acc = 0.0;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    if ((i & mask) == keep) {
+      if (want_norm) acc += abs2(st[i]);
+    } else {
+      st[i] = mk<R>(R(0), R(0));
+    }
+  }
+  acc = block_sum(acc, wsum);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_collapse_scale(cplx<R>* __restrict__ st, u64 nkept, Ins ins, const double* __restrict__ norm2) {
+  const double norm = sqrt(*norm2);
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 g = (u64)blockIdx.x * kBlock + threadIdx.x; g < nkept; g += stride) {
+    const u64 i = insert_bits(g, ins);
+    const cplx<R> x = st[i];
+    st[i] = mk<R>((R)((double)x.re / norm), (R)((double)x.im / norm));
+  }
+}
+
+// ---------------------------------------------------------- probabilities --
+// Few outcomes (2^k <= 32): each thread keeps the 2^k partial sums in
+// registers while it walks the free index; workgroup partials are then
+// summed by k_finalize.
+template <typename R, int NB>
+__global__ void __launch_bounds__(kBlock)
+    k_probs_bins(const cplx<R>* __restrict__ st, u64 nfree, Ins fins, Offs32 hoff,
+                 double* __restrict__ partials) {
+  __shared__ double wsum[kBlock / 64];
+  double acc[NB];
+#pragma unroll
+  for (int h = 0; h < NB; ++h) acc[h] = 0.0;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 g = (u64)blockIdx.x * kBlock + threadIdx.x; g < nfree; g += stride) {
+    const u64 base = insert_bits(g, fins);
+#pragma unroll
+    for (int h = 0; h < NB; ++h) acc[h] += abs2(st[base + hoff.o[h]]);
+  }
+#pragma unroll
+  for (int h = 0; h < NB; ++h) {
+    const double t = block_sum(acc[h], wsum);
+    if (threadIdx.x == 0) partials[(u64)h * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// Many outcomes: one work item per (chunk of the free index, outcome h).
+template <typename R, typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_probs_wide(const cplx<R>* __restrict__ st, u64 nbins, u64 nfree, u64 nchunks, Ins fins,
+                 Ins obits, double* __restrict__ partials, T* __restrict__ out) {
+  const u64 total = nbins * nchunks;
+  const u64 per = (nfree + nchunks - 1) / nchunks;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 w = (u64)blockIdx.x * kBlock + threadIdx.x; w < total; w += stride) {
+    const u64 h = w % nbins, c = w / nbins;
+    // output bit (k-1-j) of h is the value of bit position obits.pos[j]
+    u64 off = 0;
+    for (int j = 0; j < obits.npos; ++j)
+      if ((h >> (obits.npos - 1 - j)) & 1) off |= u64(1) << obits.pos[j];
+    const u64 g0 = c * per, g1 = (g0 + per < nfree) ? g0 + per : nfree;
+    double acc = 0.0;
+    for (u64 g = g0; g < g1; ++g) acc += abs2(st[insert_bits(g, fins) | off]);
+    if (nchunks == 1) out[h] = (T)acc;
+    else partials[c * nbins + h] = acc;
+  }
+}
+
+// ------------------------------------------------------ Metropolis sampler --
+// MT19937 with NumPy's legacy RandomState semantics: seed(int) is the
+// Knuth/Matsumoto init_genrand recurrence; outputs are tempered key words.
+constexpr int kMtN = 624, kMtM = 397;
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
+  y ^= (y >> 11);
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= (y >> 18);
+  return y;
+}
+__device__ __forceinline__ uint32_t mt_mix(uint32_t a, uint32_t b, uint32_t c) {
+  const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+  return c ^ (y >> 1) ^ ((0u - (y & 1u)) & 0x9908b0dfu);
+}
+
+// Regenerates key[0..623] with the whole workgroup.  Each index range reads
+// old values into registers before the barrier that precedes its writes;
+// the ranges are ordered by their dependencies: [0,227) uses old words,
+// [227,454) uses [0,227), [454,623) uses [227,396), 623 uses key[0], key[396].
+__device__ void mt_twist_block(uint32_t* key) {
+  const int t = threadIdx.x;
+  uint32_t v = 0;
+  // [0, 227)
+  for (int i0 = 0; i0 < kMtN - kMtM; i0 += kBlock) {
+    const int i = i0 + t;
+    if (i < kMtN - kMtM) v = mt_mix(key[i], key[i + 1], key[i + kMtM]);
+    __syncthreads();
+    if (i < kMtN - kMtM) key[i] = v;
+    __syncthreads();
+  }
+  // [227, 454) then [454, 623)
+  for (int lo = kMtN - kMtM; lo < kMtN - 1; lo += kMtN - kMtM) {
+    const int hi = min(lo + (kMtN - kMtM), kMtN - 1);
+    for (int i0 = lo; i0 < hi; i0 += kBlock) {
+      const int i = i0 + t;
+      if (i < hi) v = mt_mix(key[i], key[i + 1], key[i - (kMtN - kMtM)]);
+      __syncthreads();
+      if (i < hi) key[i] = v;
+      __syncthreads();
+    }
+  }
+  if (t == 0) key[kMtN - 1] = mt_mix(key[kMtN - 1], key[0], key[kMtM - 1]);
+  __syncthreads();
+}
+
+__device__ void mt_seed_block(uint32_t* key, uint32_t seed) {
+  if (threadIdx.x == 0) {
+    uint32_t s = seed;
+    for (int i = 0; i < kMtN; ++i) {
+      key[i] = s;
+      s = 1812433253u * (s ^ (s >> 30)) + (uint32_t)(i + 1);
+    }
+  }
+  __syncthreads();
+}
+
+// random_sample(): 53-bit double from two consecutive words.
+__device__ __forceinline__ double mt_double(uint32_t w1, uint32_t w2) {
+  const double a = (double)(w1 >> 5), b = (double)(w2 >> 6);
+  return (a * 67108864.0 + b) / 9007199254740992.0;
+}
+
+// One Metropolis step: propose (x + off) mod N, accept when the probability
+// ratio, computed in the probs type, exceeds u (compared in double).
+template <typename T>
+__device__ __forceinline__ int mh_step(int x, uint32_t off, double u, const T* p, int nmask) {
+  const int y = (int)((x + off) & (uint32_t)nmask);
+  const T ratio = p[y] / p[x];
+  return ((double)ratio > u) ? y : x;
+}
+
+constexpr int kSegSteps = 65536;  // steps per segment of a chain
+constexpr int kMaxScanStates = 64;
+
+// Power-of-two N <= 64: one workgroup per chain.  Per segment: (1) the
+// workgroup generates the chain's words (3 per step: randint then the two
+// halves of random()); (2) thread t composes the transition functions of its
+// run of steps into a table T_t; (3) thread 0 chains the tables to find each
+// run's start state; (4) every thread replays its run from that state and
+// counts visits.  The visited sequence is identical to the sequential chain.
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock)
+    k_metropolis_scan(int64_t* __restrict__ freqs, const T* __restrict__ probs,
+                      const uint32_t* __restrict__ seeds, const int64_t* __restrict__ chain_shots,
+                      uint32_t* __restrict__ words_all, int start_state) {
+  __shared__ uint32_t key[kMtN];
+  __shared__ T p[N];
+  __shared__ unsigned char tables[kBlock][N];
+  __shared__ int run_start[kBlock];
+  __shared__ unsigned int counts[N];
+  __shared__ int carried;
+  const int chain = blockIdx.x;
+  uint32_t* words = words_all + (size_t)chain * (3 * (size_t)kSegSteps);
+  for (int i = threadIdx.x; i < N; i += kBlock) {
+    p[i] = probs[i];
+    counts[i] = 0;
+  }
+  mt_seed_block(key, seeds[chain]);
+  int pos = kMtN;  // words of the current key batch already used
+  int state = start_state;
+  const int64_t total = chain_shots[chain];
+  for (int64_t done = 0; done < total; done += kSegSteps) {
+    const int steps = (int)((total - done < kSegSteps) ? (total - done) : kSegSteps);
+    const int need = 3 * steps;
+    // (1) words
+    for (int filled = 0; filled < need;) {
+      if (pos == kMtN) {
+        mt_twist_block(key);
+        pos = 0;
+      }
+      const int take = min(kMtN - pos, need - filled);
+      for (int i = threadIdx.x; i < take; i += kBlock) words[filled + i] = mt_temper(key[pos + i]);
+      filled += take;
+      pos += take;
+    }
+    __syncthreads();
+    __threadfence_block();
+    // (2) per-thread composition over a run of steps
+    const int per = (steps + kBlock - 1) / kBlock;
+    const int s0 = min(steps, threadIdx.x * per), s1 = min(steps, s0 + per);
+    int tbl[N];
+#pragma unroll
+    for (int x = 0; x < N; ++x) tbl[x] = x;
+    for (int s = s0; s < s1; ++s) {
+      const uint32_t off = words[3 * s] & (uint32_t)(N - 1);
+      const double u = mt_double(words[3 * s + 1], words[3 * s + 2]);
+#pragma unroll
+      for (int x = 0; x < N; ++x) tbl[x] = mh_step<T>(tbl[x], off, u, p, N - 1);
+    }
+#pragma unroll
+    for (int x = 0; x < N; ++x) tables[threadIdx.x][x] = (unsigned char)tbl[x];
+    __syncthreads();
+    // (3) run start states
+    if (threadIdx.x == 0) {
+      int x = state;
+      for (int t = 0; t < kBlock; ++t) {
+        run_start[t] = x;
+        x = tables[t][x];
+      }
+      carried = x;  // start state of the next segment
+    }
+    __syncthreads();
+    // (4) replay and count
+    int x = run_start[threadIdx.x];
+    for (int s = s0; s < s1; ++s) {
+      const uint32_t off = words[3 * s] & (uint32_t)(N - 1);
+      const double u = mt_double(words[3 * s + 1], words[3 * s + 2]);
+      x = mh_step<T>(x, off, u, p, N - 1);
+      atomicAdd(&counts[x], 1u);
+    }
+    state = carried;
+    __syncthreads();  // words/tables are rewritten by the next segment
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < N; i += kBlock)
+    if (counts[i]) atomicAdd((unsigned long long*)&freqs[i], (unsigned long long)counts[i]);
+}
+
+// General N (any size, rejection possible): one thread per chain walks the
+// chain exactly as the reference loop does, drawing words on demand.
+template <typename T>
+__global__ void __launch_bounds__(64)
+    k_metropolis_seq(int64_t* __restrict__ freqs, const T* __restrict__ probs, int64_t nstates,
+                     const uint32_t* __restrict__ seeds, const int64_t* __restrict__ chain_shots,
+                     int64_t start_state) {
+  __shared__ uint32_t key[kMtN];
+  if (threadIdx.x != 0) return;
+  const int chain = blockIdx.x;
+  {
+    uint32_t s = seeds[chain];
+    for (int i = 0; i < kMtN; ++i) {
+      key[i] = s;
+      s = 1812433253u * (s ^ (s >> 30)) + (uint32_t)(i + 1);
+    }
+  }
+  int pos = kMtN;
+  auto next32 = [&]() -> uint32_t {
+    if (pos == kMtN) {
+      for (int i = 0; i < kMtN - kMtM; ++i) key[i] = mt_mix(key[i], key[i + 1], key[i + kMtM]);
+      for (int i = kMtN - kMtM; i < kMtN - 1; ++i)
+        key[i] = mt_mix(key[i], key[i + 1], key[i - (kMtN - kMtM)]);
+      key[kMtN - 1] = mt_mix(key[kMtN - 1], key[0], key[kMtM - 1]);
+      pos = 0;
+    }
+    return mt_temper(key[pos++]);
+  };
+  const uint64_t rng = (uint64_t)nstates - 1;
+  uint32_t mask = (uint32_t)rng;
+  mask |= mask >> 1; mask |= mask >> 2; mask |= mask >> 4; mask |= mask >> 8; mask |= mask >> 16;
+  int64_t x = start_state;
+  const int64_t total = chain_shots[chain];
+  for (int64_t i = 0; i < total; ++i) {
+    uint64_t off = 0;
+    if (rng != 0) {
+      uint32_t v;
+      if (rng == 0xFFFFFFFFull) v = next32();
+      else do { v = next32() & mask; } while (v > rng);
+      off = v;
+    }
+    const uint32_t w1 = next32(), w2 = next32();
+    const double u = mt_double(w1, w2);
+    const int64_t y = (int64_t)(((uint64_t)x + off) % (uint64_t)nstates);
+    const T ratio = probs[y] / probs[x];
+    if ((double)ratio > u) x = y;
+    atomicAdd((unsigned long long*)&freqs[x], 1ull);
+  }
+}
+
+// ------------------------------------------------------- FP64 reductions --
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_norm2(const cplx<R>* __restrict__ st, u64 n, double* __restrict__ partials) {
+  __shared__ double wsum[kBlock / 64];
+  double acc = 0.0;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) acc += abs2(st[i]);
+  acc = block_sum(acc, wsum);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_vdot(const cplx<R>* __restrict__ a, const cplx<R>* __restrict__ b, u64 n,
+           double* __restrict__ partials) {
+  __shared__ double wsum[kBlock / 64];
+  double re = 0.0, im = 0.0;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const cplx<R> x = a[i], y = b[i];
+    const double xr = x.re, xi = x.im, yr = y.re, yi = y.im;
+    re += xr * yr + xi * yi;
+    im += xr * yi - xi * yr;
+  }
+  re = block_sum(re, wsum);
+  im = block_sum(im, wsum);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = re;
+    partials[gridDim.x + blockIdx.x] = im;
+  }
+}
+
+// <psi|H|psi>, H = -sum Z_i Z_{i+1} - g sum X_i on an open chain of n qubits.
+// Diagonal: sum_i z_i z_{i+1} = (n-1) - 2 popcount((k ^ (k >> 1)) & (2^(n-1)-1)).
+// X_b term: sum_k Re(conj(psi_k) psi_{k^2^b}) = 2 sum_{bit b of k = 0} Re(...).
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_tfim(const cplx<R>* __restrict__ st, int nq, double g, double* __restrict__ partials) {
+  __shared__ double wsum[kBlock / 64];
+  const u64 n = u64(1) << nq;
+  const u64 zzmask = (u64(1) << (nq - 1)) - 1;
+  double acc = 0.0;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 k = (u64)blockIdx.x * kBlock + threadIdx.x; k < n; k += stride) {
+    const cplx<R> x = st[k];
+    const int pc = __popcll((k ^ (k >> 1)) & zzmask);
+    const double zz = (double)(nq - 1) - 2.0 * (double)pc;
+    acc -= zz * abs2(x);
+    double xs = 0.0;
+    for (int b = 0; b < nq; ++b) {
+      const u64 bit = u64(1) << b;
+      if (!(k & bit)) {
+        const cplx<R> y = st[k | bit];
+        xs += (double)x.re * (double)y.re + (double)x.im * (double)y.im;
+      }
+    }
+    acc -= 2.0 * g * xs;
+  }
+  acc = block_sum(acc, wsum);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_axpby(const cplx<R>* __restrict__ x, cplx<R>* __restrict__ y, u64 n, cplx<R> a, cplx<R> b) {
+  const u64 stride = (u64)gridDim.x * kBlock;
+  // beta == 0 never reads y, so y may start uninitialised
+  const bool read_y = (b.re != R(0)) || (b.im != R(0));
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    y[i] = read_y ? cadd(cmul(a, x[i]), cmul(b, y[i])) : cmul(a, x[i]);
+}
+
+// ------------------------------------------------------------ host side --
+
+static int reduce_to_host(int nvals, u64 np, u64 sv, u64 sb, double* out, hipStream_t s,
+                          DeviceScratch* sc) {
+  k_finalize<double><<<dim3(nvals), dim3(kBlock), 0, s>>>(sc->partials, np, sv, sb, sc->scalars);
+  int rc = check_launch("finalize launch");
+  if (rc) return rc;
+  rc = hip_status(hipMemcpyAsync(out, sc->scalars, nvals * sizeof(double), hipMemcpyDeviceToHost, s),
+                  "reduction readback");
+  if (rc) return rc;
+  return hip_status(hipStreamSynchronize(s), "reduction sync");
+}
+
+template <typename R>
+static int collapse_impl(void* state, int nqubits, const int32_t* qk, int nm, int64_t result,
+                         int normalize, double* norm_out, hipStream_t s) {
+  if (nqubits < 1 || nqubits > 62) return set_error(QJ_EVALUE, "collapse_state: bad nqubits %d", nqubits);
+  if (nm < 1 || nm > nqubits) return set_error(QJ_EVALUE, "collapse_state: bad number of measured qubits %d", nm);
+  if (result < 0 || (nm < 63 && result >= (int64_t(1) << nm)))
+    return set_error(QJ_EVALUE, "collapse_state: result %lld out of range", (long long)result);
+  for (int j = 0; j < nm; ++j) {
+    if (qk[j] < 0 || qk[j] >= nqubits)
+      return set_error(QJ_EVALUE, "collapse_state: bit %d out of range", qk[j]);
+    for (int i = 0; i < j; ++i)
+      if (qk[i] == qk[j]) return set_error(QJ_EVALUE, "collapse_state: repeated qubit");
+  }
+  // collapse_index (ops.py:33-40) inserts bit j of h at qk[j], in list
+  // order; the map is linear over GF(2), so its image splits into
+  // index(g, 0) | index(0, h).  mask = index(0, all ones), keep = index(0, result).
+  auto cindex = [&](u64 g, u64 h) {
+    u64 i = g;
+    for (int j = 0; j < nm; ++j) {
+      const int n = qk[j];
+      const u64 k = u64(1) << n;
+      i = ((i >> n) << (n + 1)) + (i & (k - 1)) + ((h >> j) & 1) * k;
+    }
+    return i;
+  };
+  const u64 mask = cindex(0, (nm >= 64) ? ~u64(0) : ((u64(1) << nm) - 1));
+  const u64 keep = cindex(0, (u64)result);
+  DeviceScratch* sc = nullptr;
+  int rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  const u64 n = u64(1) << nqubits;
+  cplx<R>* st = static_cast<cplx<R>*>(state);
+  const u64 grid = std::min<u64>(kRedBlocks, grid_for(n, kBlock));
+  k_collapse_zero<R><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, n, mask, keep, normalize || norm_out, sc->partials);
+  rc = check_launch("collapse launch");
+  if (rc) return rc;
+  if (!normalize && !norm_out) return QJ_OK;
+  k_finalize<double><<<dim3(1), dim3(kBlock), 0, s>>>(sc->partials, grid, 0, 1, sc->scalars);
+  rc = check_launch("collapse finalize launch");
+  if (rc) return rc;
+  if (normalize) {
+    Ins ins;
+    ins.npos = 0;
+    for (int b = 0; b < nqubits; ++b)
+      if ((mask >> b) & 1) ins.pos[ins.npos++] = b;
+    for (int j = ins.npos; j < kMaxPos; ++j) ins.pos[j] = 0;
+    ins.ones = keep;
+    const u64 nkept = n >> nm;
+    k_collapse_scale<R><<<dim3((unsigned)grid_for(nkept, kBlock)), dim3(kBlock), 0, s>>>(st, nkept, ins, sc->scalars);
+    rc = check_launch("collapse scale launch");
+    if (rc) return rc;
+  }
+  if (norm_out) {
+    double n2 = 0;
+    rc = hip_status(hipMemcpyAsync(&n2, sc->scalars, sizeof(double), hipMemcpyDeviceToHost, s), "norm readback");
+    if (rc) return rc;
+    rc = hip_status(hipStreamSynchronize(s), "collapse sync");
+    if (rc) return rc;
+    *norm_out = std::sqrt(n2);
+  }
+  return QJ_OK;
+}
+
+template <typename R, typename T>
+static int probs_impl(const void* state, int nqubits, const int32_t* qubits, int nm, T* out,
+                      hipStream_t s) {
+  if (nqubits < 1 || nqubits > 62) return set_error(QJ_EVALUE, "calculate_probabilities: bad nqubits");
+  if (nm < 1 || nm > nqubits || nm > kMaxPos)
+    return set_error(QJ_EVALUE, "calculate_probabilities: bad number of qubits %d", nm);
+  Ins obits;  // bit positions in output order (first qubit = MSB)
+  obits.npos = nm;
+  std::vector<int> sorted;
+  for (int j = 0; j < nm; ++j) {
+    if (qubits[j] < 0 || qubits[j] >= nqubits)
+      return set_error(QJ_EVALUE, "calculate_probabilities: qubit %d out of range", qubits[j]);
+    obits.pos[j] = nqubits - 1 - qubits[j];
+    sorted.push_back(obits.pos[j]);
+  }
+  for (int j = nm; j < kMaxPos; ++j) obits.pos[j] = 0;
+  obits.ones = 0;
+  std::sort(sorted.begin(), sorted.end());
+  for (size_t j = 1; j < sorted.size(); ++j)
+    if (sorted[j] == sorted[j - 1]) return set_error(QJ_EVALUE, "calculate_probabilities: repeated qubit");
+  Ins fins;
+  fins.npos = nm;
+  for (int j = 0; j < nm; ++j) fins.pos[j] = sorted[j];
+  for (int j = nm; j < kMaxPos; ++j) fins.pos[j] = 0;
+  fins.ones = 0;
+  DeviceScratch* sc = nullptr;
+  int rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  const cplx<R>* st = static_cast<const cplx<R>*>(state);
+  const u64 nfree = u64(1) << (nqubits - nm);
+  const u64 nbins = u64(1) << nm;
+  if (nm <= 5) {
+    Offs32 hoff;
+    for (u64 h = 0; h < 32; ++h) {
+      u64 o = 0;
+      if (h < nbins)
+        for (int j = 0; j < nm; ++j)
+          if ((h >> (nm - 1 - j)) & 1) o |= u64(1) << obits.pos[j];
+      hoff.o[h] = o;
+    }
+    const u64 grid = std::min<u64>(kRedBlocks, grid_for(nfree, kBlock));
+    dim3 gd((unsigned)grid), bd(kBlock);
+    switch (nm) {
+      case 1: k_probs_bins<R, 2><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, sc->partials); break;
+      case 2: k_probs_bins<R, 4><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, sc->partials); break;
+      case 3: k_probs_bins<R, 8><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, sc->partials); break;
+      case 4: k_probs_bins<R, 16><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, sc->partials); break;
+      default: k_probs_bins<R, 32><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, sc->partials); break;
+    }
+    rc = check_launch("probabilities launch");
+    if (rc) return rc;
+    k_finalize<T><<<dim3((unsigned)nbins), dim3(kBlock), 0, s>>>(sc->partials, grid, grid, 1, out);
+    return check_launch("probabilities finalize launch");
+  }
+  const u64 target_items = u64(1) << 20;
+  u64 nchunks = 1;
+  if (nbins < target_items) nchunks = std::min<u64>(nfree, target_items / nbins);
+  if (nchunks < 1) nchunks = 1;
+  const u64 items = nbins * nchunks;
+  k_probs_wide<R, T><<<dim3((unsigned)grid_for(items, kBlock)), dim3(kBlock), 0, s>>>(
+      st, nbins, nfree, nchunks, fins, obits, sc->partials, out);
+  rc = check_launch("probabilities launch");
+  if (rc || nchunks == 1) return rc;
+  k_finalize<T><<<dim3((unsigned)nbins), dim3(kBlock), 0, s>>>(sc->partials, nchunks, 1, nbins, out);
+  return check_launch("probabilities finalize launch");
+}
+
+template <typename T>
+static int metropolis_impl(int64_t* freqs, const T* probs, int64_t nstates, int64_t nshots,
+                           const uint32_t* seeds, int nchains, hipStream_t s) {
+  if (nstates < 1) return set_error(QJ_EVALUE, "measure_frequencies: empty probabilities");
+  if (nchains < 1) return set_error(QJ_EVALUE, "measure_frequencies: nthreads must be >= 1");
+  if (nshots < 0) return set_error(QJ_EVALUE, "measure_frequencies: negative nshots");
+  if (nstates > (int64_t(1) << 32))
+    return set_error(QJ_EVALUE, "measure_frequencies: more than 2^32 states");
+  // argmax (first maximum) of the probabilities, as probs.argmax() does.
+  std::vector<T> hp(nstates);
+  int rc = hip_status(hipMemcpyAsync(hp.data(), probs, nstates * sizeof(T), hipMemcpyDeviceToHost, s),
+                      "probabilities readback");
+  if (rc) return rc;
+  rc = hip_status(hipStreamSynchronize(s), "metropolis sync");
+  if (rc) return rc;
+  int64_t start = 0;
+  for (int64_t i = 1; i < nstates; ++i)
+    if (hp[i] > hp[start]) start = i;
+  std::vector<int64_t> shots(nchains, nshots / nchains);
+  shots[nchains - 1] += nshots % nchains;
+  // seeds + per-chain shot counts travel in one staging upload
+  std::vector<char> buf(nchains * (sizeof(uint32_t) + sizeof(int64_t)) + 8);
+  std::memcpy(buf.data(), shots.data(), nchains * sizeof(int64_t));
+  std::memcpy(buf.data() + nchains * sizeof(int64_t), seeds, nchains * sizeof(uint32_t));
+  void* dbuf = nullptr;
+  int ticket = -1;
+  rc = ring_upload(buf.data(), buf.size(), s, &dbuf, &ticket);
+  if (rc) return rc;
+  const int64_t* dshots = static_cast<const int64_t*>(dbuf);
+  const uint32_t* dseeds =
+      reinterpret_cast<const uint32_t*>(static_cast<const char*>(dbuf) + nchains * sizeof(int64_t));
+  const bool pow2 = (nstates & (nstates - 1)) == 0;
+  if (pow2 && nstates >= 2 && nstates <= kMaxScanStates) {
+    DeviceScratch* sc = nullptr;
+    rc = scratch_for_current_device(&sc);
+    if (rc) return rc;
+    // words: 3 per step per chain, one segment at a time
+    const size_t need = (size_t)nchains * 3 * kSegSteps * sizeof(uint32_t);
+    rc = scratch_reserve_big(sc, need, s);
+    if (rc) return rc;
+    uint32_t* words = static_cast<uint32_t*>(sc->big);
+    dim3 gd(nchains), bd(kBlock);
+    switch (nstates) {
+      case 2: k_metropolis_scan<T, 2><<<gd, bd, 0, s>>>(freqs, probs, dseeds, dshots, words, (int)start); break;
+      case 4: k_metropolis_scan<T, 4><<<gd, bd, 0, s>>>(freqs, probs, dseeds, dshots, words, (int)start); break;
+      case 8: k_metropolis_scan<T, 8><<<gd, bd, 0, s>>>(freqs, probs, dseeds, dshots, words, (int)start); break;
+      case 16: k_metropolis_scan<T, 16><<<gd, bd, 0, s>>>(freqs, probs, dseeds, dshots, words, (int)start); break;
+      case 32: k_metropolis_scan<T, 32><<<gd, bd, 0, s>>>(freqs, probs, dseeds, dshots, words, (int)start); break;
+      default: k_metropolis_scan<T, 64><<<gd, bd, 0, s>>>(freqs, probs, dseeds, dshots, words, (int)start); break;
+    }
+    rc = check_launch("metropolis launch");
+  } else {
+    k_metropolis_seq<T><<<dim3(nchains), dim3(64), 0, s>>>(freqs, probs, nstates, dseeds, dshots, start);
+    rc = check_launch("metropolis launch");
+  }
+  ring_release(ticket, s);
+  return rc;
+}
+
+}  // namespace qj
+
+using namespace qj;
+
+extern "C" {
+
+int qj_initial_state(void* state, int dtype, uint64_t n, void* stream) {
+  if (!state) return set_error(QJ_EVALUE, "initial_state: null state");
+  hipStream_t s = (hipStream_t)stream;
+  const u64 grid = grid_for(n, kBlock);
+  if (dtype == QJ_COMPLEX64)
+    k_initial_state<float><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((cplx<float>*)state, n);
+  else if (dtype == QJ_COMPLEX128)
+    k_initial_state<double><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((cplx<double>*)state, n);
+  else
+    return set_error(QJ_EVALUE, "initial_state: dtype must be complex64/complex128");
+  return check_launch("initial_state launch");
+}
+
+int qj_collapse_state(void* state, int dtype, int nqubits, const int32_t* qubits_kernel,
+                      int nmeasured, int64_t result, int normalize, double* norm_out,
+                      void* stream) {
+  if (!state || !qubits_kernel) return set_error(QJ_EVALUE, "collapse_state: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == QJ_COMPLEX64)
+    return collapse_impl<float>(state, nqubits, qubits_kernel, nmeasured, result, normalize, norm_out, s);
+  if (dtype == QJ_COMPLEX128)
+    return collapse_impl<double>(state, nqubits, qubits_kernel, nmeasured, result, normalize, norm_out, s);
+  return set_error(QJ_EVALUE, "collapse_state: dtype must be complex64/complex128");
+}
+
+int qj_calculate_probabilities(const void* state, int dtype, int nqubits, const int32_t* qubits,
+                               int nmeasured, void* probs, int prob_dtype, void* stream) {
+  if (!state || !qubits || !probs) return set_error(QJ_EVALUE, "calculate_probabilities: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == QJ_COMPLEX64 && prob_dtype == QJ_FLOAT32)
+    return probs_impl<float, float>(state, nqubits, qubits, nmeasured, (float*)probs, s);
+  if (dtype == QJ_COMPLEX64 && prob_dtype == QJ_FLOAT64)
+    return probs_impl<float, double>(state, nqubits, qubits, nmeasured, (double*)probs, s);
+  if (dtype == QJ_COMPLEX128 && prob_dtype == QJ_FLOAT32)
+    return probs_impl<double, float>(state, nqubits, qubits, nmeasured, (float*)probs, s);
+  if (dtype == QJ_COMPLEX128 && prob_dtype == QJ_FLOAT64)
+    return probs_impl<double, double>(state, nqubits, qubits, nmeasured, (double*)probs, s);
+  return set_error(QJ_EVALUE, "calculate_probabilities: unsupported dtype pair");
+}
+
+int qj_measure_frequencies(int64_t* frequencies, const void* probs, int prob_dtype, int64_t nstates,
+                           int64_t nshots, const uint32_t* chain_seeds, int nchains, void* stream) {
+  if (!frequencies || !probs || !chain_seeds)
+    return set_error(QJ_EVALUE, "measure_frequencies: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (prob_dtype == QJ_FLOAT32)
+    return metropolis_impl<float>(frequencies, (const float*)probs, nstates, nshots, chain_seeds, nchains, s);
+  if (prob_dtype == QJ_FLOAT64)
+    return metropolis_impl<double>(frequencies, (const double*)probs, nstates, nshots, chain_seeds, nchains, s);
+  return set_error(QJ_EVALUE, "measure_frequencies: probs must be float32/float64");
+}
+
+int qj_norm_squared(const void* state, int dtype, uint64_t n, double* out, void* stream) {
+  if (!state || !out) return set_error(QJ_EVALUE, "norm_squared: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  DeviceScratch* sc = nullptr;
+  int rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  const u64 grid = std::min<u64>(kRedBlocks, grid_for(n, kBlock));
+  if (dtype == QJ_COMPLEX64)
+    k_norm2<float><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<float>*)state, n, sc->partials);
+  else if (dtype == QJ_COMPLEX128)
+    k_norm2<double><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<double>*)state, n, sc->partials);
+  else
+    return set_error(QJ_EVALUE, "norm_squared: dtype must be complex");
+  rc = check_launch("norm launch");
+  if (rc) return rc;
+  return reduce_to_host(1, grid, 0, 1, out, s, sc);
+}
+
+int qj_vdot(const void* a, const void* b, int dtype, uint64_t n, double* out, void* stream) {
+  if (!a || !b || !out) return set_error(QJ_EVALUE, "vdot: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  DeviceScratch* sc = nullptr;
+  int rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  const u64 grid = std::min<u64>(kRedBlocks, grid_for(n, kBlock));
+  if (dtype == QJ_COMPLEX64)
+    k_vdot<float><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<float>*)a, (const cplx<float>*)b, n, sc->partials);
+  else if (dtype == QJ_COMPLEX128)
+    k_vdot<double><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<double>*)a, (const cplx<double>*)b, n, sc->partials);
+  else
+    return set_error(QJ_EVALUE, "vdot: dtype must be complex");
+  rc = check_launch("vdot launch");
+  if (rc) return rc;
+  return reduce_to_host(2, grid, grid, 1, out, s, sc);
+}
+
+int qj_expectation_tfim(const void* state, int dtype, int nqubits, double g, double* out,
+                        void* stream) {
+  if (!state || !out) return set_error(QJ_EVALUE, "expectation_tfim: null pointer");
+  if (nqubits < 2 || nqubits > 40) return set_error(QJ_EVALUE, "expectation_tfim: nqubits %d out of range", nqubits);
+  hipStream_t s = (hipStream_t)stream;
+  DeviceScratch* sc = nullptr;
+  int rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  const u64 n = u64(1) << nqubits;
+  const u64 grid = std::min<u64>(kRedBlocks, grid_for(n, kBlock));
+  if (dtype == QJ_COMPLEX64)
+    k_tfim<float><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<float>*)state, nqubits, g, sc->partials);
+  else if (dtype == QJ_COMPLEX128)
+    k_tfim<double><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<double>*)state, nqubits, g, sc->partials);
+  else
+    return set_error(QJ_EVALUE, "expectation_tfim: dtype must be complex");
+  rc = check_launch("tfim launch");
+  if (rc) return rc;
+  return reduce_to_host(1, grid, 0, 1, out, s, sc);
+}
+
+int qj_axpby(const void* x, void* y, int dtype, uint64_t n, const double* alpha, const double* beta,
+             void* stream) {
+  if (!x || !y || !alpha || !beta) return set_error(QJ_EVALUE, "axpby: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const u64 grid = grid_for(n, kBlock);
+  if (dtype == QJ_COMPLEX64)
+    k_axpby<float><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(
+        (const cplx<float>*)x, (cplx<float>*)y, n, mk<float>((float)alpha[0], (float)alpha[1]),
+        mk<float>((float)beta[0], (float)beta[1]));
+  else if (dtype == QJ_COMPLEX128)
+    k_axpby<double><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(
+        (const cplx<double>*)x, (cplx<double>*)y, n, mk<double>(alpha[0], alpha[1]), mk<double>(beta[0], beta[1]));
+  else
+    return set_error(QJ_EVALUE, "axpby: dtype must be complex");
+  return check_launch("axpby launch");
+}
+
+}  // extern "C"

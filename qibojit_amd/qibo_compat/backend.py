@@ -1,0 +1,72 @@
+"""Base backend with the qibo ``Backend`` methods the state-vector path uses.
+
+Only the generic parts live here (dtype bookkeeping, seeding, the shot
+sampler below the Metropolis threshold, tolerance checks); every array
+operation is implemented by the concrete backend.
+"""
+
+from collections import Counter
+
+import numpy as np
+
+
+class Backend:
+    """Subset of qibo.backends.abstract.Backend."""
+
+    def __init__(self):
+        self.name = "backend"
+        self.platform = None
+        self.dtype = "complex128"
+        self.matrices = None
+        self.device = "/CPU:0"
+        self.nthreads = 1
+        self.supports_multigpu = False
+        self.oom_error = MemoryError
+        self.versions = {"qibo": "compat"}
+        self.np = np
+        self.numeric_types = (int, float, complex)
+        self.int8, self.int32, self.int64 = np.int8, np.int32, np.int64
+        self.float32, self.float64 = np.float32, np.float64
+        self.complex64, self.complex128 = np.complex64, np.complex128
+        self._rng = np.random.RandomState()
+
+    def __repr__(self):
+        return f"{self.name} ({self.platform})" if self.platform else self.name
+
+    @property
+    def rtype(self):
+        return "float32" if self.dtype == "complex64" else "float64"
+
+    def set_dtype(self, dtype):
+        dtype = str(np.dtype(dtype))
+        if dtype not in ("complex64", "complex128", "float32", "float64"):
+            raise ValueError(f"Unknown dtype {dtype}.")
+        self.dtype = dtype
+
+    def set_seed(self, seed):
+        self._rng = np.random.RandomState(seed)
+
+    def random_integers(self, low, high=None, size=None, dtype=None):
+        if size is None:
+            size = 1
+        return self._rng.randint(low, high, size=size, dtype=dtype or np.int64)
+
+    def sample_shots(self, probabilities, nshots):
+        probabilities = np.asarray(self.to_numpy(probabilities), dtype=np.float64)
+        probabilities = probabilities / probabilities.sum()
+        return self._rng.choice(range(len(probabilities)), size=nshots, p=probabilities)
+
+    def calculate_frequencies(self, samples):
+        res, counts = np.unique(np.asarray(samples), return_counts=True)
+        return Counter({int(k): int(v) for k, v in zip(res, counts)})
+
+    def sample_frequencies(self, probabilities, nshots):
+        return self.calculate_frequencies(self.sample_shots(probabilities, nshots))
+
+    def assert_allclose(self, value, target, rtol=1e-7, atol=0.0):
+        value = np.asarray(self.to_numpy(value))
+        target = np.asarray(self.to_numpy(target))
+        np.testing.assert_allclose(value, target, rtol=rtol, atol=atol)
+
+    def to_numpy(self, array):
+        return np.asarray(array)

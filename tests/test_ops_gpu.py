@@ -1,0 +1,168 @@
+"""State preparation, collapse, probabilities, Metropolis sampling and FP64
+reductions on the GPU, against the CPU oracle (collapse and sampling
+indices bit-exact)."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import einsum_ref as er
+from oracle import gate_matrices as gm
+from oracle import port
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.mark.parametrize("is_matrix", [False, True])
+def test_zero_state(backend, dtype, is_matrix):
+    """tests/test_ops.py:12-22."""
+    backend.set_dtype(dtype)
+    if is_matrix:
+        got = backend.zero_state(4, density_matrix=True)
+        want = np.reshape(np.array([1] + [0] * 255, dtype=dtype), (16, 16))
+    else:
+        got = backend.zero_state(4)
+        want = np.array([1] + [0] * 15, dtype=dtype)
+    got = backend.to_numpy(got)
+    assert got.dtype == np.dtype(dtype) and got.shape == want.shape
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("n,targets,results", cases.COLLAPSE)
+@pytest.mark.parametrize("normalize", [True, False])
+def test_collapse_state(backend, n, targets, results, normalize, dtype, rng):
+    """tests/test_ops.py:52-83; kept index set bit-exact vs the port."""
+    backend.set_dtype(dtype)
+    psi = gm.random_state(n, rng, dtype)
+    b2d = 2 ** np.arange(len(results) - 1, -1, -1)
+    result = np.array([np.array(results).dot(b2d)])
+    got = backend.to_numpy(backend.collapse_state(backend.cast(psi, dtype=dtype, copy=True), sorted(targets), result, n, normalize))
+    want = port.PortBackend(4).collapse_state(psi.copy(), sorted(targets), int(result[0]), n, normalize)
+    np.testing.assert_array_equal(got == 0, want == 0)
+    np.testing.assert_allclose(got, want, atol=1e-14 if dtype == "complex128" else 1e-7, rtol=0)
+    np.testing.assert_allclose(got, er.collapse(psi, targets, results, normalize), atol=1e-14 if dtype == "complex128" else 1e-7)
+
+
+def test_collapse_unsorted_qubit_order_matches_reference_index(backend, rng):
+    """collapse_index inserts bits in list order (ops.py:33-40), also unsorted."""
+    n = 7
+    psi = gm.random_state(n, rng)
+    for qubits, shot in (([5, 1, 3], 5), ([0, 6], 2), ([4, 2, 6, 0], 9)):
+        got = backend.to_numpy(backend.collapse_state(backend.cast(psi, copy=True), qubits, shot, n, False))
+        want = port.PortBackend(2).collapse_state(psi.copy(), qubits, shot, n, False)
+        np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("qubits", [[0], [3], [1, 4], [4, 1], [5, 0, 2], [0, 1, 2, 3, 4], [2, 0, 5, 1, 4, 3], list(range(9))])
+def test_calculate_probabilities(backend, qubits, dtype, rng):
+    backend.set_dtype(dtype)
+    n = 9
+    psi = gm.random_state(n, rng, dtype)
+    got = backend.to_numpy(backend.calculate_probabilities(backend.cast(psi, dtype=dtype), qubits, n))
+    assert got.dtype == (np.float32 if dtype == "complex64" else np.float64)
+    np.testing.assert_allclose(got, er.probabilities(psi, qubits), atol=1e-6 if dtype == "complex64" else 1e-14)
+
+
+def test_probabilities_deterministic(backend, rng):
+    psi = backend.cast(gm.random_state(22, rng))
+    a = backend.to_numpy(backend.calculate_probabilities(psi, [0, 7, 21], 22))
+    b = backend.to_numpy(backend.calculate_probabilities(psi, [0, 7, 21], 22))
+    np.testing.assert_array_equal(a, b)
+    assert abs(a.sum() - 1) < 1e-12
+
+
+@pytest.mark.parametrize("realtype", ["float32", "float64"])
+@pytest.mark.parametrize("inttype", ["int32", "int64"])
+def test_measure_frequencies_golden(backend, realtype, inttype):
+    """tests/test_ops.py:236-256 golden vector."""
+    with open(os.path.join(GOLDEN, "metropolis_uniform16.json"), encoding="utf-8") as f:
+        gold = json.load(f)
+    probs = np.ones(16, dtype=realtype) / 16
+    freqs = np.zeros(16, dtype=inttype)
+    out = backend.measure_frequencies_op(freqs, probs, nshots=1000, nqubits=4, seed=1234, nthreads=4)
+    assert out.sum() == 1000
+    np.testing.assert_array_equal(out, np.array(gold["frequencies"], dtype=inttype))
+
+
+@pytest.mark.parametrize(
+    "nstates,nthreads,nshots",
+    [(2, 1, 5000), (8, 4, 100000), (8, 16, 1000000), (16, 64, 777777), (64, 3, 200003), (5, 4, 30001), (128, 8, 50000), (1, 2, 100)],
+)
+@pytest.mark.parametrize("realtype", ["float32", "float64"])
+def test_measure_frequencies_bitexact_vs_port(backend, nstates, nthreads, nshots, realtype):
+    rng = np.random.default_rng(nstates + 1000 * nthreads)
+    probs = rng.random(nstates).astype(realtype)
+    if nstates > 2:
+        probs[rng.integers(nstates)] = 0
+    probs = (probs / probs.sum()).astype(realtype)
+    seed = int(rng.integers(0, 10**8))
+    got = backend.measure_frequencies_op(np.zeros(nstates, dtype=np.int64), probs, nshots, 0, seed, nthreads)
+    want = port.measure_frequencies(np.zeros(nstates, dtype=np.int64), probs, nshots, 0, seed, nthreads)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_sample_frequencies_sparse_probabilities(backend):
+    """tests/test_ops.py:259-275 (both sides of the Metropolis threshold)."""
+    import itertools
+
+    nonzero = []
+    for r in (1, 2, 3, 4):
+        nonzero.extend(itertools.combinations(range(8), r))
+    for i, nz in enumerate(nonzero):
+        probs = np.zeros(8)
+        probs[list(nz)] = 1
+        probs = probs / probs.sum()
+        nshots = 200000 if i % 2 else 1000
+        freqs = backend.sample_frequencies(probs, nshots)
+        assert sum(freqs.values()) == nshots
+        for k, v in freqs.items():
+            assert v != 0 and k in nz
+
+
+def test_sample_frequencies_uses_seeded_chains(backend):
+    backend.set_seed(42)
+    probs = np.array([0.1, 0.2, 0.3, 0.4])
+    a = backend.sample_frequencies(probs, 300000)
+    backend.set_seed(42)
+    seed = np.random.RandomState(42).randint(0, int(1e8), size=1)[0]
+    want = port.measure_frequencies(np.zeros(4, dtype=np.int64), probs, 300000, 2, seed, backend.nthreads)
+    assert dict(a) == {i: int(f) for i, f in enumerate(want) if f > 0}
+
+
+def test_reductions(backend, dtype, rng):
+    backend.set_dtype(dtype)
+    n = 12
+    a = gm.random_state(n, rng, dtype)
+    b = gm.random_state(n, rng, dtype)
+    da, db = backend.cast(a, dtype=dtype), backend.cast(b, dtype=dtype)
+    tol = 1e-13 if dtype == "complex128" else 1e-6
+    assert abs(backend.norm_squared(da) - np.vdot(a.astype(complex), a.astype(complex)).real) < tol
+    assert abs(backend.vdot(da, db) - np.vdot(a.astype(complex), b.astype(complex))) < tol
+    for g in (1.0, 0.35):
+        assert abs(backend.expectation_tfim(da, n, g) - er.tfim_energy(a, g)) < tol * n
+
+
+def test_reductions_deterministic(backend, rng):
+    a = backend.cast(gm.random_state(24, rng))
+    assert backend.norm_squared(a) == backend.norm_squared(a)
+    assert backend.expectation_tfim(a, 24) == backend.expectation_tfim(a, 24)
+
+
+def test_collapse_large_state_idempotent(backend):
+    """26-qubit collapse on {0, 7, 25}: normalisation and the collapsed
+    marginal are size-independent checks."""
+    n = 26
+    backend.set_dtype("complex64")
+    rng = np.random.default_rng(0)
+    psi = backend.cast(gm.random_state(n, rng, np.complex64), dtype="complex64")
+    probs = backend.to_numpy(backend.calculate_probabilities(psi, [0, 7, n - 1], n))
+    assert abs(probs.sum() - 1) < 1e-5
+    out = backend.collapse_state(psi, [0, 7, n - 1], 5, n)
+    assert abs(backend.norm_squared(out) - 1) < 1e-5
+    p2 = backend.to_numpy(backend.calculate_probabilities(out, [0, 7, n - 1], n))
+    np.testing.assert_allclose(p2, np.eye(8)[5], atol=1e-5)

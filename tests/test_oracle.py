@@ -1,0 +1,337 @@
+"""Pins the CPU oracle (CPU only, no GPU).
+
+The oracle is checked against the reference's golden vector, NumPy's own
+legacy RNG, Kronecker-product brute force and closed forms, so that a
+dropped term, a swapped index or a transposed operand in it fails here.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import einsum_ref as er
+from oracle import gate_matrices as gm
+from oracle import port
+from qibojit_amd.qibo_compat import gates
+from tests import cases
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ----------------------------------------------------------- Metropolis --
+@pytest.mark.parametrize("realtype", ["float32", "float64"])
+@pytest.mark.parametrize("inttype", ["int32", "int64"])
+def test_port_measure_frequencies_golden(realtype, inttype):
+    """tests/test_ops.py:236-256 of the reference."""
+    with open(os.path.join(GOLDEN, "metropolis_uniform16.json"), encoding="utf-8") as f:
+        gold = json.load(f)
+    probs = np.ones(gold["nstates"], dtype=realtype) / gold["nstates"]
+    freqs = np.zeros(gold["nstates"], dtype=inttype)
+    freqs = port.measure_frequencies(freqs, probs, gold["nshots"], 4, gold["seed"], gold["nthreads"])
+    assert freqs.sum() == gold["nshots"]
+    np.testing.assert_array_equal(freqs, np.array(gold["frequencies"], dtype=inttype))
+
+
+def test_port_mt19937_matches_numpy_legacy():
+    for seed in (0, 1, 1234, 2**32 - 1):
+        rs = np.random.RandomState(seed)
+        ref = np.array([rs.randint(0, 2**32, dtype=np.uint32) for _ in range(1500)], dtype=np.uint32)
+        np.testing.assert_array_equal(port.mt_words(seed, 1500), ref)
+
+
+@pytest.mark.parametrize("high", [1, 2, 3, 8, 13, 16, 1000, 100000000, 2**32])
+def test_port_randint_matches_numpy_legacy(high):
+    rs = np.random.RandomState(99)
+    ref = [rs.randint(0, high) for _ in range(800)]
+    np.testing.assert_array_equal(port.mt_randint(99, 0, high, 800), ref)
+
+
+def test_port_random_matches_numpy_legacy():
+    rs = np.random.RandomState(7)
+    ref = [rs.random() for _ in range(1000)]
+    np.testing.assert_array_equal(port.mt_random(7, 1000), ref)
+
+
+def test_port_chain_seeds():
+    rs = np.random.RandomState(4321)
+    np.testing.assert_array_equal(port.chain_seeds(4321, 6), [rs.randint(0, int(1e8)) for _ in range(6)])
+
+
+def _numpy_measure_frequencies(probs, nshots, seed, nthreads):
+    """ops.py:82-108 written with NumPy's legacy RandomState in Python."""
+    nstates = len(probs)
+    per = [nshots // nthreads] * nthreads
+    per[-1] += nshots % nthreads
+    rs = np.random.RandomState(seed)
+    seeds = [rs.randint(0, int(1e8)) for _ in range(nthreads)]
+    out = np.zeros(nstates, dtype=np.int64)
+    for n in range(nthreads):
+        r = np.random.RandomState(seeds[n])
+        shot = int(np.argmax(probs))
+        for _ in range(per[n]):
+            new = (shot + r.randint(0, nstates)) % nstates
+            if float(probs[new] / probs[shot]) > r.random():
+                shot = new
+            out[shot] += 1
+    return out
+
+
+@pytest.mark.parametrize("nstates,nthreads", [(8, 3), (5, 2), (32, 4)])
+@pytest.mark.parametrize("realtype", ["float32", "float64"])
+def test_port_measure_frequencies_matches_python_chain(nstates, nthreads, realtype):
+    rng = np.random.default_rng(nstates * 31 + nthreads)
+    probs = rng.random(nstates).astype(realtype)
+    probs[rng.integers(nstates)] = 0
+    probs = (probs / probs.sum()).astype(realtype)
+    ref = _numpy_measure_frequencies(probs, 3001, 55, nthreads)
+    got = port.measure_frequencies(np.zeros(nstates, dtype=np.int64), probs, 3001, 0, 55, nthreads)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_port_measure_frequencies_sparse_support():
+    """tests/test_ops.py:259-275: every sampled outcome has p > 0."""
+    import itertools
+
+    nonzero = []
+    for r in (1, 2, 3, 4):
+        nonzero.extend(itertools.combinations(range(8), r))
+    for i, nz in enumerate(nonzero):
+        probs = np.zeros(8)
+        probs[list(nz)] = 1
+        probs /= probs.sum()
+        nshots = 200000 if i % 2 else 1000
+        f = port.measure_frequencies(np.zeros(8, dtype=np.int64), probs, nshots, 3, i, 4)
+        assert f.sum() == nshots
+        assert set(np.nonzero(f)[0]) <= set(nz)
+
+
+# ------------------------------------------------------ gate closed forms --
+def test_einsum_matches_kron_bruteforce():
+    rng = np.random.default_rng(3)
+    for n, targets, controls in [(3, [0], []), (4, [2, 0], [1]), (5, [4, 1, 2], [0]), (6, [1, 5], [0, 3, 4]), (5, [0, 1, 2, 3, 4], [])]:
+        u = gm.haar_unitary(2 ** len(targets), rng)
+        psi = gm.random_state(n, rng)
+        a = er.apply_gate(psi, u, targets, controls, n)
+        b = er.kron_operator(u, targets, controls, n) @ psi
+        np.testing.assert_allclose(a, b, atol=1e-13)
+
+
+def test_planted_transpose_is_caught():
+    """A transposed operand in the oracle would fail the brute-force check."""
+    rng = np.random.default_rng(4)
+    u = gm.haar_unitary(4, rng)
+    psi = gm.random_state(4, rng)
+    good = er.kron_operator(u, [2, 0], [], 4) @ psi
+    assert np.abs(er.apply_gate(psi, u.T, [2, 0], [], 4) - good).max() > 1e-3
+    assert np.abs(er.apply_gate(psi, u, [0, 2], [], 4) - good).max() > 1e-3
+
+
+def test_gate_matrix_identities():
+    t = 0.731
+    np.testing.assert_allclose(gm.U3(t, -np.pi / 2, np.pi / 2), gm.RX(t), atol=1e-14)
+    np.testing.assert_allclose(gm.U3(t, 0, 0), gm.RY(t), atol=1e-14)
+    np.testing.assert_allclose(gm.U1(t), np.exp(0.5j * t) * gm.RZ(t), atol=1e-14)
+    np.testing.assert_allclose(gm.H @ gm.X @ gm.H, gm.Z, atol=1e-14)
+    np.testing.assert_allclose(gm.SX @ gm.SX, gm.X, atol=1e-14)
+    np.testing.assert_allclose(gm.S @ gm.S, gm.Z, atol=1e-14)
+    np.testing.assert_allclose(gm.T @ gm.T, gm.S, atol=1e-14)
+    # fSim(pi/2, 0) swaps |01>,|10> with phase -i
+    f = gm.fSim(np.pi / 2, 0)
+    np.testing.assert_allclose(f[1:3, 1:3], [[0, -1j], [-1j, 0]], atol=1e-14)
+    for m in (gm.U3(0.1, 0.2, 0.3), gm.fSim(0.4, 0.5), gm.DEUTSCH(0.3), gm.haar_unitary(8, np.random.default_rng(1))):
+        np.testing.assert_allclose(m @ m.conj().T, np.eye(m.shape[0]), atol=1e-13)
+
+
+def test_product_matrices_match_oracle():
+    """qibo-convention matrices of the product against the oracle's own."""
+    from qibojit_amd.backends.matrices import CustomMatrices
+    from qibojit_amd.qibo_compat.npmatrices import NumpyMatrices
+
+    nm, cm = NumpyMatrices("complex128"), CustomMatrices("complex128")
+    for name in ("H", "X", "Y", "Z", "S", "SDG", "T", "TDG", "SX", "SXDG"):
+        np.testing.assert_allclose(getattr(nm, name), getattr(gm, name), atol=1e-15)
+    for name in ("RX", "RY", "RZ"):
+        np.testing.assert_allclose(getattr(nm, name)(0.37), getattr(gm, name)(0.37), atol=1e-15)
+    np.testing.assert_allclose(nm.U3(0.1, 0.2, 0.3), gm.U3(0.1, 0.2, 0.3), atol=1e-15)
+    np.testing.assert_allclose(nm.U2(0.2, 0.3), gm.U2(0.2, 0.3), atol=1e-15)
+    np.testing.assert_allclose(nm.U1(0.2), gm.U1(0.2), atol=1e-15)
+    np.testing.assert_allclose(nm.CNOT, gm.controlled(gm.X), atol=1e-15)
+    np.testing.assert_allclose(nm.CU3(0.1, 0.2, 0.3), gm.controlled(gm.U3(0.1, 0.2, 0.3)), atol=1e-15)
+    np.testing.assert_allclose(nm.fSim(0.3, 0.4), gm.fSim(0.3, 0.4), atol=1e-15)
+    np.testing.assert_allclose(nm.TOFFOLI, gm.controlled(gm.X, 2), atol=1e-15)
+    np.testing.assert_allclose(nm.DEUTSCH(0.3), gm.DEUTSCH(0.3), atol=1e-15)
+    # kernel-facing encodings (matrices.py:13-70)
+    np.testing.assert_allclose(cm.U1(0.2), np.exp(0.2j), atol=1e-15)
+    np.testing.assert_allclose(cm.CU3(0.1, 0.2, 0.3), gm.U3(0.1, 0.2, 0.3), atol=1e-15)
+    np.testing.assert_allclose(cm.CZ, gm.Z)
+    np.testing.assert_allclose(cm.DEUTSCH(0.3), 1j * gm.RX(0.6), atol=1e-15)
+    f = cm.fSim(0.3, 0.4)
+    np.testing.assert_allclose(f, port.custom_matrix(gates.fSim(0, 1, 0.3, 0.4)), atol=1e-15)
+
+
+# -------------------------------------------- port kernels vs closed form --
+def _gate_matrix(gate):
+    """Mathematical target-block matrix of a gate for the einsum oracle."""
+    name = gate.__class__.__name__
+    p = gate.parameters
+    table = {"X": gm.X, "Y": gm.Y, "Z": gm.Z, "H": gm.H, "CY": gm.Y, "CH": gm.H, "CSX": gm.SX, "CSXDG": gm.SXDG,
+             "CZ": gm.Z, "CCZ": gm.Z, "CNOT": gm.X, "TOFFOLI": gm.X, "SWAP": gm.SWAP}
+    if name in table:
+        return table[name]
+    if name == "Unitary":
+        return np.asarray(p[0])
+    if name in ("U1", "CU1"):
+        return gm.U1(p[0])
+    if name == "GeneralizedfSim":
+        return gm.GeneralizedfSim(p[0], p[1])
+    if name == "fSim":
+        return gm.fSim(*p)
+    if name == "DEUTSCH":
+        return 1j * gm.RX(2 * p[0])
+    return port.custom_matrix(gate)
+
+
+def _check_port(n, gate, dtype, rng, density_matrix=False):
+    pb = port.PortBackend(nthreads=4)
+    atol = {"complex64": 1e-5, "complex128": 1e-13}[dtype]
+    u = _gate_matrix(gate)
+    if density_matrix:
+        rho = gm.random_density_matrix(n, rng, dtype)
+        got = pb.apply_gate(gate, rho.copy(), n)
+        ref = er.apply_gate_density_matrix(rho, u, gate.target_qubits, gate.control_qubits, n)
+    else:
+        psi = gm.random_state(n, rng, dtype)
+        got = pb.apply_gate(gate, psi.copy(), n)
+        ref = er.apply_gate(psi, u, gate.target_qubits, gate.control_qubits, n)
+    np.testing.assert_allclose(got, ref, atol=atol)
+
+
+@pytest.mark.parametrize("n,target,controls", cases.APPLY_GATE)
+def test_port_apply_gate(n, target, controls, dtype, rng):
+    u = gm.haar_unitary(2, rng)
+    _check_port(n, gates.Unitary(u, target).controlled_by(*controls), dtype, rng)
+
+
+@pytest.mark.parametrize("n,target,controls", cases.PAULI)
+@pytest.mark.parametrize("pauli", ["X", "Y", "Z"])
+def test_port_pauli(n, target, controls, pauli, dtype, rng):
+    _check_port(n, getattr(gates, pauli)(target).controlled_by(*controls), dtype, rng)
+
+
+@pytest.mark.parametrize("n,target,controls", cases.ZPOW)
+def test_port_zpow(n, target, controls, dtype, rng):
+    _check_port(n, gates.U1(target, theta=0.1234).controlled_by(*controls), dtype, rng)
+
+
+@pytest.mark.parametrize("n,targets,controls", cases.TWO_QUBIT)
+@pytest.mark.parametrize("density_matrix", [False, True])
+def test_port_two_qubit(n, targets, controls, density_matrix, dtype, rng):
+    u = gm.haar_unitary(4, rng)
+    _check_port(n, gates.Unitary(u, *targets).controlled_by(*controls), dtype, rng, density_matrix)
+
+
+@pytest.mark.parametrize("n,targets,controls", cases.SWAP)
+def test_port_swap(n, targets, controls, dtype, rng):
+    _check_port(n, gates.SWAP(*targets).controlled_by(*controls), dtype, rng)
+
+
+@pytest.mark.parametrize("n,targets,controls", cases.FSIM)
+def test_port_fsim(n, targets, controls, dtype, rng):
+    u = rng.random((2, 2)) + 1j * rng.random((2, 2))
+    _check_port(n, gates.GeneralizedfSim(*targets, u, cases.FSIM_PHI).controlled_by(*controls), dtype, rng)
+
+
+@pytest.mark.parametrize("n,targets", cases.DEUTSCH)
+def test_port_deutsch(n, targets, dtype, rng):
+    _check_port(n, gates.DEUTSCH(*targets, 0.37), dtype, rng)
+
+
+@pytest.mark.parametrize("n,targets,controls", cases.MULTI_QUBIT)
+@pytest.mark.parametrize("density_matrix", [False, True])
+def test_port_multi_qubit(n, targets, controls, density_matrix, dtype, rng):
+    if density_matrix and n > 5:
+        pytest.skip("dense density-matrix oracle kept to n <= 5 on CPU")
+    k = len(targets)
+    u = rng.random((2**k, 2**k)) + 1j * rng.random((2**k, 2**k))
+    _check_port(n, gates.Unitary(u, *targets).controlled_by(*controls), dtype, rng, density_matrix)
+
+
+@pytest.mark.parametrize("n,target,results", cases.COLLAPSE)
+@pytest.mark.parametrize("normalize", [True, False])
+def test_port_collapse(n, target, results, normalize, dtype, rng):
+    psi = gm.random_state(n, rng, dtype)
+    b2d = 2 ** np.arange(len(results) - 1, -1, -1)
+    shot = int(np.array(results).dot(b2d))
+    got = port.PortBackend(4).collapse_state(psi.copy(), sorted(target), shot, n, normalize)
+    ref = er.collapse(psi, target, results, normalize)
+    atol = 1e-7 if dtype == "complex64" else 1e-14
+    np.testing.assert_allclose(got, ref, atol=atol)
+
+
+def test_probabilities_closed_form():
+    rng = np.random.default_rng(8)
+    psi = gm.random_state(6, rng)
+    for qubits in ([0], [5], [2, 4], [4, 2], [5, 0, 3], list(range(6))):
+        p = er.probabilities(psi, qubits)
+        # brute force: bin of index i is the bits of `qubits` in list order
+        ref = np.zeros(2 ** len(qubits))
+        for i, a in enumerate(psi):
+            b = 0
+            for q in qubits:
+                b = (b << 1) | ((i >> (5 - q)) & 1)
+            ref[b] += abs(a) ** 2
+        np.testing.assert_allclose(p, ref, atol=1e-15)
+
+
+def test_tfim_closed_form_matches_dense():
+    rng = np.random.default_rng(9)
+    for n in (2, 3, 6):
+        psi = gm.random_state(n, rng)
+        h = er.tfim_matrix(n, 0.7)
+        np.testing.assert_allclose(er.tfim_energy(psi, 0.7), np.real(np.vdot(psi, h @ psi)), atol=1e-12)
+
+
+# ------------------------------------------------------------- fusion logic --
+def test_fused_matrix_embedding():
+    from qibojit_amd.fusion import embed
+
+    rng = np.random.default_rng(10)
+    block = [1, 3, 4, 6]
+    for targets, controls in ([[3], [6]], [[6, 1], []], [[4, 3], [1]], [[1, 3, 6], [4]]):
+        u = gm.haar_unitary(2 ** len(targets), rng)
+        got = embed(block, targets, controls, u)
+        # relabel block qubits to 0..3 and use the brute force
+        lbl = {q: i for i, q in enumerate(block)}
+        ref = er.kron_operator(u, [lbl[q] for q in targets], [lbl[q] for q in controls], len(block))
+        np.testing.assert_allclose(got, ref, atol=1e-13)
+
+
+def test_circuit_fuse_preserves_action():
+    from qibojit_amd.fusion import fused_matrix
+    from qibojit_amd.qibo_compat.models import Circuit
+
+    rng = np.random.default_rng(11)
+    n = 6
+    c = Circuit(n)
+    for layer in range(4):
+        for q in range(n):
+            c.add(gates.U3(q, *rng.random(3)))
+        for q in range(layer % 2, n - 1, 2):
+            c.add(gates.CZ(q, q + 1))
+        c.add(gates.CNOT(layer % n, (layer + 3) % n))
+        c.add(gates.fSim(1, 4, 0.3, 0.2))
+    psi0 = gm.random_state(n, rng)
+    ref = port.PortBackend(4).execute_circuit(c, psi0)
+    for width in (2, 3, 5):
+        fused = c.fuse(max_qubits=width)
+        assert len(fused.queue) < len(c.queue)
+        psi = psi0.copy()
+        for g in fused.queue:
+            if g.__class__.__name__ == "FusedGate":
+                assert len(g.target_qubits) <= width
+                psi = er.apply_gate(psi, fused_matrix(port.custom_matrix, g), g.target_qubits, (), n)
+            else:
+                psi = port.PortBackend(4).apply_gate(g, psi, n)
+        np.testing.assert_allclose(psi, ref, atol=1e-12)
