@@ -40,6 +40,7 @@ def parse():
     p.add_argument("--no-extras", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--only-sweep", action="store_true", help="sweep only (for profiler runs)")
+    p.add_argument("--copy-probe", action="store_true", help="plain 4 GiB copy (counter scale reference)")
     return p.parse_args()
 
 
@@ -101,7 +102,7 @@ def kernel_name(backend, gate, n, dtype):
 
     r = "float" if dtype == "complex64" else "double"
     qubits = backend._create_qubits_tensor(gate, n)  # pylint: disable=protected-access
-    v = 2 if int(qubits.min()) >= 1 else 1
+    v = 2 if dtype == "complex64" and int(qubits.min()) >= 1 else 1
     k = len(gate.target_qubits)
     name = gate.__class__.__name__
     if k == 1:
@@ -203,8 +204,9 @@ def dominant_kernel(rows):
 
 
 # ---------------------------------------------------------------- extras --
-def run_qft(backend, n, dtype):
-    """QFT (H + controlled-phase ladder + final swaps), unfused, from a seeded state."""
+def run_qft(backend, n, dtype, fused=False):
+    """QFT (H + controlled-phase ladder + final swaps) from a seeded state,
+    gate by gate or through the fused tile executor."""
     import torch
 
     from qibojit_amd._qibo import Circuit, gates
@@ -223,9 +225,18 @@ def run_qft(backend, n, dtype):
     state /= torch.linalg.vector_norm(state)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    if fused:
+        from qibojit_amd import tiles
+
+        low, max_high = backend.tile_geometry(n, tdt)
+        ops = [tiles.gate_to_op(backend._as_custom_matrix, g, n) for g in c.queue]  # pylint: disable=protected-access
+        npasses = len(tiles.plan(ops, n, low, max_high))
     e0.record()
-    for g in c.queue:
-        state = backend.apply_gate(g, state, n)
+    if fused:
+        state = backend.apply_gates_fused(c.queue, state, n)
+    else:
+        for g in c.queue:
+            state = backend.apply_gate(g, state, n)
     e1.record()
     torch.cuda.synchronize()
     sec = e0.elapsed_time(e1) / 1e3
@@ -245,8 +256,15 @@ def run_qft(backend, n, dtype):
     norm = backend.norm_squared(state)
     del state
     torch.cuda.empty_cache()
-    return {"nqubits": n, "dtype": dtype, "gates": len(c.queue), "seconds": sec,
-            "algorithmic_GB": total / 1e9, "GBps": total / sec / 1e9, "final_norm2": norm}
+    out = {"nqubits": n, "dtype": dtype, "gates": len(c.queue), "seconds": sec, "final_norm2": norm}
+    if fused:
+        # one read + one write of the state per pass
+        passes_bytes = npasses * 2 * amp * 2**n
+        out.update({"fused": True, "passes": npasses, "pass_GB": passes_bytes / 1e9,
+                    "GBps_passes": passes_bytes / sec / 1e9})
+    else:
+        out.update({"fused": False, "algorithmic_GB": total / 1e9, "GBps": total / sec / 1e9})
+    return out
 
 
 def run_measurement_ops(backend, n, dtype):
@@ -360,6 +378,17 @@ def main():
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import torch
 
+    if args.copy_probe:
+        # A plain device copy of exactly 4 GiB each way: the reference for
+        # scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run.
+        a = torch.ones(2**28, dtype=torch.complex128, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": 16 * 2**28, "launches": 3}), flush=True)
+        return
+
     from qibojit_amd.backends.hip import HipBackend
 
     backend = HipBackend()
@@ -376,6 +405,7 @@ def main():
         print("# measurement ops", file=sys.stderr, flush=True)
         extras["measurement_complex128"] = run_measurement_ops(backend, n, "complex128")
         print("# qft 32q complex64", file=sys.stderr, flush=True)
+        extras["qft_32q_complex64_fused"] = run_qft(backend, 32, "complex64", fused=True)
         extras["qft_32q_complex64"] = run_qft(backend, 32, "complex64")
         print("# capacity 33q complex128", file=sys.stderr, flush=True)
         extras["capacity_33q_complex128"] = run_capacity(backend, 33)

@@ -127,11 +127,57 @@ int qj_apply_multi_qubit(void* state, int dtype, int nqubits,
                          const int32_t* qubits, int nqubits_sorted,
                          const void* gate, void* stream);
 
+/* ---- fused passes (LDS tiles) ----
+ * One pass streams the state through LDS in tiles of 2^(low_bits + nhigh)
+ * amplitudes: a tile holds every combination of the `low_bits` lowest bits
+ * (contiguous in memory) and of the `nhigh` bit positions `high_bits`
+ * (ascending, all >= low_bits), for one assignment of the other bits.  The
+ * `nops` operations are applied to each tile in order, in LDS, and the tile
+ * is written back: the whole list costs one read and one write of the state.
+ * This is the gate-fusion executor; it has no single counterpart in the
+ * reference, which applies a fused block (qibo FusedGate via
+ * matrix_fused, cpu.py:535-537) as one dense gate per pass.
+ *
+ * Tile-local bit b means bit b for b < low_bits and high_bits[b - low_bits]
+ * otherwise.  Op kinds:
+ *   QJ_TILE_DENSE: 2^k x 2^k row-major matrix (k <= 5) at `data + offset` on
+ *     the tile-local target bits target[0..k-1] (target[0] = matrix MSB);
+ *   QJ_TILE_PHASE: multiplies amplitude i by exp(i*angle(i)) with
+ *     angle(i) = c0 + sum_j theta_j * bit_j(i), where in-tile bits enter
+ *     through two tables at `data + offset`: {double c0; int32 nout; int32
+ *     pad; then nout x (int32 bit, int32 pad, double theta) for the
+ *     out-of-tile bits; then complex table A[2^la] = exp(i*sum of thetas of
+ *     tile bits < la set in the index) and table B[2^(tb-la)] likewise for
+ *     tile bits >= la}, la = (low_bits + nhigh) / 2.
+ * Both kinds apply only where every bit of `local_ctrl` (tile-local mask)
+ * and of `global_ctrl` (mask of out-of-tile bit positions) is 1. */
+#define QJ_TILE_DENSE 0
+#define QJ_TILE_PHASE 1
+#define QJ_TILE_MAX_K 5
+#define QJ_TILE_MAX_BITS 13
+
+typedef struct {
+  int32_t kind;
+  int32_t k;
+  int32_t target[QJ_TILE_MAX_K + 1];
+  uint64_t local_ctrl;
+  uint64_t global_ctrl;
+  uint64_t offset; /* byte offset of the op's data in `data` (16-aligned) */
+  uint64_t reserved;
+} qj_tile_op; /* 64 bytes */
+
+int qj_apply_tile_pass(void* state, int dtype, int nqubits, int low_bits,
+                       const int32_t* high_bits, int nhigh,
+                       const qj_tile_op* ops, int nops, const void* data,
+                       uint64_t data_bytes, void* stream);
+
 /* ---- measurement ---- */
 
 /* Collapse onto outcome `result` of the measured bits `qubits_kernel`
  * (= [n - q - 1 for q in reversed(measured qubits)], cpu.py:550-552); bit j of
- * `result` is the value of bit qubits_kernel[j] (ops.py:33-40).  Zeroes every
+ * `result` is the value of bit qubits_kernel[j] (ops.py:33-40; for ascending
+ * qubits_kernel this is the reference's collapse_index set exactly, and it
+ * stays well defined for any order).  Zeroes every
  * other amplitude and, when `normalize`, divides the kept ones by the FP64
  * norm sqrt(sum |kept|^2).  Replaces ops.collapse_state /
  * collapse_state_normalized (ops.py:43-79) and CupyBackend.collapse_state

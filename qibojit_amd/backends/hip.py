@@ -34,6 +34,7 @@ from qibojit_amd._qibo import (
     QuantumState,
     raise_error,
 )
+from qibojit_amd import tiles
 from qibojit_amd.backends.matrices import CustomMatrices
 from qibojit_amd.fusion import GATE_OPS, fused_matrix
 
@@ -99,6 +100,8 @@ class HipBackend(Backend):
         self.measure_frequencies_op = self.measure_frequencies
         # Metropolis chain count, as NumbaBackend (cpu.py:86-89).
         self.nthreads = len(os.sched_getaffinity(0))
+        # execute_circuit runs unitary segments through the fused tile executor
+        self.fusion = True
 
     # ------------------------------------------------------------ config --
     def set_device(self, device):
@@ -563,7 +566,13 @@ class HipBackend(Backend):
 
     # ------------------------------------------------------- execution --
     def execute_circuit(self, circuit, initial_state=None, nshots=1000):
-        """Apply the circuit's queue to |0...0> or ``initial_state``."""
+        """Apply the circuit's queue to |0...0> or ``initial_state``.
+
+        With ``self.fusion`` (default), runs of unitary gates go through the
+        fused tile executor (a few HBM passes per run instead of one per
+        gate); other gates (measurements, channels, wide gates) are applied
+        one by one, as qibo's execute_circuit does.
+        """
         del nshots
         nqubits = circuit.nqubits
         density_matrix = bool(getattr(circuit, "density_matrix", False))
@@ -574,12 +583,62 @@ class HipBackend(Backend):
                 initial_state = initial_state.state()
             dtype = _dtype_name(initial_state.dtype) if hasattr(initial_state, "dtype") else self.dtype
             state = self.cast(initial_state, dtype=dtype, copy=True)
+        if self.fusion:
+            return QuantumState(self.apply_gates_fused(circuit.queue, state, nqubits), self)
         for gate in circuit.queue:
             if density_matrix:
                 state = gate.apply_density_matrix(self, state, nqubits)
             else:
                 state = gate.apply(self, state, nqubits)
         return QuantumState(state, self)
+
+    def tile_geometry(self, nbits, tdtype):
+        """(low bits, max high bits) of fused tiles: 64 KiB of LDS per tile,
+        runs of 2^low contiguous amplitudes (512 B) for coalescing."""
+        if tdtype == torch.complex64:
+            low, tb = 6, 13
+        else:
+            low, tb = 5, 12
+        low = min(low, nbits)
+        return low, max(0, min(tb - low, nbits - low))
+
+    @staticmethod
+    def _is_unitary_gate(gate):
+        name = gate.__class__.__name__
+        return not (name in ("M", "Align", "CallbackGate") or "Channel" in name or hasattr(gate, "coefficients"))
+
+    def apply_gates_fused(self, gates, state, nqubits):
+        """Apply ``gates`` in order with the fused tile executor."""
+        state = self._as_state(state)
+        density_matrix = len(tuple(state.shape)) == 2
+        shape = state.shape
+        flat = state.reshape(-1)
+        nbits = 2 * nqubits if density_matrix else nqubits
+        low, max_high = self.tile_geometry(nbits, flat.dtype)
+        npdt = np.complex64 if flat.dtype == torch.complex64 else np.complex128
+        pending = []
+
+        def flush():
+            if pending:
+                passes = tiles.plan(pending, nbits, low, max_high)
+                tiles.run_passes(call, _ptr(flat), _NATIVE_CODES[flat.dtype], nbits, passes, npdt, self._stream())
+                pending.clear()
+
+        for gate in gates:
+            seq = tiles.expand_fanout(gate) if gate.name == "fanout" else [gate]
+            for g in seq:
+                op = tiles.gate_to_op(self._as_custom_matrix, g, nbits) if self._is_unitary_gate(g) else None
+                if op is None or not tiles.fits_alone(op, low, max_high):
+                    flush()
+                    st = flat.reshape(shape)
+                    st = g.apply_density_matrix(self, st, nqubits) if density_matrix else g.apply(self, st, nqubits)
+                    flat = self._as_state(st).reshape(-1)
+                    continue
+                pending.append(op)
+                if density_matrix:
+                    pending.append(tiles.column_op(op, nqubits))
+        flush()
+        return flat.reshape(shape)
 
     def assert_allclose(self, value, target, rtol=1e-7, atol=0.0):
         if isinstance(value, QuantumState):

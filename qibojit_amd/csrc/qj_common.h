@@ -15,7 +15,7 @@ typedef int64_t i64;
 
 constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
 constexpr int kMaxPos = 48;      // max bit positions in an insertion spec
-constexpr u64 kMaxGrid = 1u << 20;  // grid-stride cap: keeps grid*block < 2^32
+constexpr u64 kMaxGrid = 1u << 23;  // grid-stride cap: keeps grid*block < 2^32
 
 // Complex number stored as {re, im}; aligned to its size so that one
 // amplitude is one 8 B (complex64) or 16 B (complex128) vector access.

@@ -11,7 +11,8 @@
 // are contiguous runs of the state for every target position; when every
 // gate bit is >= 1 the lowest free bit is bit 0 and a lane moves V = 2
 // consecutive amplitudes per access (16 B for complex64, 32 B for
-// complex128).  Z-like kernels (z, z_pow) read and write only the half they
+// complex128; only complex64 uses it, complex128 moves one 16 B amplitude
+// per access).  Z-like kernels (z, z_pow) read and write only the half they
 // change; swap only touches |01>,|10>; fsim skips |00>.
 #include <algorithm>
 #include <cstring>
@@ -30,7 +31,10 @@ struct M2 {
   cplx<R> m[16];
 };
 
-constexpr int kIlp = 2;  // independent work items per thread per iteration
+// One work item per thread and a full grid: measured fastest on gfx950 for
+// every target stride (more items per thread or a persistent grid lowered
+// the in-place rate by 5-10%).
+constexpr int kIlp = 1;
 
 // --------------------------------------------------------------- 1 qubit --
 template <typename R, int KIND, int V>
@@ -272,7 +276,8 @@ static int one_qubit_impl(void* state, int nqubits, int kind, int m, const int32
     return set_error(QJ_EVALUE, "apply_one_qubit: unknown kind %d", kind);
   }
   const u64 npairs = u64(1) << (nqubits - ins.npos);
-  const bool vec = ins.pos[0] >= 1;  // lowest gate bit above bit 0
+  // complex64 lanes move two amplitudes (16 B) when bit 0 is free
+  const bool vec = sizeof(R) == 4 && ins.pos[0] >= 1;
   const u64 ngroups = vec ? npairs / 2 : npairs;
   const u64 tmask = u64(1) << m;
   cplx<R>* st = static_cast<cplx<R>*>(state);
@@ -313,7 +318,7 @@ static int two_qubit_impl(void* state, int nqubits, int kind, int bit1, int bit2
     return set_error(QJ_EVALUE, "apply_two_qubit: unknown kind %d", kind);
   }
   const u64 nquads = u64(1) << (nqubits - ins.npos);
-  const bool vec = ins.pos[0] >= 1;
+  const bool vec = sizeof(R) == 4 && ins.pos[0] >= 1;
   const u64 ngroups = vec ? nquads / 2 : nquads;
   const u64 q1 = u64(1) << bit1, q2 = u64(1) << bit2;
   cplx<R>* st = static_cast<cplx<R>*>(state);

@@ -452,20 +452,17 @@ static int collapse_impl(void* state, int nqubits, const int32_t* qk, int nm, in
     for (int i = 0; i < j; ++i)
       if (qk[i] == qk[j]) return set_error(QJ_EVALUE, "collapse_state: repeated qubit");
   }
-  // collapse_index (ops.py:33-40) inserts bit j of h at qk[j], in list
-  // order; the map is linear over GF(2), so its image splits into
-  // index(g, 0) | index(0, h).  mask = index(0, all ones), keep = index(0, result).
-  auto cindex = [&](u64 g, u64 h) {
-    u64 i = g;
-    for (int j = 0; j < nm; ++j) {
-      const int n = qk[j];
-      const u64 k = u64(1) << n;
-      i = ((i >> n) << (n + 1)) + (i & (k - 1)) + ((h >> j) & 1) * k;
-    }
-    return i;
-  };
-  const u64 mask = cindex(0, (nm >= 64) ? ~u64(0) : ((u64(1) << nm) - 1));
-  const u64 keep = cindex(0, (u64)result);
+  // Bit j of `result` is the value of bit position qk[j].  For ascending qk
+  // (what the callers pass: qubits sorted, then reversed into bit
+  // positions) this is exactly the index set of collapse_index
+  // (ops.py:33-40), which inserts bit j of h at qk[j] in list order; for a
+  // non-ascending list that insertion shifts earlier bits past the vector,
+  // so positions are used directly instead.
+  u64 mask = 0, keep = 0;
+  for (int j = 0; j < nm; ++j) {
+    mask |= u64(1) << qk[j];
+    keep |= (u64)((result >> j) & 1) << qk[j];
+  }
   DeviceScratch* sc = nullptr;
   int rc = scratch_for_current_device(&sc);
   if (rc) return rc;

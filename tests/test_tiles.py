@@ -1,0 +1,147 @@
+"""Fused tile planner and encoder, on the CPU: plan + encode + a numpy
+emulation of the documented pass semantics equals gate-by-gate execution
+by the CPU port of NumbaBackend."""
+
+import numpy as np
+import pytest
+
+from oracle import einsum_ref as er
+from oracle import gate_matrices as gm
+from oracle import port
+from qibojit_amd import tiles
+from qibojit_amd.qibo_compat import gates
+from tests.tile_emulator import apply_tile_pass
+
+
+def _run_planned(gate_list, psi, nbits, low, max_high, density_nq=None):
+    """Same loop as HipBackend.apply_gates_fused, with the emulator."""
+    out = psi.copy()
+    pending, all_passes = [], []
+
+    def flush(state):
+        if pending:
+            passes = tiles.plan(pending, nbits, low, max_high)
+            for p in passes:
+                assert len(p.high) <= max_high and all(b >= low for b in p.high)
+                enc_ops, data, high = tiles.encode(p, nbits, psi.dtype)
+                state = apply_tile_pass(state, nbits, p.low, high, enc_ops, data, psi.dtype)
+            all_passes.extend(passes)
+            pending.clear()
+        return state
+
+    for g in gate_list:
+        for h in (tiles.expand_fanout(g) if g.name == "fanout" else [g]):
+            op = tiles.gate_to_op(port.custom_matrix, h, nbits)
+            if op is None or not tiles.fits_alone(op, low, max_high):
+                out = flush(out)
+                nq = density_nq or nbits
+                shaped = out.reshape(2**nq, 2**nq) if density_nq else out
+                out = port.PortBackend(2).apply_gate(h, shaped.copy(), nq).reshape(-1)
+                continue
+            pending.append(op)
+            if density_nq is not None:
+                pending.append(tiles.column_op(op, density_nq))
+    out = flush(out)
+    return out, all_passes
+
+
+def _random_circuit(n, depth, rng):
+    out = []
+    for layer in range(depth):
+        for q in range(n):
+            out.append(gates.U3(q, *rng.uniform(0, 2 * np.pi, 3)))
+            out.append(gates.RZ(q, rng.uniform(0, 2 * np.pi)))
+        for q in range(layer % 2, n - 1, 2):
+            out.append(gates.CZ(q, q + 1))
+        a, b = rng.choice(n, 2, replace=False)
+        out.append(gates.CNOT(int(a), int(b)))
+        a, b = rng.choice(n, 2, replace=False)
+        out.append(gates.fSim(int(a), int(b), 0.3, 0.7))
+        a, b, c = rng.choice(n, 3, replace=False)
+        out.append(gates.Unitary(gm.haar_unitary(8, rng), int(a), int(b), int(c)))
+        out.append(gates.CU1(int(a), int(c), 0.41))
+        out.append(gates.Z(int(b)).controlled_by(int(a)))
+        out.append(gates.CCZ(int(a), int(b), int(c)))
+        out.append(gates.T(int(c)))
+        out.append(gates.Y(int(a)).controlled_by(int(c)))
+    return [g for g in out if len(set(g.qubits)) == len(g.qubits)]
+
+
+@pytest.mark.parametrize("n,low,max_high", [(6, 2, 2), (9, 3, 4), (10, 5, 5), (8, 1, 6)])
+def test_tile_plan_matches_gate_by_gate(n, low, max_high):
+    rng = np.random.default_rng(n * 7 + low)
+    circuit = _random_circuit(n, 4, rng)
+    psi = gm.random_state(n, rng)
+    want = psi.copy()
+    for g in circuit:
+        want = port.PortBackend(2).apply_gate(g, want, n)
+    got, passes = _run_planned(circuit, psi, n, low, max_high)
+    assert len(passes) < len(circuit)
+    np.testing.assert_allclose(got, want, atol=1e-12)
+
+
+def test_qft_phase_ladders_merge():
+    n = 12
+    c = []
+    for i in range(n):
+        c.append(gates.H(i))
+        for j in range(i + 1, n):
+            c.append(gates.CU1(j, i, np.pi / 2 ** (j - i)))
+    for i in range(n // 2):
+        c.append(gates.SWAP(i, n - i - 1))
+    rng = np.random.default_rng(3)
+    psi = gm.random_state(n, rng)
+    want = psi.copy()
+    for g in c:
+        want = port.PortBackend(2).apply_gate(g, want, n)
+    got, passes = _run_planned(c, psi, n, 4, 5)
+    np.testing.assert_allclose(got, want, atol=1e-12)
+    nphase = sum(isinstance(op, tiles.PhaseOp) for p in passes for op in p.ops)
+    assert nphase <= n  # one merged ladder per H
+    # closed form: QFT matrix on the bit-reversed... compare with the DFT
+    dft = np.fft.ifft(psi) * np.sqrt(2**n)
+    np.testing.assert_allclose(got, dft, atol=1e-12)
+
+
+def test_tile_plan_density_matrix():
+    n = 4
+    rng = np.random.default_rng(5)
+    circuit = _random_circuit(n, 2, rng)
+    rho = gm.random_density_matrix(n, rng)
+    want = rho.copy()
+    for g in circuit:
+        want = port.PortBackend(2).apply_gate(g, want, n)
+    got, _ = _run_planned(circuit, rho.reshape(-1), 2 * n, 3, 4, density_nq=n)
+    np.testing.assert_allclose(got.reshape(rho.shape), want, atol=1e-12)
+
+
+def test_fanout_expands_to_cnots():
+    n = 5
+    rng = np.random.default_rng(6)
+    psi = gm.random_state(n, rng)
+    g = gates.FanOut(1, 0, 3, 4)
+    want = port.PortBackend(2).apply_gate(g, psi.copy(), n)
+    got, _ = _run_planned([g], psi, n, 2, 3)
+    np.testing.assert_allclose(got, want, atol=1e-14)
+
+
+def test_phase_merge_rules():
+    a = tiles.PhaseOp({3}, 0.0, {5: 0.2})
+    b = tiles.PhaseOp({4}, 0.0, {5: 0.3})  # shares bit 5 -> merges as control 5
+    m = a.merged(b)
+    assert m is not None and m.controls == {5} and m.thetas == {3: 0.2, 4: 0.3}
+    c = tiles.PhaseOp(set(), -0.1, {1: 0.2})
+    d = tiles.PhaseOp(set(), 0.05, {1: 0.1, 2: 0.4})
+    m = c.merged(d)
+    assert m.controls == frozenset() and abs(m.c0 + 0.05) < 1e-15 and m.thetas == {1: 0.30000000000000004, 2: 0.4}
+    assert tiles.PhaseOp({1, 2}, 0.0, {3: 0.1}).merged(tiles.PhaseOp({1}, 0.0, {3: 0.1})) is None
+
+
+def test_diagonal_gates_become_phase_ops():
+    for g in (gates.Z(0), gates.RZ(1, 0.3), gates.CU1(0, 2, 0.5), gates.CZ(1, 2), gates.T(0), gates.CCZ(0, 1, 2)):
+        assert isinstance(tiles.gate_to_op(port.custom_matrix, g, 3), tiles.PhaseOp), g.name
+    for g in (gates.H(0), gates.X(1), gates.CNOT(0, 1), gates.SWAP(0, 2), gates.fSim(0, 1, 0.1, 0.2)):
+        assert isinstance(tiles.gate_to_op(port.custom_matrix, g, 3), tiles.DenseOp), g.name
+    psi = gm.random_state(3, np.random.default_rng(1))
+    got, _ = _run_planned([gates.RZ(1, 0.3)], psi, 3, 1, 2)
+    np.testing.assert_allclose(got, er.apply_gate(psi, gm.RZ(0.3), [1]), atol=1e-15)
