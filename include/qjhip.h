@@ -143,12 +143,13 @@ int qj_apply_multi_qubit(void* state, int dtype, int nqubits,
  *   QJ_TILE_DENSE: 2^k x 2^k row-major matrix (k <= 5) at `data + offset` on
  *     the tile-local target bits target[0..k-1] (target[0] = matrix MSB);
  *   QJ_TILE_PHASE: multiplies amplitude i by exp(i*angle(i)) with
- *     angle(i) = c0 + sum_j theta_j * bit_j(i), where in-tile bits enter
- *     through two tables at `data + offset`: {double c0; int32 nout; int32
- *     pad; then nout x (int32 bit, int32 pad, double theta) for the
- *     out-of-tile bits; then complex table A[2^la] = exp(i*sum of thetas of
- *     tile bits < la set in the index) and table B[2^(tb-la)] likewise for
- *     tile bits >= la}, la = (low_bits + nhigh) / 2.
+ *     angle(i) = c0 + sum_j theta_j * bit_j(i), as a product of four complex
+ *     tables at `data + offset`: {int32 ga, gb, la, pad; OA[2^ga], OB[2^gb],
+ *     A[2^la], B[2^(tb-la)]}.  With t the tile index (the out-of-tile bits of
+ *     i, ascending, packed; ga + gb = nqubits - tb) and e the tile-local
+ *     index (tb = low_bits + nhigh bits, la = min(8, tb)), the factor is
+ *     OA[t mod 2^ga] * OB[t >> ga] * A[e mod 2^la] * B[e >> la]; each table
+ *     holds exp(i * sum of the thetas of its set bits), c0 folded into OA.
  * Both kinds apply only where every bit of `local_ctrl` (tile-local mask)
  * and of `global_ctrl` (mask of out-of-tile bit positions) is 1. */
 #define QJ_TILE_DENSE 0

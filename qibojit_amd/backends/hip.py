@@ -102,6 +102,7 @@ class HipBackend(Backend):
         self.nthreads = len(os.sched_getaffinity(0))
         # execute_circuit runs unitary segments through the fused tile executor
         self.fusion = True
+        self._custom_matrices_c128 = CustomMatrices("complex128")
 
     # ------------------------------------------------------------ config --
     def set_device(self, device):
@@ -323,12 +324,15 @@ class HipBackend(Backend):
         flat = self._one_qubit_base(flat, 2 * nqubits, *targets_dm, "apply_gate", np.conj(matrix), qubits)
         return flat.reshape(shape)
 
-    def _as_custom_matrix(self, gate):
-        """Kernel-facing encoding of ``gate`` (cpu.py:519-539)."""
+    def _as_custom_matrix(self, gate, matrices=None):
+        """Kernel-facing encoding of ``gate`` (cpu.py:519-539), from
+        ``matrices`` (default: the backend's, in its dtype)."""
         name = gate.__class__.__name__
+        if matrices is None:
+            matrices = self.custom_matrices
         if isinstance(gate, FusedGate):
-            return self.matrix_fused(gate)
-        _matrix = getattr(self.custom_matrices, name)
+            return fused_matrix(lambda g: self._as_custom_matrix(g, matrices), gate)
+        _matrix = getattr(matrices, name)
         if name == "FanOut":
             return _matrix(*gate.init_args)
         if isinstance(gate, ParametrizedGate):
@@ -602,6 +606,9 @@ class HipBackend(Backend):
         low = min(low, nbits)
         return low, max(0, min(tb - low, nbits - low))
 
+    def _custom_matrix_c128(self, gate):
+        return self._as_custom_matrix(gate, self._custom_matrices_c128)
+
     @staticmethod
     def _is_unitary_gate(gate):
         name = gate.__class__.__name__
@@ -627,7 +634,9 @@ class HipBackend(Backend):
         for gate in gates:
             seq = tiles.expand_fanout(gate) if gate.name == "fanout" else [gate]
             for g in seq:
-                op = tiles.gate_to_op(self._as_custom_matrix, g, nbits) if self._is_unitary_gate(g) else None
+                # ops are built from complex128 matrices (exact phase angles)
+                # and cast to the state's precision when a pass is encoded
+                op = tiles.gate_to_op(self._custom_matrix_c128, g, nbits) if self._is_unitary_gate(g) else None
                 if op is None or not tiles.fits_alone(op, low, max_high):
                     flush()
                     st = flat.reshape(shape)

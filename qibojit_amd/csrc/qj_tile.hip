@@ -3,13 +3,23 @@
 //
 // A tile is 2^TB amplitudes: all combinations of the L lowest bits (one
 // contiguous run of 2^L amplitudes in memory) and of H = TB - L chosen high
-// bit positions, for one assignment of the remaining bits (the tile base).
-// Workgroups take tiles in a grid-stride loop: load (coalesced: a wave reads
-// 64 consecutive vectors of one run), apply every op with a barrier between
-// ops, store.  Dense ops on tile-local targets run as per-group register
-// mat-vecs; diagonal (phase) ops multiply each element by
-// exp(i*angle) built from two per-pass lookup tables and one per-tile
-// factor for the out-of-tile bits, so no trigonometry runs per element.
+// bit positions, for one assignment of the remaining bits (the tile base,
+// whose compact index is the tile number t).  Workgroups take tiles in a
+// grid-stride loop: load (a wave reads 64 consecutive 16-byte vectors of one
+// run), apply every op with a barrier between ops, store.
+//
+// Thread layout: thread `tid` of a BS-thread workgroup owns tile elements
+// e = tid + j*BS.  The low 8 bits of e are then fixed per thread and the
+// remaining bits are the same for all 64 lanes of a wave.
+//  * Dense ops run as register mat-vecs on GB independent groups at a time
+//    (all loads, then all arithmetic, then all stores), so LDS latency of
+//    one group hides behind the others.
+//  * Phase ops multiply element e by F(t) * A(e & 255) * B(e >> 8): F from
+//    two tables over the bits of t (per tile, wave-uniform), A per thread,
+//    B wave-uniform -- one complex product per element, no trigonometry.
+// Kernel instances are specialised on the widest dense op of the pass
+// (register budget) and on the block size.
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -18,13 +28,14 @@
 namespace qj {
 
 constexpr int kTileMaxHighBits = 8;
+constexpr int kTileLA = 8;  // tile bits indexing phase table A
 
 struct TileSpec {
   int L;                       // low contiguous bits
   int TB;                      // tile bits
-  int LA;                      // phase table split: tile bits < LA index table A
   int H;                       // high bits
-  int hb[kTileMaxHighBits];    // their global positions, ascending
+  int nout;                    // bits outside the tile (bits of the tile index t)
+  int hb[kTileMaxHighBits];    // high bit positions, ascending
 };
 
 // Global offset of the high-bit combination h (bit j of h -> position hb[j]).
@@ -36,45 +47,37 @@ __device__ __forceinline__ u64 high_offset(int h, const TileSpec& ts) {
   return o;
 }
 
+// Phase data at op.offset: int32 ga, int32 gb, int32 la, int32 pad; then
+// complex tables OA[2^ga], OB[2^gb] over the tile index t (t & (2^ga-1), t >> ga),
+// A[2^la] over tile bits < la, B[2^(TB-la)] over tile bits >= la.
 struct PhaseHeader {
-  double c0;
-  int32_t nout;
-  int32_t pad;
-};
-struct PhaseTerm {
-  int32_t bit;
-  int32_t pad;
-  double theta;
+  int32_t ga, gb, la, pad;
 };
 
-template <typename R, int V>
+template <typename R, int V, int BS>
 __device__ __forceinline__ void tile_load(cplx<R>* __restrict__ tile, const cplx<R>* __restrict__ st,
                                           u64 base, const TileSpec& ts) {
   const int n = 1 << ts.TB;
   const int lmask = (1 << ts.L) - 1;
-  for (int e = threadIdx.x * V; e < n; e += kBlock * V) {
+  for (int e = threadIdx.x * V; e < n; e += BS * V) {
     const u64 gi = base | (u64)(e & lmask) | high_offset(e >> ts.L, ts);
     const cvec<R, V> v = ldv<R, V>(st + gi);
-#pragma unroll
-    for (int j = 0; j < V; ++j) tile[e + j] = v.v[j];
+    *reinterpret_cast<cvec<R, V>*>(tile + e) = v;
   }
 }
 
-template <typename R, int V>
+template <typename R, int V, int BS>
 __device__ __forceinline__ void tile_store(const cplx<R>* __restrict__ tile, cplx<R>* __restrict__ st,
                                            u64 base, const TileSpec& ts) {
   const int n = 1 << ts.TB;
   const int lmask = (1 << ts.L) - 1;
-  for (int e = threadIdx.x * V; e < n; e += kBlock * V) {
+  for (int e = threadIdx.x * V; e < n; e += BS * V) {
     const u64 gi = base | (u64)(e & lmask) | high_offset(e >> ts.L, ts);
-    cvec<R, V> v;
-#pragma unroll
-    for (int j = 0; j < V; ++j) v.v[j] = tile[e + j];
-    stv<R, V>(st + gi, v);
+    stv<R, V>(st + gi, *reinterpret_cast<const cvec<R, V>*>(tile + e));
   }
 }
 
-template <typename R, int K>
+template <typename R, int K, int GB, int BS>
 __device__ __forceinline__ void tile_dense(cplx<R>* __restrict__ tile, const qj_tile_op& op,
                                            const cplx<R>* __restrict__ m, int TB) {
   constexpr int N = 1 << K;
@@ -87,8 +90,7 @@ __device__ __forceinline__ void tile_dense(cplx<R>* __restrict__ tile, const qj_
       if ((r >> (K - 1 - u)) & 1) o |= 1 << op.target[u];
     off[r] = o;
   }
-  // ascending target bits for the zero insertion
-  int srt[K];
+  int srt[K];  // ascending target bits for the zero insertion
 #pragma unroll
   for (int u = 0; u < K; ++u) srt[u] = op.target[u];
 #pragma unroll
@@ -102,79 +104,121 @@ __device__ __forceinline__ void tile_dense(cplx<R>* __restrict__ tile, const qj_
       }
   const int lc = (int)op.local_ctrl;
   const int ngr = 1 << (TB - K);
-  for (int g = threadIdx.x; g < ngr; g += kBlock) {
-    int e0 = g;
+  for (int g0 = threadIdx.x; g0 < ngr; g0 += BS * GB) {
+    int e0[GB];
+    bool act[GB];
+    cplx<R> x[GB][N];
 #pragma unroll
-    for (int u = 0; u < K; ++u) {
-      const int lo = e0 & ((1 << srt[u]) - 1);
-      e0 = ((e0 ^ lo) << 1) | lo;
+    for (int b = 0; b < GB; ++b) {
+      int e = g0 + b * BS;
+      act[b] = e < ngr;
+#pragma unroll
+      for (int u = 0; u < K; ++u) {
+        const int lo = e & ((1 << srt[u]) - 1);
+        e = ((e ^ lo) << 1) | lo;
+      }
+      e0[b] = e;
+      act[b] = act[b] && ((e & lc) == lc);
+      if (act[b]) {
+#pragma unroll
+        for (int c = 0; c < N; ++c) x[b][c] = tile[e + off[c]];
+      }
     }
-    if ((e0 & lc) != lc) continue;
-    cplx<R> x[N];
 #pragma unroll
-    for (int c = 0; c < N; ++c) x[c] = tile[e0 + off[c]];
+    for (int b = 0; b < GB; ++b) {
+      if (!act[b]) continue;
 #pragma unroll
-    for (int r = 0; r < N; ++r) {
-      cplx<R> acc = cmul(m[r * N], x[0]);
+      for (int r = 0; r < N; ++r) {
+        cplx<R> acc = cmul(m[r * N], x[b][0]);
 #pragma unroll
-      for (int c = 1; c < N; ++c) acc = cadd(acc, cmul(m[r * N + c], x[c]));
-      tile[e0 + off[r]] = acc;
+        for (int c = 1; c < N; ++c) acc = cadd(acc, cmul(m[r * N + c], x[b][c]));
+        tile[e0[b] + off[r]] = acc;
+      }
     }
   }
 }
 
-template <typename R>
+template <typename R, int BS>
 __device__ __forceinline__ void tile_phase(cplx<R>* __restrict__ tile, const qj_tile_op& op,
-                                           const char* __restrict__ data, u64 base, const TileSpec& ts) {
-  const PhaseHeader* h = reinterpret_cast<const PhaseHeader*>(data + op.offset);
-  const PhaseTerm* terms = reinterpret_cast<const PhaseTerm*>(h + 1);
-  double ang = h->c0;
-  for (int j = 0; j < h->nout; ++j)
-    if ((base >> terms[j].bit) & 1) ang += terms[j].theta;
-  double sn, cs;
-  sincos(ang, &sn, &cs);
-  const cplx<R> f = mk<R>((R)cs, (R)sn);
-  const cplx<R>* ta = reinterpret_cast<const cplx<R>*>(terms + h->nout);
-  const cplx<R>* tb = ta + (1 << ts.LA);
-  const int amask = (1 << ts.LA) - 1;
+                                           const char* __restrict__ data, u64 t, int TB) {
+  const PhaseHeader h = *reinterpret_cast<const PhaseHeader*>(data + op.offset);
+  const cplx<R>* oa = reinterpret_cast<const cplx<R>*>(data + op.offset + sizeof(PhaseHeader));
+  const cplx<R>* ob = oa + (1 << h.ga);
+  const cplx<R>* ta = ob + (1 << h.gb);
+  const cplx<R>* tb = ta + (1 << h.la);
+  const cplx<R> f = cmul(oa[t & ((u64(1) << h.ga) - 1)], ob[t >> h.ga]);
+  const int amask = (1 << h.la) - 1;
   const int lc = (int)op.local_ctrl;
-  const int n = 1 << ts.TB;
-  for (int e = threadIdx.x; e < n; e += kBlock) {
+  const int n = 1 << TB;
+  // BS is a multiple of 2^la (la <= 8), so e & amask == threadIdx.x & amask
+  // for every element of this thread.
+  const cplx<R> af = cmul(f, ta[threadIdx.x & amask]);
+  for (int e = threadIdx.x; e < n; e += BS) {
     if ((e & lc) != lc) continue;
-    const cplx<R> ph = cmul(cmul(f, ta[e & amask]), tb[e >> ts.LA]);
-    tile[e] = cmul(tile[e], ph);
+    tile[e] = cmul(tile[e], cmul(af, tb[e >> h.la]));
   }
 }
 
-template <typename R, int V>
-__global__ void __launch_bounds__(kBlock)
+template <typename R, int V, int MAXK, int BS>
+__global__ void __launch_bounds__(BS)
     k_tile_pass(cplx<R>* __restrict__ st, u64 ntiles, Ins tins, TileSpec ts,
                 const qj_tile_op* __restrict__ ops, int nops, const char* __restrict__ data) {
-  extern __shared__ char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   cplx<R>* tile = reinterpret_cast<cplx<R>*>(smem_raw);
   for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const u64 base = insert_bits(t, tins);
-    tile_load<R, V>(tile, st, base, ts);
+    tile_load<R, V, BS>(tile, st, base, ts);
     __syncthreads();
     for (int o = 0; o < nops; ++o) {
       const qj_tile_op& op = ops[o];
       if ((base & op.global_ctrl) != op.global_ctrl) continue;  // uniform
       if (op.kind == QJ_TILE_PHASE) {
-        tile_phase<R>(tile, op, data, base, ts);
+        tile_phase<R, BS>(tile, op, data, t, ts.TB);
       } else {
         const cplx<R>* m = reinterpret_cast<const cplx<R>*>(data + op.offset);
         switch (op.k) {
-          case 1: tile_dense<R, 1>(tile, op, m, ts.TB); break;
-          case 2: tile_dense<R, 2>(tile, op, m, ts.TB); break;
-          case 3: tile_dense<R, 3>(tile, op, m, ts.TB); break;
-          case 4: tile_dense<R, 4>(tile, op, m, ts.TB); break;
-          default: tile_dense<R, 5>(tile, op, m, ts.TB); break;
+          case 1: tile_dense<R, 1, 4, BS>(tile, op, m, ts.TB); break;
+          case 2: if (MAXK >= 2) tile_dense<R, 2, 2, BS>(tile, op, m, ts.TB); break;
+          case 3: if (MAXK >= 3) tile_dense<R, 3, 1, BS>(tile, op, m, ts.TB); break;
+          case 4: if (MAXK >= 4) tile_dense<R, 4, 1, BS>(tile, op, m, ts.TB); break;
+          default: if (MAXK >= 5) tile_dense<R, 5, 1, BS>(tile, op, m, ts.TB); break;
         }
       }
       __syncthreads();
     }
-    tile_store<R, V>(tile, st, base, ts);
+    tile_store<R, V, BS>(tile, st, base, ts);
     __syncthreads();
+  }
+}
+
+// Block size of the tile kernel: 1024 threads (16 waves) by default, so the
+// two 64 KiB tiles an MI355X CU holds keep 32 waves in flight; QJ_TILE_BLOCK
+// (256/512/1024) overrides it for tuning.
+static int tile_block_size() {
+  static int bs = [] {
+    const char* s = std::getenv("QJ_TILE_BLOCK");
+    const int v = s ? std::atoi(s) : 1024;
+    return (v == 256 || v == 512 || v == 1024) ? v : 1024;
+  }();
+  return bs;
+}
+
+template <typename R, int V, int MAXK>
+static void launch_tile(int bs, u64 grid, size_t lds, hipStream_t s, cplx<R>* st, u64 ntiles, const Ins& tins,
+                        const TileSpec& ts, const qj_tile_op* ops, int nops, const char* data) {
+  dim3 gd((unsigned)grid);
+  // 1024-thread blocks cap a thread at 128 VGPRs, which the complex128
+  // 5-qubit mat-vec exceeds: that instance runs with 512 threads.
+  constexpr bool wide_regs = sizeof(R) == 8 && MAXK >= 4;
+  if (bs == 256) {
+    k_tile_pass<R, V, MAXK, 256><<<gd, dim3(256), lds, s>>>(st, ntiles, tins, ts, ops, nops, data);
+  } else if constexpr (wide_regs) {
+    k_tile_pass<R, V, MAXK, 512><<<gd, dim3(512), lds, s>>>(st, ntiles, tins, ts, ops, nops, data);
+  } else {
+    if (bs == 512)
+      k_tile_pass<R, V, MAXK, 512><<<gd, dim3(512), lds, s>>>(st, ntiles, tins, ts, ops, nops, data);
+    else
+      k_tile_pass<R, V, MAXK, 1024><<<gd, dim3(1024), lds, s>>>(st, ntiles, tins, ts, ops, nops, data);
   }
 }
 
@@ -194,8 +238,8 @@ static int tile_pass_impl(void* state, int nqubits, int L, const int32_t* high, 
   std::memset(&ts, 0, sizeof(ts));
   ts.L = L;
   ts.TB = TB;
-  ts.LA = TB / 2;
   ts.H = nhigh;
+  ts.nout = nqubits - TB;
   Ins tins;
   std::memset(&tins, 0, sizeof(tins));
   tins.npos = TB;
@@ -208,11 +252,14 @@ static int tile_pass_impl(void* state, int nqubits, int L, const int32_t* high, 
   }
   u64 tile_mask = (u64(1) << L) - 1;
   for (int j = 0; j < nhigh; ++j) tile_mask |= u64(1) << high[j];
+  int maxk = 1;
+  const int la = TB < kTileLA ? TB : kTileLA;
   for (int o = 0; o < nops; ++o) {
     const qj_tile_op& op = ops[o];
     if (op.global_ctrl & tile_mask)
       return set_error(QJ_EVALUE, "apply_tile_pass: op %d has a global control inside the tile", o);
     if (op.local_ctrl >> TB) return set_error(QJ_EVALUE, "apply_tile_pass: op %d local control out of tile", o);
+    if (op.offset % 16) return set_error(QJ_EVALUE, "apply_tile_pass: op %d data offset not 16-aligned", o);
     if (op.kind == QJ_TILE_DENSE) {
       if (op.k < 1 || op.k > QJ_TILE_MAX_K) return set_error(QJ_EVALUE, "apply_tile_pass: op %d has k=%d", o, op.k);
       u64 seen = 0;
@@ -224,23 +271,19 @@ static int tile_pass_impl(void* state, int nqubits, int L, const int32_t* high, 
       }
       const u64 need = op.offset + ((u64(1) << (2 * op.k)) * sizeof(cplx<R>));
       if (need > data_bytes) return set_error(QJ_EVALUE, "apply_tile_pass: op %d matrix outside data", o);
+      if (op.k > maxk) maxk = op.k;
     } else if (op.kind == QJ_TILE_PHASE) {
       if (op.offset + sizeof(PhaseHeader) > data_bytes)
         return set_error(QJ_EVALUE, "apply_tile_pass: op %d phase data outside data", o);
       PhaseHeader hh;
       std::memcpy(&hh, static_cast<const char*>(data) + op.offset, sizeof(hh));
-      const u64 need = op.offset + sizeof(PhaseHeader) + (u64)hh.nout * sizeof(PhaseTerm) +
-                       ((u64(1) << ts.LA) + (u64(1) << (TB - ts.LA))) * sizeof(cplx<R>);
-      if (hh.nout < 0 || need > data_bytes)
-        return set_error(QJ_EVALUE, "apply_tile_pass: op %d phase tables outside data", o);
-      const PhaseTerm* terms = reinterpret_cast<const PhaseTerm*>(static_cast<const char*>(data) + op.offset +
-                                                                  sizeof(PhaseHeader));
-      for (int j = 0; j < hh.nout; ++j) {
-        PhaseTerm tt;
-        std::memcpy(&tt, terms + j, sizeof(tt));
-        if (tt.bit < 0 || tt.bit >= nqubits || ((tile_mask >> tt.bit) & 1))
-          return set_error(QJ_EVALUE, "apply_tile_pass: op %d phase bit %d invalid", o, tt.bit);
-      }
+      if (hh.ga < 0 || hh.gb < 0 || hh.ga + hh.gb != ts.nout || hh.la != la)
+        return set_error(QJ_EVALUE, "apply_tile_pass: op %d phase table shape (%d,%d,%d) does not match the tile",
+                         o, hh.ga, hh.gb, hh.la);
+      const u64 need = op.offset + sizeof(PhaseHeader) +
+                       ((u64(1) << hh.ga) + (u64(1) << hh.gb) + (u64(1) << la) + (u64(1) << (TB - la))) *
+                           sizeof(cplx<R>);
+      if (need > data_bytes) return set_error(QJ_EVALUE, "apply_tile_pass: op %d phase tables outside data", o);
     } else {
       return set_error(QJ_EVALUE, "apply_tile_pass: op %d has unknown kind %d", o, op.kind);
     }
@@ -248,8 +291,9 @@ static int tile_pass_impl(void* state, int nqubits, int L, const int32_t* high, 
   // ops and data travel in one staging upload, ops first (64 B each, so the
   // data block starts 16-byte aligned)
   const size_t ops_bytes = (size_t)nops * sizeof(qj_tile_op);
-  std::vector<char> buf(ops_bytes + data_bytes);
-  std::memcpy(buf.data(), ops, ops_bytes);
+  // never empty: an empty vector's data() is null and the ring copies from it
+  std::vector<char> buf(ops_bytes + data_bytes > 16 ? ops_bytes + data_bytes : 16, 0);
+  if (ops_bytes) std::memcpy(buf.data(), ops, ops_bytes);
   if (data_bytes) std::memcpy(buf.data() + ops_bytes, data, data_bytes);
   void* dbuf = nullptr;
   int ticket = -1;
@@ -261,10 +305,14 @@ static int tile_pass_impl(void* state, int nqubits, int L, const int32_t* high, 
   const u64 grid = ntiles < kMaxGrid ? ntiles : kMaxGrid;
   const size_t lds = ((size_t)1 << TB) * sizeof(cplx<R>);
   cplx<R>* st = static_cast<cplx<R>*>(state);
-  if (V == 2)
-    k_tile_pass<R, 2><<<dim3((unsigned)grid), dim3(kBlock), lds, s>>>(st, ntiles, tins, ts, dops, nops, ddata);
+  const int bs = tile_block_size();
+  constexpr int VV = 16 / (int)sizeof(cplx<R>);
+  if (maxk <= 2)
+    launch_tile<R, VV, 2>(bs, grid, lds, s, st, ntiles, tins, ts, dops, nops, ddata);
+  else if (maxk == 3)
+    launch_tile<R, VV, 3>(bs, grid, lds, s, st, ntiles, tins, ts, dops, nops, ddata);
   else
-    k_tile_pass<R, 1><<<dim3((unsigned)grid), dim3(kBlock), lds, s>>>(st, ntiles, tins, ts, dops, nops, ddata);
+    launch_tile<R, VV, 5>(bs, grid, lds, s, st, ntiles, tins, ts, dops, nops, ddata);
   rc = check_launch("apply_tile_pass launch");
   ring_release(ticket, s);
   return rc;
@@ -280,6 +328,7 @@ extern "C" int qj_apply_tile_pass(void* state, int dtype, int nqubits, int low_b
   if (!state) return set_error(QJ_EVALUE, "apply_tile_pass: null state");
   if (nops < 0 || (nops > 0 && !ops)) return set_error(QJ_EVALUE, "apply_tile_pass: bad ops");
   if (nhigh > 0 && !high_bits) return set_error(QJ_EVALUE, "apply_tile_pass: null high bits");
+  if (data_bytes > 0 && !data) return set_error(QJ_EVALUE, "apply_tile_pass: null data");
   hipStream_t s = (hipStream_t)stream;
   if (dtype == QJ_COMPLEX64)
     return tile_pass_impl<float>(state, nqubits, low_bits, high_bits, nhigh, ops, nops, data, data_bytes, s);

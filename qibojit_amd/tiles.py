@@ -171,13 +171,21 @@ def plan(ops, nqubits, low, max_high):
     return passes
 
 
+PHASE_LA = 8  # tile-local bits indexing phase table A (kernel constant)
+
+
 def encode(p, nqubits, npdtype):
     """(ops array, data bytes, high bits) of a pass for qj_apply_tile_pass."""
     low, high = p.low, p.high
     tb = low + len(high)
-    la = tb // 2
+    la = min(PHASE_LA, tb)
     local = {b: b for b in range(low)}
     local.update({b: low + j for j, b in enumerate(high)})
+    # out-of-tile bits, ascending: bit r of the tile index t is outside[r]
+    outside = [b for b in range(nqubits) if b not in local]
+    nout = len(outside)
+    ga = (nout + 1) // 2
+    rank = {b: r for r, b in enumerate(outside)}
     ops = np.zeros(len(p.ops), dtype=TILE_OP_DTYPE)
     chunks, off = [], 0
 
@@ -201,24 +209,21 @@ def encode(p, nqubits, npdtype):
             ops[i]["offset"] = put(np.ascontiguousarray(op.matrix, dtype=npdtype).tobytes())
         else:
             ops[i]["kind"] = TILE_PHASE
-            inner = {local[b]: th for b, th in op.thetas.items() if b in local}
-            outer = [(b, th) for b, th in op.thetas.items() if b not in local]
-            head = np.zeros(1, dtype=[("c0", "<f8"), ("nout", "<i4"), ("pad", "<i4")])
-            head["c0"], head["nout"] = op.c0, len(outer)
-            terms = np.zeros(len(outer), dtype=[("bit", "<i4"), ("pad", "<i4"), ("theta", "<f8")])
-            for j, (b, th) in enumerate(outer):
-                terms[j]["bit"], terms[j]["theta"] = b, th
-            ia = np.arange(2**la)
-            ib = np.arange(2 ** (tb - la))
-            ang_a = np.zeros(2**la)
-            ang_b = np.zeros(2 ** (tb - la))
-            for lb, th in inner.items():
-                if lb < la:
-                    ang_a += th * ((ia >> lb) & 1)
+            head = np.array([ga, nout - ga, la, 0], dtype="<i4")
+            # angle tables: OA/OB over the tile index t (c0 folded into OA),
+            # A/B over the tile-local bits below / from la
+            tabs = [np.zeros(2**ga), np.zeros(2 ** (nout - ga)), np.zeros(2**la), np.zeros(2 ** (tb - la))]
+            tabs[0] += op.c0
+            for b, th in op.thetas.items():
+                if b in local:
+                    lb = local[b]
+                    k, sh = (2, lb) if lb < la else (3, lb - la)
                 else:
-                    ang_b += th * ((ib >> (lb - la)) & 1)
-            blob = head.tobytes() + terms.tobytes()
-            blob += np.exp(1j * ang_a).astype(npdtype).tobytes() + np.exp(1j * ang_b).astype(npdtype).tobytes()
+                    r = rank[b]
+                    k, sh = (0, r) if r < ga else (1, r - ga)
+                idx = np.arange(tabs[k].size)
+                tabs[k] += th * ((idx >> sh) & 1)
+            blob = head.tobytes() + b"".join(np.exp(1j * a).astype(npdtype).tobytes() for a in tabs)
             ops[i]["offset"] = put(blob)
     data = b"".join(chunks)
     return ops, data, high

@@ -54,6 +54,18 @@ def test_tile_kernel_matches_emulator(backend, n, low, nhigh, dtype, rng):
     np.testing.assert_allclose(got, want, atol=PARITY[dtype] * 10, rtol=0)
 
 
+def test_tile_pass_without_ops_is_identity(backend, rng):
+    """Regression: an empty op list once made the staging copy read from a
+    null host pointer; the pass must just copy every tile back in place."""
+    psi = gm.random_state(12, rng)
+    state = backend.cast(psi, copy=True)
+    p = tiles.Pass(5)
+    p.high = [6, 8, 11]
+    tiles.run_passes(_native.call, ctypes.c_void_p(state.data_ptr()), _native.QJ_COMPLEX128, 12, [p],
+                     np.complex128, backend._stream())
+    np.testing.assert_array_equal(backend.to_numpy(state), psi)
+
+
 def test_tile_pass_rejects_bad_ops(backend):
     state = backend.zero_state(8)
     op = np.zeros(1, dtype=tiles.TILE_OP_DTYPE)

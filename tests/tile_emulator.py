@@ -10,13 +10,16 @@ from qibojit_amd.tiles import TILE_DENSE, TILE_PHASE
 def apply_tile_pass(state, nqubits, low, high, ops, data, npdtype):
     state = np.asarray(state)
     tb = low + len(high)
-    la = tb // 2
     pos = list(range(low)) + list(high)  # tile-local bit -> global bit
+    outside = [b for b in range(nqubits) if b not in pos]
     idx = np.arange(state.size, dtype=np.int64)
-    # tile-local coordinate of every amplitude
+    # tile-local coordinate and tile index of every amplitude
     local = np.zeros(state.size, dtype=np.int64)
     for lb, gb in enumerate(pos):
         local |= ((idx >> gb) & 1) << lb
+    tindex = np.zeros(state.size, dtype=np.int64)
+    for r, gb in enumerate(outside):
+        tindex |= ((idx >> gb) & 1) << r
     itemsize = np.dtype(npdtype).itemsize
     out = state.astype(np.complex128).copy()
     for op in ops:
@@ -37,15 +40,13 @@ def apply_tile_pass(state, nqubits, low, high, ops, data, npdtype):
             out = new
         else:
             assert op["kind"] == TILE_PHASE
-            c0 = np.frombuffer(data, dtype="<f8", count=1, offset=off)[0]
-            nout = int(np.frombuffer(data, dtype="<i4", count=1, offset=off + 8)[0])
-            terms = np.frombuffer(data, dtype=[("bit", "<i4"), ("pad", "<i4"), ("theta", "<f8")], count=nout, offset=off + 16)
-            toff = off + 16 + 16 * nout
-            ta = np.frombuffer(data, dtype=npdtype, count=2**la, offset=toff).astype(np.complex128)
-            tbl = np.frombuffer(data, dtype=npdtype, count=2 ** (tb - la), offset=toff + itemsize * 2**la).astype(np.complex128)
-            ang = np.full(state.size, c0)
-            for t in terms:
-                ang += t["theta"] * ((idx >> int(t["bit"])) & 1)
-            f = np.exp(1j * ang) * ta[local & (2**la - 1)] * tbl[local >> la]
+            ga, gb, la, _ = (int(v) for v in np.frombuffer(data, dtype="<i4", count=4, offset=off))
+            assert ga + gb == len(outside) and la == min(8, tb)
+            tabs, o = [], off + 16
+            for size in (2**ga, 2**gb, 2**la, 2 ** (tb - la)):
+                tabs.append(np.frombuffer(data, dtype=npdtype, count=size, offset=o).astype(np.complex128))
+                o += size * itemsize
+            f = (tabs[0][tindex & (2**ga - 1)] * tabs[1][tindex >> ga]
+                 * tabs[2][local & (2**la - 1)] * tabs[3][local >> la])
             out = np.where(ok, out * f, out)
     return out.astype(state.dtype)
