@@ -77,9 +77,36 @@ __device__ __forceinline__ void tile_store(const cplx<R>* __restrict__ tile, cpl
   }
 }
 
+// A phase op folded into the store of the dense op before it: element e is
+// multiplied by f * A[e & amask] * B[e >> la] where its local controls hold.
+template <typename R>
+struct PhaseCtx {
+  const cplx<R>* ta;
+  const cplx<R>* tb;
+  cplx<R> f;
+  int amask, la, lc;
+  bool on;
+};
+
+template <typename R>
+__device__ __forceinline__ PhaseCtx<R> phase_ctx(const qj_tile_op& op, const char* __restrict__ data, u64 t) {
+  PhaseCtx<R> c;
+  const PhaseHeader h = *reinterpret_cast<const PhaseHeader*>(data + op.offset);
+  const cplx<R>* oa = reinterpret_cast<const cplx<R>*>(data + op.offset + sizeof(PhaseHeader));
+  const cplx<R>* ob = oa + (1 << h.ga);
+  c.ta = ob + (1 << h.gb);
+  c.tb = c.ta + (1 << h.la);
+  c.f = cmul(oa[t & ((u64(1) << h.ga) - 1)], ob[t >> h.ga]);
+  c.amask = (1 << h.la) - 1;
+  c.la = h.la;
+  c.lc = (int)op.local_ctrl;
+  c.on = true;
+  return c;
+}
+
 template <typename R, int K, int GB, int BS>
 __device__ __forceinline__ void tile_dense(cplx<R>* __restrict__ tile, const qj_tile_op& op,
-                                           const cplx<R>* __restrict__ m, int TB) {
+                                           const cplx<R>* __restrict__ m, int TB, const PhaseCtx<R>& ph) {
   constexpr int N = 1 << K;
   int off[N];
 #pragma unroll
@@ -132,7 +159,10 @@ __device__ __forceinline__ void tile_dense(cplx<R>* __restrict__ tile, const qj_
         cplx<R> acc = cmul(m[r * N], x[b][0]);
 #pragma unroll
         for (int c = 1; c < N; ++c) acc = cadd(acc, cmul(m[r * N + c], x[b][c]));
-        tile[e0[b] + off[r]] = acc;
+        const int e = e0[b] + off[r];
+        if (ph.on && (e & ph.lc) == ph.lc)
+          acc = cmul(acc, cmul(cmul(ph.f, ph.ta[e & ph.amask]), ph.tb[e >> ph.la]));
+        tile[e] = acc;
       }
     }
   }
@@ -153,9 +183,12 @@ __device__ __forceinline__ void tile_phase(cplx<R>* __restrict__ tile, const qj_
   // BS is a multiple of 2^la (la <= 8), so e & amask == threadIdx.x & amask
   // for every element of this thread.
   const cplx<R> af = cmul(f, ta[threadIdx.x & amask]);
+#pragma unroll 4
   for (int e = threadIdx.x; e < n; e += BS) {
     if ((e & lc) != lc) continue;
-    tile[e] = cmul(tile[e], cmul(af, tb[e >> h.la]));
+    // e >> la is the same for the 64 lanes of a wave: a scalar load
+    const int bi = __builtin_amdgcn_readfirstlane(e >> h.la);
+    tile[e] = cmul(tile[e], cmul(af, tb[bi]));
   }
 }
 
@@ -176,12 +209,21 @@ __global__ void __launch_bounds__(BS)
         tile_phase<R, BS>(tile, op, data, t, ts.TB);
       } else {
         const cplx<R>* m = reinterpret_cast<const cplx<R>*>(data + op.offset);
+        // A phase op right after a dense op without local controls (which
+        // then writes every element once) is applied in its store.
+        PhaseCtx<R> ph;
+        ph.on = false;
+        if (op.local_ctrl == 0 && o + 1 < nops && ops[o + 1].kind == QJ_TILE_PHASE) {
+          const qj_tile_op& nx = ops[o + 1];
+          if ((base & nx.global_ctrl) == nx.global_ctrl) ph = phase_ctx<R>(nx, data, t);
+          ++o;  // consumed (or skipped: its global controls do not hold here)
+        }
         switch (op.k) {
-          case 1: tile_dense<R, 1, 4, BS>(tile, op, m, ts.TB); break;
-          case 2: if (MAXK >= 2) tile_dense<R, 2, 2, BS>(tile, op, m, ts.TB); break;
-          case 3: if (MAXK >= 3) tile_dense<R, 3, 1, BS>(tile, op, m, ts.TB); break;
-          case 4: if (MAXK >= 4) tile_dense<R, 4, 1, BS>(tile, op, m, ts.TB); break;
-          default: if (MAXK >= 5) tile_dense<R, 5, 1, BS>(tile, op, m, ts.TB); break;
+          case 1: tile_dense<R, 1, 4, BS>(tile, op, m, ts.TB, ph); break;
+          case 2: if (MAXK >= 2) tile_dense<R, 2, 2, BS>(tile, op, m, ts.TB, ph); break;
+          case 3: if (MAXK >= 3) tile_dense<R, 3, 1, BS>(tile, op, m, ts.TB, ph); break;
+          case 4: if (MAXK >= 4) tile_dense<R, 4, 1, BS>(tile, op, m, ts.TB, ph); break;
+          default: if (MAXK >= 5) tile_dense<R, 5, 1, BS>(tile, op, m, ts.TB, ph); break;
         }
       }
       __syncthreads();
@@ -191,14 +233,14 @@ __global__ void __launch_bounds__(BS)
   }
 }
 
-// Block size of the tile kernel: 1024 threads (16 waves) by default, so the
-// two 64 KiB tiles an MI355X CU holds keep 32 waves in flight; QJ_TILE_BLOCK
+// Block size of the tile kernel: 512 threads (8 waves) by default, measured
+// best on MI355X for 32-64 KiB tiles (two per CU); QJ_TILE_BLOCK
 // (256/512/1024) overrides it for tuning.
 static int tile_block_size() {
   static int bs = [] {
     const char* s = std::getenv("QJ_TILE_BLOCK");
-    const int v = s ? std::atoi(s) : 1024;
-    return (v == 256 || v == 512 || v == 1024) ? v : 1024;
+    const int v = s ? std::atoi(s) : 512;
+    return (v == 256 || v == 512 || v == 1024) ? v : 512;
   }();
   return bs;
 }
