@@ -30,6 +30,11 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale from a plain 4 GiB copy):
+# profiles/sweep_30q_c128_hbm_traffic.md.  Keyed by (kernel, nqubits).
+PROFILED_TRAFFIC = {("k_one_qubit<double,0,1>", 30): 29.588e9}
+
 
 def parse():
     p = argparse.ArgumentParser()
@@ -417,7 +422,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": PROFILED_TRAFFIC.get((name, n)),
+        "traffic_source": "profiles/sweep_30q_c128_hbm_traffic.md" if (name, n) in PROFILED_TRAFFIC else None,
         "kernel": name,
         "launches_per_step": launches,
         "avg_launch_ms": avg_ms,
