@@ -172,6 +172,58 @@ int qj_apply_tile_pass(void* state, int dtype, int nqubits, int low_bits,
                        const qj_tile_op* ops, int nops, const void* data,
                        uint64_t data_bytes, void* stream);
 
+/* ---- layered variational circuits (batched forward, TFIM, adjoint) ----
+ * No counterpart in the reference (qibojit has no differentiation or
+ * batching).  These serve circuits of single-qubit rotation layers and CNOT
+ * ladders CNOT(q, q+1), q = 0..n-2, applied to many states at once.
+ *
+ * States: launch row b (< nrows) works on state s = sel[b] (sel is a device
+ * array of int32, or NULL for s = b) stored at s * stride amplitudes from the
+ * buffer start.  Matrices: a device array, state s's at mats + s * mstride
+ * complex entries (the state's precision), laid out [bit j][step][2x2
+ * row-major].
+ *
+ * qj_rot_pass owns the nbits (1..QJ_ROT_MAX_BITS, adjoint mode
+ * 1..QJ_ROT_MAX_ADJ_BITS) ascending bit positions
+ * bits[] and applies to each of them its nrot (<= QJ_ROT_MAX_STEPS) matrices
+ * in order, after loading through `load`:
+ *   QJ_LOAD_PLAIN    x = in (in == out allowed)
+ *   QJ_LOAD_LADDER   x = CNOT ladder applied to in, x[k] = in[k ^ (k >> 1)]
+ *   QJ_LOAD_UNLADDER x = ladder undone, x[k] = in[P(k)], bit j of P(k) the
+ *                    parity of the bits >= j of k
+ *   QJ_LOAD_ZERO     x = |0...0> (in not read)
+ * Permuted loads need in != out.  With lam_in/lam_out set the pass runs
+ * backwards (adjoint method) on psi and lam: for step j = nrot-1..0 of each
+ * bit it adds Im<lam|G|psi> (G = X 1, Y 2, Z 3 from gens[bit * nrot + j],
+ * 0 = no gradient) into partials[((bit * nrot + j) * nrows + b) * ncols + c]
+ * (c < ncols, the workgroups per state) in FP64, then applies the step's
+ * U^+ to both.  For exp(-i theta G / 2) and lam = U_{>k}^+ H psi_L this is
+ * d<psi_L|H|psi_L>/d theta. */
+#define QJ_ROT_MAX_BITS 5
+#define QJ_ROT_MAX_ADJ_BITS 4 /* with lam: psi and lam groups in registers */
+#define QJ_ROT_MAX_STEPS 3
+#define QJ_LOAD_PLAIN 0
+#define QJ_LOAD_LADDER 1
+#define QJ_LOAD_UNLADDER 2
+#define QJ_LOAD_ZERO 3
+
+int qj_rot_pass(const void* psi_in, void* psi_out, const void* lam_in,
+                void* lam_out, int dtype, int nqubits, uint64_t stride,
+                const int32_t* sel, int nrows, const int32_t* bits, int nbits,
+                int nrot, const int32_t* gens, int load, const void* mats,
+                uint64_t mstride, double* partials, int ncols, void* stream);
+/* Open-chain transverse-field Ising model over the bits,
+ * H = -sum_j Z_j Z_{j+1} - g sum_q X_q.  For each row b: out = H' psi (out
+ * may be NULL: energy only; it must not alias psi) and
+ * partials[b * ncols + c] summing to Re<psi|H' psi>, where H' = H or, with
+ * `ladder` set, L^+ H L for the CNOT ladder L (psi is the state before it). */
+int qj_tfim_batch(const void* psi, void* out, int dtype, int nqubits,
+                  uint64_t stride, const int32_t* sel, int nrows, double g,
+                  int ladder, double* partials, int ncols, void* stream);
+/* out[r] = sum_{c < ncols} in[r * ncols + c] in FP64, fixed order. */
+int qj_reduce_rows(const double* in, uint64_t nrows, uint64_t ncols,
+                   double* out, void* stream);
+
 /* ---- measurement ---- */
 
 /* Collapse onto outcome `result` of the measured bits `qubits_kernel`

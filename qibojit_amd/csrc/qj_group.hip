@@ -1,0 +1,439 @@
+// Layered variational circuits: batched forward evolution, TFIM energies and
+// adjoint-method gradients for circuits made of single-qubit rotation layers
+// and CNOT ladders (the hardware-efficient ansatz).
+//
+// Rotation pass.  A pass owns k <= 5 bit positions (ascending).  Each thread
+// holds the 2^k amplitudes of one group (the other bits fixed) in registers
+// and applies, on every local bit, a short sequence of 2x2 matrices; local
+// bits are always compile-time indices, so the register arrays never spill.
+// The ladder CNOT(q, q+1) for q = 0..n-2 is the GF(2)-linear permutation
+// k -> P(k) with bit j of P(k) = parity of the bits >= j of k (qubit q is
+// bit n-1-q); a pass can apply it, or its inverse, while loading:
+//   QJ_LOAD_LADDER:   x[k] = in[k ^ (k >> 1)]            (= in[P^-1(k)])
+//   QJ_LOAD_UNLADDER: x[k] = in[P(k)]
+//   QJ_LOAD_ZERO:     x = |0...0>, `in` is not read.
+// Both permutations keep 64-amplitude aligned blocks together, so permuted
+// loads stay line-coalesced.
+//
+// Adjoint mode (lam != NULL) walks the sequence backwards on psi and lam: for
+// step j = nrot-1..0 on each local bit it first accumulates, in FP64,
+// Im<lam|G|psi> (G = X, Y or Z of that step) -- the derivative of
+// <psi_L|H|psi_L> by the angle of a rotation exp(-i theta G / 2) when lam is
+// U_{>k}^+ H psi_L -- then applies the step's U^+ to both states.
+//
+// Batches: launch row b works on state s = sel ? sel[b] : b at s * stride,
+// with its matrices at mats + s * mstride.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "qj_common.h"
+
+namespace qj {
+
+constexpr int kRotBlock = 256;
+constexpr int kRotMaxCols = 65535;
+
+struct RotArgs {
+  Ins ins;          // group positions, ascending, for insert_bits
+  u64 off[32];      // offset of register r (bit A of r <-> position pos[A])
+  int gen[QJ_ROT_MAX_BITS][QJ_ROT_MAX_STEPS];  // generator per step, 0 = none
+  int nrot;         // steps per bit
+  int load;         // QJ_LOAD_*
+  u64 stride;       // amplitudes between states
+  u64 mstride;      // complex matrix entries between states' matrices
+  int nsel;         // launch rows
+};
+
+__device__ __forceinline__ u64 suffix_parity(u64 k) {
+  k ^= k >> 1;
+  k ^= k >> 2;
+  k ^= k >> 4;
+  k ^= k >> 8;
+  k ^= k >> 16;
+  k ^= k >> 32;
+  return k;
+}
+
+__device__ __forceinline__ u64 load_index(u64 k, int load) {
+  if (load == QJ_LOAD_LADDER) return k ^ (k >> 1);
+  if (load == QJ_LOAD_UNLADDER) return suffix_parity(k);
+  return k;
+}
+
+template <typename R>
+__device__ __forceinline__ cplx<R> cconj(cplx<R> a) {
+  return mk(a.re, -a.im);
+}
+
+// Applies the 2x2 matrix m (row-major; its conjugate transpose if DAG) to
+// local bit A of the 2^K registers x.
+template <typename R, int K, int A, bool DAG>
+__device__ __forceinline__ void rot_apply(cplx<R> (&x)[1 << K], const cplx<R>* m) {
+  const cplx<R> m00 = DAG ? cconj(m[0]) : m[0];
+  const cplx<R> m01 = DAG ? cconj(m[2]) : m[1];
+  const cplx<R> m10 = DAG ? cconj(m[1]) : m[2];
+  const cplx<R> m11 = DAG ? cconj(m[3]) : m[3];
+#pragma unroll
+  for (int r = 0; r < (1 << K); ++r) {
+    if (r & (1 << A)) continue;
+    const cplx<R> a = x[r], b = x[r | (1 << A)];
+    x[r] = mac2(m00, a, m01, b);
+    x[r | (1 << A)] = mac2(m10, a, m11, b);
+  }
+}
+
+// Im <lam|G|psi> over the group for G = X (1), Y (2) or Z (3) on local bit A.
+template <typename R, int K, int A>
+__device__ __forceinline__ double rot_gen(const cplx<R> (&p)[1 << K], const cplx<R> (&l)[1 << K], int gen) {
+  double acc = 0.0;
+#pragma unroll
+  for (int r = 0; r < (1 << K); ++r) {
+    if (r & (1 << A)) continue;
+    const int s = r | (1 << A);
+    const double p0r = p[r].re, p0i = p[r].im, p1r = p[s].re, p1i = p[s].im;
+    const double l0r = l[r].re, l0i = l[r].im, l1r = l[s].re, l1i = l[s].im;
+    // Im(conj(l0) g0 + conj(l1) g1), (g0, g1) = G (p0, p1)
+    if (gen == 1)  // X: (p1, p0)
+      acc += l0r * p1i - l0i * p1r + l1r * p0i - l1i * p0r;
+    else if (gen == 2)  // Y: (-i p1, i p0)
+      acc += -l0r * p1r - l0i * p1i + l1r * p0r + l1i * p0i;
+    else  // Z: (p0, -p1)
+      acc += l0r * p0i - l0i * p0r - l1r * p1i + l1i * p1r;
+  }
+  return acc;
+}
+
+template <typename R, int K, bool ADJ>
+struct RotBit {
+  // Runs the step sequence of local bit A (and recurses to A + 1).
+  template <int A>
+  static __device__ __forceinline__ void run(cplx<R> (&x)[1 << K], cplx<R> (&y)[1 << K], const cplx<R>* m,
+                                             const RotArgs& a, double (&acc)[K * QJ_ROT_MAX_STEPS]) {
+    if constexpr (A < K) {
+      if (!ADJ) {
+#pragma unroll
+        for (int j = 0; j < QJ_ROT_MAX_STEPS; ++j)
+          if (j < a.nrot) rot_apply<R, K, A, false>(x, m + (A * a.nrot + j) * 4);
+      } else {
+#pragma unroll
+        for (int jj = QJ_ROT_MAX_STEPS - 1; jj >= 0; --jj) {
+          if (jj < a.nrot) {
+            const int g = a.gen[A][jj];
+            if (g) acc[A * QJ_ROT_MAX_STEPS + jj] += rot_gen<R, K, A>(x, y, g);
+            rot_apply<R, K, A, true>(x, m + (A * a.nrot + jj) * 4);
+            rot_apply<R, K, A, true>(y, m + (A * a.nrot + jj) * 4);
+          }
+        }
+      }
+      run<A + 1>(x, y, m, a, acc);
+    }
+  }
+};
+
+__device__ __forceinline__ double rot_block_sum(double v, double* wsum) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) wsum[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < kRotBlock / 64; ++i) t += wsum[i];
+  __syncthreads();
+  return t;
+}
+
+// Register r of the group whose first index is `base`: load from `in`
+// through the pass's load permutation.  V = 2 pairs (r, r+1) differ in
+// position 0; both permutations map an aligned pair onto an aligned pair,
+// swapped when the image of the even index is odd.
+template <typename R, int K, int V>
+__device__ __forceinline__ void rot_load(cplx<R> (&x)[1 << K], const cplx<R>* in, u64 base, const RotArgs& a) {
+  if (a.load == QJ_LOAD_ZERO) {
+#pragma unroll
+    for (int r = 0; r < (1 << K); ++r) x[r] = mk<R>(R(base + a.off[r] == 0 ? 1 : 0), R(0));
+    return;
+  }
+#pragma unroll
+  for (int r = 0; r < (1 << K); r += V) {
+    const u64 src = load_index(base + a.off[r], a.load);
+    if (V == 2) {
+      const cvec<R, 2> v = ldv<R, 2>(in + (src & ~u64(1)));
+      const bool sw = src & 1;
+      x[r] = sw ? v.v[1] : v.v[0];
+      x[r + 1] = sw ? v.v[0] : v.v[1];
+    } else {
+      x[r] = in[src];
+    }
+  }
+}
+
+template <typename R, int K, int V>
+__device__ __forceinline__ void rot_store(const cplx<R> (&x)[1 << K], cplx<R>* out, u64 base, const RotArgs& a) {
+#pragma unroll
+  for (int r = 0; r < (1 << K); r += V) {
+    if (V == 2) {
+      cvec<R, 2> v;
+      v.v[0] = x[r];
+      v.v[1] = x[r + 1];
+      stv<R, 2>(out + base + a.off[r], v);
+    } else {
+      out[base + a.off[r]] = x[r];
+    }
+  }
+}
+
+template <typename R, int K, bool ADJ, int V>
+__global__ void __launch_bounds__(kRotBlock)
+    k_rot(const cplx<R>* psi_in, cplx<R>* psi_out, const cplx<R>* lam_in, cplx<R>* lam_out,
+          const int32_t* __restrict__ sel, const cplx<R>* __restrict__ mats, double* __restrict__ partials,
+          u64 ngroups, RotArgs a) {
+  __shared__ double wsum[kRotBlock / 64];
+  const int b = blockIdx.y;
+  const u64 s = sel ? (u64)sel[b] : (u64)b;
+  const cplx<R>* pin = psi_in + s * a.stride;
+  cplx<R>* pout = psi_out + s * a.stride;
+  const cplx<R>* lin = ADJ ? lam_in + s * a.stride : nullptr;
+  cplx<R>* lout = ADJ ? lam_out + s * a.stride : nullptr;
+  const cplx<R>* m = mats + s * a.mstride;
+  double acc[K * QJ_ROT_MAX_STEPS];
+#pragma unroll
+  for (int i = 0; i < K * QJ_ROT_MAX_STEPS; ++i) acc[i] = 0.0;
+  const u64 step = (u64)gridDim.x * kRotBlock;
+  for (u64 g = (u64)blockIdx.x * kRotBlock + threadIdx.x; g < ngroups; g += step) {
+    const u64 base = insert_bits(g, a.ins);
+    cplx<R> x[1 << K], y[1 << K];
+    rot_load<R, K, V>(x, pin, base, a);
+    if (ADJ) rot_load<R, K, V>(y, lin, base, a);
+    RotBit<R, K, ADJ>::template run<0>(x, y, m, a, acc);
+    rot_store<R, K, V>(x, pout, base, a);
+    if (ADJ) rot_store<R, K, V>(y, lout, base, a);
+  }
+  if (ADJ) {
+    const u64 cols = gridDim.x, rows = (u64)a.nsel;
+#pragma unroll
+    for (int A = 0; A < K; ++A)
+#pragma unroll
+      for (int j = 0; j < QJ_ROT_MAX_STEPS; ++j)
+        if (j < a.nrot && a.gen[A][j]) {
+          const double t = rot_block_sum(acc[A * QJ_ROT_MAX_STEPS + j], wsum);
+          if (threadIdx.x == 0) partials[((u64)(A * a.nrot + j) * rows + b) * cols + blockIdx.x] = t;
+        }
+  }
+}
+
+// ---------------------------------------------------------------- TFIM --
+// For H = -sum_{j} Z_j Z_{j+1} - g sum_q X_q (open chain over bits), row b:
+// out = H' psi and partials[b][block] of Re<psi|H' psi>, where H' = H, or
+// with `ladder` set H' = L^+ H L for the CNOT ladder L (psi is the state
+// before the ladder): there the diagonal reads bits of P(k) and X on bit q
+// flips bits q and q-1.  out == NULL computes the energy only.
+template <typename R>
+__global__ void __launch_bounds__(kRotBlock)
+    k_tfim_batch(const cplx<R>* psi, cplx<R>* out, const int32_t* __restrict__ sel, u64 stride, int nq, double g,
+                 int ladder, double* __restrict__ partials) {
+  __shared__ double wsum[kRotBlock / 64];
+  const u64 n = u64(1) << nq;
+  const int b = blockIdx.y;
+  const u64 s = sel ? (u64)sel[b] : (u64)b;
+  const cplx<R>* p = psi + s * stride;
+  cplx<R>* o = out ? out + s * stride : nullptr;
+  const u64 zzmask = (u64(1) << (nq - 1)) - 1;
+  double acc = 0.0;
+  const u64 step = (u64)gridDim.x * kRotBlock;
+  for (u64 k = (u64)blockIdx.x * kRotBlock + threadIdx.x; k < n; k += step) {
+    const cplx<R> x = p[k];
+    const u64 kd = ladder ? suffix_parity(k) : k;
+    const int pc = __popcll((kd ^ (kd >> 1)) & zzmask);
+    const double zz = (double)(nq - 1) - 2.0 * (double)pc;
+    const double xr = x.re, xi = x.im;
+    if (o) {
+      double re = -zz * xr, im = -zz * xi;
+      for (int q = 0; q < nq; ++q) {
+        const u64 f = (u64(1) << q) | (ladder && q ? u64(1) << (q - 1) : 0);
+        const cplx<R> y = p[k ^ f];
+        re -= g * (double)y.re;
+        im -= g * (double)y.im;
+      }
+      o[k] = mk<R>((R)re, (R)im);
+      acc += xr * re + xi * im;
+    } else {
+      // each unordered pair {k, k^f} once, from its lower index
+      double xs = 0.0;
+      for (int q = 0; q < nq; ++q) {
+        const u64 f = (u64(1) << q) | (ladder && q ? u64(1) << (q - 1) : 0);
+        if (!(k & (u64(1) << q))) {
+          const cplx<R> y = p[k ^ f];
+          xs += xr * (double)y.re + xi * (double)y.im;
+        }
+      }
+      acc += -zz * (xr * xr + xi * xi) - 2.0 * g * xs;
+    }
+  }
+  acc = rot_block_sum(acc, wsum);
+  if (threadIdx.x == 0) partials[(u64)b * gridDim.x + blockIdx.x] = acc;
+}
+
+// out[r] = sum_c in[r * ncols + c], one workgroup per row, fixed order.
+__global__ void __launch_bounds__(kRotBlock) k_reduce_rows(const double* __restrict__ in, u64 ncols,
+                                                           double* __restrict__ out) {
+  __shared__ double wsum[kRotBlock / 64];
+  const u64 r = blockIdx.x;
+  double acc = 0.0;
+  for (u64 c = threadIdx.x; c < ncols; c += kRotBlock) acc += in[r * ncols + c];
+  acc = rot_block_sum(acc, wsum);
+  if (threadIdx.x == 0) out[r] = acc;
+}
+
+template <typename R, int K, bool ADJ>
+static void launch_rot(bool vec, dim3 gd, hipStream_t st, const cplx<R>* pin, cplx<R>* pout, const cplx<R>* lin,
+                       cplx<R>* lout, const int32_t* sel, const cplx<R>* mats, double* partials, u64 ngroups,
+                       const RotArgs& a) {
+  if (vec)
+    k_rot<R, K, ADJ, 2><<<gd, dim3(kRotBlock), 0, st>>>(pin, pout, lin, lout, sel, mats, partials, ngroups, a);
+  else
+    k_rot<R, K, ADJ, 1><<<gd, dim3(kRotBlock), 0, st>>>(pin, pout, lin, lout, sel, mats, partials, ngroups, a);
+}
+
+template <typename R, bool ADJ>
+static void launch_rot_k(int k, bool vec, dim3 gd, hipStream_t st, const cplx<R>* pin, cplx<R>* pout,
+                         const cplx<R>* lin, cplx<R>* lout, const int32_t* sel, const cplx<R>* mats,
+                         double* partials, u64 ngroups, const RotArgs& a) {
+  switch (k) {
+    case 1: launch_rot<R, 1, ADJ>(vec, gd, st, pin, pout, lin, lout, sel, mats, partials, ngroups, a); break;
+    case 2: launch_rot<R, 2, ADJ>(vec, gd, st, pin, pout, lin, lout, sel, mats, partials, ngroups, a); break;
+    case 3: launch_rot<R, 3, ADJ>(vec, gd, st, pin, pout, lin, lout, sel, mats, partials, ngroups, a); break;
+    case 4: launch_rot<R, 4, ADJ>(vec, gd, st, pin, pout, lin, lout, sel, mats, partials, ngroups, a); break;
+    default:
+      // five bits of psi and lam exceed the register file in adjoint mode
+      if constexpr (!ADJ) launch_rot<R, 5, ADJ>(vec, gd, st, pin, pout, lin, lout, sel, mats, partials, ngroups, a);
+      break;
+  }
+}
+
+static int check_rows(int nrows, const int32_t* sel, uint64_t stride, int nqubits, const char* what) {
+  if (nrows < 1 || nrows > 65535) return set_error(QJ_EVALUE, "%s: bad number of states %d", what, nrows);
+  if (stride < (u64(1) << nqubits) && !(sel == nullptr && nrows == 1))
+    return set_error(QJ_EVALUE, "%s: state stride %llu below 2^nqubits", what, (unsigned long long)stride);
+  return QJ_OK;
+}
+
+template <typename R>
+static int rot_pass_impl(const void* psi_in, void* psi_out, const void* lam_in, void* lam_out, int nqubits,
+                         uint64_t stride, const int32_t* sel, int nrows, const int32_t* bits, int k, int nrot,
+                         const int32_t* gens, int load, const void* mats, uint64_t mstride, double* partials,
+                         int ncols, hipStream_t st) {
+  const char* what = "rot_pass";
+  if (nqubits < 1 || nqubits > 40) return set_error(QJ_EVALUE, "%s: bad nqubits %d", what, nqubits);
+  int rc = check_rows(nrows, sel, stride, nqubits, what);
+  if (rc) return rc;
+  if (k < 1 || k > QJ_ROT_MAX_BITS || k > nqubits) return set_error(QJ_EVALUE, "%s: bad group size %d", what, k);
+  if (nrot < 1 || nrot > QJ_ROT_MAX_STEPS) return set_error(QJ_EVALUE, "%s: bad step count %d", what, nrot);
+  if (mstride < (u64)(k * nrot * 4) && !(sel == nullptr && nrows == 1))
+    return set_error(QJ_EVALUE, "%s: matrix stride below %d", what, k * nrot * 4);
+  if (load < QJ_LOAD_PLAIN || load > QJ_LOAD_ZERO) return set_error(QJ_EVALUE, "%s: bad load mode %d", what, load);
+  if (ncols < 1 || ncols > kRotMaxCols) return set_error(QJ_EVALUE, "%s: bad column count %d", what, ncols);
+  const bool adj = lam_in != nullptr;
+  if (adj != (lam_out != nullptr)) return set_error(QJ_EVALUE, "%s: lam_in and lam_out go together", what);
+  if (adj && k > QJ_ROT_MAX_ADJ_BITS) return set_error(QJ_EVALUE, "%s: adjoint groups hold at most %d bits", what, QJ_ROT_MAX_ADJ_BITS);
+  if (load == QJ_LOAD_ZERO && adj) return set_error(QJ_EVALUE, "%s: zero load is forward only", what);
+  if ((load == QJ_LOAD_LADDER || load == QJ_LOAD_UNLADDER) && (psi_in == psi_out || (adj && lam_in == lam_out)))
+    return set_error(QJ_EVALUE, "%s: a permuted load needs distinct input and output", what);
+  for (int j = 0; j < k; ++j) {
+    if (bits[j] < 0 || bits[j] >= nqubits) return set_error(QJ_EVALUE, "%s: bit %d out of range", what, bits[j]);
+    if (j && bits[j] <= bits[j - 1]) return set_error(QJ_EVALUE, "%s: bits must be ascending and unique", what);
+  }
+  RotArgs a;
+  std::memset(&a, 0, sizeof(a));
+  bool any_gen = false;
+  if (gens) {
+    for (int A = 0; A < k; ++A)
+      for (int j = 0; j < nrot; ++j) {
+        const int g = gens[A * nrot + j];
+        if (g < 0 || g > 3) return set_error(QJ_EVALUE, "%s: bad generator %d", what, g);
+        a.gen[A][j] = g;
+        any_gen |= g != 0;
+      }
+  }
+  if (adj && any_gen && !partials) return set_error(QJ_EVALUE, "%s: gradients need a partials buffer", what);
+  a.ins.npos = k;
+  for (int j = 0; j < k; ++j) a.ins.pos[j] = bits[j];
+  for (int r = 0; r < (1 << k); ++r) {
+    u64 o = 0;
+    for (int j = 0; j < k; ++j)
+      if ((r >> j) & 1) o |= u64(1) << bits[j];
+    a.off[r] = o;
+  }
+  a.nrot = nrot;
+  a.load = load;
+  a.stride = stride;
+  a.mstride = mstride;
+  a.nsel = nrows;
+  const u64 ngroups = (u64(1) << nqubits) >> k;
+  dim3 gd((unsigned)ncols, (unsigned)nrows);
+  const bool vec = sizeof(R) == 4 && bits[0] == 0;
+  const cplx<R>* m = static_cast<const cplx<R>*>(mats);
+  if (adj)
+    launch_rot_k<R, true>(k, vec, gd, st, (const cplx<R>*)psi_in, (cplx<R>*)psi_out, (const cplx<R>*)lam_in,
+                          (cplx<R>*)lam_out, sel, m, partials, ngroups, a);
+  else
+    launch_rot_k<R, false>(k, vec, gd, st, (const cplx<R>*)psi_in, (cplx<R>*)psi_out, nullptr, nullptr, sel, m,
+                           nullptr, ngroups, a);
+  return check_launch("rot_pass launch");
+}
+
+}  // namespace qj
+
+using namespace qj;
+
+extern "C" {
+
+int qj_rot_pass(const void* psi_in, void* psi_out, const void* lam_in, void* lam_out, int dtype, int nqubits,
+                uint64_t stride, const int32_t* sel, int nrows, const int32_t* bits, int nbits, int nrot,
+                const int32_t* gens, int load, const void* mats, uint64_t mstride, double* partials, int ncols,
+                void* stream) {
+  if ((!psi_in && load != QJ_LOAD_ZERO) || !psi_out || !bits || !mats)
+    return set_error(QJ_EVALUE, "rot_pass: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == QJ_COMPLEX64)
+    return rot_pass_impl<float>(psi_in, psi_out, lam_in, lam_out, nqubits, stride, sel, nrows, bits, nbits, nrot,
+                                gens, load, mats, mstride, partials, ncols, st);
+  if (dtype == QJ_COMPLEX128)
+    return rot_pass_impl<double>(psi_in, psi_out, lam_in, lam_out, nqubits, stride, sel, nrows, bits, nbits, nrot,
+                                 gens, load, mats, mstride, partials, ncols, st);
+  return set_error(QJ_EVALUE, "rot_pass: dtype must be complex64/complex128");
+}
+
+int qj_tfim_batch(const void* psi, void* out, int dtype, int nqubits, uint64_t stride, const int32_t* sel,
+                  int nrows, double g, int ladder, double* partials, int ncols, void* stream) {
+  if (!psi || !partials) return set_error(QJ_EVALUE, "tfim_batch: null pointer");
+  if (nqubits < 2 || nqubits > 40) return set_error(QJ_EVALUE, "tfim_batch: nqubits %d out of range", nqubits);
+  int rc = check_rows(nrows, sel, stride, nqubits, "tfim_batch");
+  if (rc) return rc;
+  if (ncols < 1 || ncols > kRotMaxCols) return set_error(QJ_EVALUE, "tfim_batch: bad column count %d", ncols);
+  if (psi == out) return set_error(QJ_EVALUE, "tfim_batch: out must not alias psi");
+  hipStream_t st = (hipStream_t)stream;
+  dim3 gd((unsigned)ncols, (unsigned)nrows);
+  if (dtype == QJ_COMPLEX64)
+    k_tfim_batch<float><<<gd, dim3(kRotBlock), 0, st>>>((const cplx<float>*)psi, (cplx<float>*)out, sel, stride,
+                                                        nqubits, g, ladder != 0, partials);
+  else if (dtype == QJ_COMPLEX128)
+    k_tfim_batch<double><<<gd, dim3(kRotBlock), 0, st>>>((const cplx<double>*)psi, (cplx<double>*)out, sel,
+                                                         stride, nqubits, g, ladder != 0, partials);
+  else
+    return set_error(QJ_EVALUE, "tfim_batch: dtype must be complex64/complex128");
+  return check_launch("tfim_batch launch");
+}
+
+int qj_reduce_rows(const double* in, uint64_t nrows, uint64_t ncols, double* out, void* stream) {
+  if (!in || !out) return set_error(QJ_EVALUE, "reduce_rows: null pointer");
+  if (nrows == 0) return QJ_OK;
+  if (nrows > 0x7fffffffull) return set_error(QJ_EVALUE, "reduce_rows: too many rows");
+  if (ncols == 0) return set_error(QJ_EVALUE, "reduce_rows: no columns");
+  k_reduce_rows<<<dim3((unsigned)nrows), dim3(kRotBlock), 0, (hipStream_t)stream>>>(in, ncols, out);
+  return check_launch("reduce_rows launch");
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+"""Batched ansatz engine, rotation passes, TFIM kernels and adjoint gradients
+on the GPU, against the numpy oracle (parameter shift, finite differences,
+closed forms)."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import ansatz_ref, einsum_ref
+from oracle.gate_matrices import haar_unitary, random_state
+
+pytestmark = pytest.mark.gpu
+
+# amplitude / energy / gradient tolerances per precision
+AMP = {"complex64": 1e-5, "complex128": 1e-12}
+ENERGY = {"complex64": 2e-5, "complex128": 1e-11}
+GRAD = {"complex64": 2e-5, "complex128": 1e-10}
+
+
+@pytest.fixture(scope="module")
+def torch_mod(hip_backend):  # the backend fixture proves the device and library
+    import torch
+
+    return torch
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("n,layers,batch,kf", [(6, 2, 3, 5), (7, 3, 2, 3), (9, 2, 4, 4), (5, 1, 1, 1), (10, 2, 2, 2)])
+def test_engine_states_and_energies(torch_mod, dtype, rng, n, layers, batch, kf):
+    from qibojit_amd.ansatz import AnsatzEngine, HardwareEfficientAnsatz
+
+    ansatz = HardwareEfficientAnsatz(n, layers)
+    eng = AnsatzEngine(ansatz, batch, dtype=dtype, g=0.8, k_forward=kf)
+    p = ansatz.random_parameters(rng, batch)
+    energies = _np(eng.energies(p))
+    states = _np(eng.final_states())
+    for b in range(batch):
+        want = ansatz_ref.hea_state(n, layers, p[b])
+        assert np.max(np.abs(states[b] - want)) < AMP[dtype]
+        assert abs(energies[b] - einsum_ref.tfim_energy(want, 0.8)) < ENERGY[dtype]
+
+
+@pytest.mark.parametrize("n,layers,batch,ka", [(6, 2, 2, 4), (7, 2, 3, 3), (5, 3, 1, 2), (8, 1, 2, 1)])
+def test_engine_gradients_match_parameter_shift(torch_mod, dtype, rng, n, layers, batch, ka):
+    from qibojit_amd.ansatz import AnsatzEngine, HardwareEfficientAnsatz
+
+    ansatz = HardwareEfficientAnsatz(n, layers)
+    eng = AnsatzEngine(ansatz, batch, dtype=dtype, g=1.0, k_adjoint=ka)
+    p = ansatz.random_parameters(rng, batch)
+    energies, grads = eng.gradients(p)
+    energies, grads = _np(energies), _np(grads)
+    for b in range(batch):
+        ps = ansatz_ref.parameter_shift(n, layers, p[b], 1.0)
+        assert np.max(np.abs(grads[b] - ps)) < GRAD[dtype], (b, np.max(np.abs(grads[b] - ps)))
+        assert abs(energies[b] - ansatz_ref.hea_energy(n, layers, p[b], 1.0)) < ENERGY[dtype]
+    # walking back through the circuit returns psi to |0...0>
+    psi0 = _np(eng.psi[eng.cur])
+    want = np.zeros_like(psi0)
+    want[:, 0] = 1
+    assert np.max(np.abs(psi0 - want)) < 10 * AMP[dtype]
+
+
+def test_gradients_after_forward_reuse_the_states(torch_mod, rng):
+    from qibojit_amd.ansatz import AnsatzEngine, HardwareEfficientAnsatz
+
+    ansatz = HardwareEfficientAnsatz(6, 2)
+    eng = AnsatzEngine(ansatz, 2, dtype="complex128")
+    p = ansatz.random_parameters(rng, 2)
+    e1 = _np(eng.energies(p))
+    e2, g2 = eng.gradients(p)
+    np.testing.assert_allclose(_np(e2), e1, atol=1e-13)
+    # a second gradient call (forward states consumed) recomputes the forward
+    e3, g3 = eng.gradients(p)
+    np.testing.assert_allclose(_np(g3), _np(g2), atol=1e-13)
+    np.testing.assert_allclose(_np(e3), e1, atol=1e-13)
+
+
+def test_gradients_match_finite_differences(torch_mod, rng):
+    from qibojit_amd.ansatz import AnsatzEngine, HardwareEfficientAnsatz
+
+    n, layers = 6, 2
+    ansatz = HardwareEfficientAnsatz(n, layers)
+    p = ansatz.random_parameters(rng, 1)
+    _, grads = AnsatzEngine(ansatz, 1, dtype="complex128").gradients(p)
+    # central differences in complex128 through the engine itself, h = 1e-4
+    h = 1e-4
+    shifted = np.repeat(p, 2 * ansatz.nparams, axis=0)
+    for i in range(ansatz.nparams):
+        shifted[2 * i, i] += h
+        shifted[2 * i + 1, i] -= h
+    e = _np(AnsatzEngine(ansatz, shifted.shape[0], dtype="complex128").energies(shifted))
+    fd = (e[0::2] - e[1::2]) / (2 * h)
+    np.testing.assert_allclose(_np(grads)[0], fd, atol=1e-7)
+
+
+def test_plain_adjoint_matches_parameter_shift(backend, rng):
+    from qibojit_amd.ansatz import HardwareEfficientAnsatz, ansatz_gradient_order, plain_adjoint
+
+    n, layers = 6, 2
+    ansatz = HardwareEfficientAnsatz(n, layers)
+    p = ansatz.random_parameters(rng)
+    energy, grads = plain_adjoint(backend, ansatz.circuit(p), g=1.0)
+    ps = ansatz_ref.parameter_shift(n, layers, p, 1.0)
+    np.testing.assert_allclose(grads, ps[ansatz_gradient_order(ansatz)], atol=1e-10)
+    assert energy == pytest.approx(ansatz_ref.hea_energy(n, layers, p, 1.0), abs=1e-11)
+
+
+def test_autograd_function(torch_mod, rng):
+    from qibojit_amd.ansatz import AnsatzEnergy, AnsatzEngine, HardwareEfficientAnsatz
+
+    torch = torch_mod
+    n, layers, batch = 5, 2, 3
+    ansatz = HardwareEfficientAnsatz(n, layers)
+    eng = AnsatzEngine(ansatz, batch, dtype="complex128")
+    p = torch.tensor(ansatz.random_parameters(rng, batch), dtype=torch.float64, requires_grad=True)
+    weights = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64)
+    loss = (AnsatzEnergy.apply(p, eng).cpu() * weights).sum()
+    loss.backward()
+    pn = p.detach().numpy()
+    for b in range(batch):
+        ps = ansatz_ref.parameter_shift(n, layers, pn[b], 1.0)
+        np.testing.assert_allclose(p.grad[b].numpy(), weights[b].item() * ps, atol=1e-10)
+
+
+def _rot_call(psi_in, psi_out, lam_in, lam_out, code, n, stride, sel, nrows, bits, nrot, gens, load, mats, mstride,
+              partials, ncols):
+    from qibojit_amd import _native
+
+    bits_ptr, _keep, nbits = _native.i32_array(bits)
+    gptr = None
+    if gens is not None:
+        gptr, _gkeep, _ = _native.i32_array(gens)
+
+    def vp(t):
+        return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+    _native.call("qj_rot_pass", vp(psi_in), vp(psi_out), vp(lam_in), vp(lam_out), code, n, stride, vp(sel), nrows,
+                 bits_ptr, nbits, nrot, gptr, load, vp(mats), mstride, vp(partials), ncols, None)
+
+
+@pytest.mark.parametrize("load", [0, 1, 2])
+@pytest.mark.parametrize("bits", [(0, 4, 7), (1, 2), (3,), (0, 1, 2, 3, 4), (2, 5, 6, 8)])
+def test_rot_pass_forward_against_numpy(torch_mod, dtype, rng, load, bits):
+    """Every load mode, with and without bit 0, several steps per bit, on a
+    selected subset of a batch (unselected states stay untouched)."""
+    from qibojit_amd import _native
+
+    torch = torch_mod
+    n, nstates, nrot = 9, 4, 2
+    code = _native.QJ_COMPLEX64 if dtype == "complex64" else _native.QJ_COMPLEX128
+    tdt = torch.complex64 if dtype == "complex64" else torch.complex128
+    host = np.stack([random_state(n, rng) for _ in range(nstates)]).astype(dtype)
+    k = len(bits)
+    mats = np.stack([np.stack([[haar_unitary(2, rng) for _ in range(nrot)] for _ in range(k)]) for _ in range(nstates)])
+    sel = [3, 1]
+    src = torch.from_numpy(host).cuda()
+    dst = src.clone() if load == 0 else torch.zeros_like(src)
+    _rot_call(src, dst, None, None, code, n, 1 << n, torch.tensor(sel, dtype=torch.int32).cuda(), len(sel),
+              list(bits), nrot, None, load, torch.from_numpy(mats.astype(dtype).reshape(nstates, -1)).cuda(),
+              k * nrot * 4, None, 3)
+    got = _np(dst)
+    idx = np.arange(1 << n)
+    for s in range(nstates):
+        if s not in sel:
+            np.testing.assert_array_equal(got[s], host[s] if load == 0 else 0)
+            continue
+        x = host[s].astype(np.complex128)
+        if load == 1:
+            x = x[idx ^ (idx >> 1)]
+        elif load == 2:
+            x = x[np.argsort(idx ^ (idx >> 1))]
+        for a, bit in enumerate(bits):
+            for j in range(nrot):
+                x = einsum_ref.apply_gate(x, mats[s, a, j], [n - 1 - bit], nqubits=n)
+        assert np.max(np.abs(got[s] - x)) < AMP[dtype]
+
+
+def test_rot_pass_zero_load_builds_product_state(torch_mod, rng):
+    from qibojit_amd import _native
+
+    torch = torch_mod
+    n, bits = 6, (0, 2, 5)
+    mats = np.stack([haar_unitary(2, rng) for _ in bits])[None, :, None]  # (1, k, 1, 2, 2)
+    dst = torch.full((1 << n,), 7.0 + 7.0j, dtype=torch.complex128).cuda()
+    _rot_call(None, dst, None, None, _native.QJ_COMPLEX128, n, 1 << n, None, 1, list(bits), 1, None, 3,
+              torch.from_numpy(mats.reshape(-1)).cuda(), 12, None, 1)
+    x = np.zeros(1 << n, dtype=np.complex128)
+    x[0] = 1
+    for a, bit in enumerate(bits):
+        x = einsum_ref.apply_gate(x, mats[0, a, 0], [n - 1 - bit], nqubits=n)
+    np.testing.assert_allclose(_np(dst), x, atol=1e-14)
+
+
+@pytest.mark.parametrize("ladder", [0, 1])
+def test_tfim_batch_energy_and_apply(torch_mod, dtype, rng, ladder):
+    from qibojit_amd import _native
+
+    torch = torch_mod
+    n, nstates, g = 8, 3, 0.6
+    code = _native.QJ_COMPLEX64 if dtype == "complex64" else _native.QJ_COMPLEX128
+    host = np.stack([random_state(n, rng) for _ in range(nstates)]).astype(dtype)
+    psi = torch.from_numpy(host).cuda()
+    out = torch.zeros_like(psi)
+    ncols = 2
+    part = torch.zeros((nstates, ncols), dtype=torch.float64).cuda()
+    part2 = torch.zeros_like(part)
+    sums = torch.zeros(nstates, dtype=torch.float64).cuda()
+    sums2 = torch.zeros_like(sums)
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    _native.call("qj_tfim_batch", vp(psi), vp(out), code, n, 1 << n, None, nstates, g, ladder, vp(part), ncols, None)
+    _native.call("qj_tfim_batch", vp(psi), None, code, n, 1 << n, None, nstates, g, ladder, vp(part2), ncols, None)
+    _native.call("qj_reduce_rows", vp(part), nstates, ncols, vp(sums), None)
+    _native.call("qj_reduce_rows", vp(part2), nstates, ncols, vp(sums2), None)
+    h = einsum_ref.tfim_matrix(n, g)
+    perm = ansatz_ref.ladder_permutation(n)
+    inv = np.argsort(perm)
+    for s in range(nstates):
+        x = host[s].astype(np.complex128)
+        if ladder:
+            hx = (h @ x[perm])[inv]  # L^+ H L x
+        else:
+            hx = h @ x
+        e = np.vdot(x, hx).real
+        tol = 50 * AMP[dtype]
+        assert np.max(np.abs(_np(out)[s] - hx)) < tol
+        assert abs(_np(sums)[s] - e) < ENERGY[dtype]
+        assert abs(_np(sums2)[s] - e) < ENERGY[dtype]
