@@ -46,6 +46,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--only-sweep", action="store_true", help="sweep only (for profiler runs)")
     p.add_argument("--copy-probe", action="store_true", help="plain 4 GiB copy (counter scale reference)")
+    p.add_argument("--only-adjoint", action="store_true", help="adjoint-gradient benchmark only")
     return p.parse_args()
 
 
@@ -334,6 +335,104 @@ def run_capacity(backend, n=33):
     return {"nqubits": n, "state_GB": 16 * 2**n / 1e9, "gate": f"H({n // 2})", "ms": ms, "GBps": b / ms * 1e3 / 1e9}
 
 
+def _shifted_energies(ansatz, base, shifts, chunk=64):
+    """Energies (complex128 engine) of ``base`` with each parameter moved by
+    each of ``shifts``: returns array (len(shifts), P)."""
+    import torch
+
+    from qibojit_amd.ansatz import AnsatzEngine
+
+    nparams = ansatz.nparams
+    sets = []
+    for sh in shifts:
+        for i in range(nparams):
+            p = base.copy()
+            p[i] += sh
+            sets.append(p)
+    sets = np.array(sets)
+    out = np.empty(len(sets))
+    eng = AnsatzEngine(ansatz, chunk, dtype="complex128")
+    for s in range(0, len(sets), chunk):
+        part = sets[s:s + chunk]
+        if part.shape[0] < chunk:
+            part = np.concatenate([part, np.repeat(part[-1:], chunk - part.shape[0], axis=0)])
+        out[s:s + chunk] = eng.energies(part).cpu().numpy()[: min(chunk, len(sets) - s)]
+    del eng
+    torch.cuda.empty_cache()
+    return out.reshape(len(shifts), nparams)
+
+
+def run_adjoint(backend, n=24, layers=8, batch=16, steps=3, g=1.0, seed=2024):
+    """Adjoint gradients of <H_TFIM> for an n-qubit, `layers`-layer
+    hardware-efficient ansatz, `batch` parameter sets: the batched engine
+    (complex64 and complex128) against the plain per-gate complex128 path,
+    with gradient errors against complex128 central differences (h = 1e-4)
+    and parameter shift for parameter set 0."""
+    import torch
+
+    from qibojit_amd.ansatz import AnsatzEngine, HardwareEfficientAnsatz, ansatz_gradient_order, plain_adjoint
+
+    ansatz = HardwareEfficientAnsatz(n, layers)
+    rng = np.random.default_rng(seed)
+    params = ansatz.random_parameters(rng, batch)
+    res = {"nqubits": n, "layers": layers, "params": ansatz.nparams, "batch": batch, "g": g}
+    grads = {}
+    for dtype in ("complex64", "complex128"):
+        eng = AnsatzEngine(ansatz, batch, dtype=dtype, g=g)
+        eng.gradients(params)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            energies, gr = eng.gradients(params)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        grads[dtype] = gr.cpu().numpy()
+        res[f"batched_{dtype}"] = {
+            "s_per_batch": dt,
+            "grads_per_s": batch * ansatz.nparams / dt,
+            "k_forward": eng.k_forward,
+            "k_adjoint": eng.k_adjoint,
+            "passes_per_batch": ansatz.nlayers * (len(eng.groups_f) + len(eng.groups_a)) + 1,
+        }
+        res[f"energy0_{dtype}"] = float(energies[0].item())
+        del eng
+        torch.cuda.empty_cache()
+    # plain per-gate complex128 path on parameter set 0
+    backend.set_dtype("complex128")
+    circuit = ansatz.circuit(params[0])
+    plain_adjoint(backend, HardwareEfficientAnsatz(6, 1).circuit(params[0][:12]), g=g)  # warm-up kernels
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e_plain, g_plain = plain_adjoint(backend, circuit, g=g)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    plain = np.empty(ansatz.nparams)
+    plain[ansatz_gradient_order(ansatz)] = g_plain
+    res["plain_complex128"] = {"s_per_circuit": dt, "grads_per_s": ansatz.nparams / dt, "energy": e_plain}
+    # references for parameter set 0
+    t0 = time.perf_counter()
+    h = 1e-4
+    e_fd = _shifted_energies(ansatz, params[0], [h, -h])
+    fd = (e_fd[0] - e_fd[1]) / (2 * h)
+    e_ps = _shifted_energies(ansatz, params[0], [np.pi / 2, -np.pi / 2])
+    ps = 0.5 * (e_ps[0] - e_ps[1])
+    res["reference_seconds"] = time.perf_counter() - t0
+    err = lambda a, b: float(np.max(np.abs(a - b)))  # noqa: E731
+    res["max_grad_error"] = {
+        "batched_c64_vs_fd": err(grads["complex64"][0], fd),
+        "batched_c64_vs_ps": err(grads["complex64"][0], ps),
+        "batched_c128_vs_fd": err(grads["complex128"][0], fd),
+        "batched_c128_vs_ps": err(grads["complex128"][0], ps),
+        "plain_c128_vs_ps": err(plain, ps),
+        "fd_vs_ps": err(fd, ps),
+        "batched_c64_vs_batched_c128_all_sets": err(grads["complex64"], grads["complex128"]),
+    }
+    res["max_abs_grad"] = float(np.max(np.abs(ps)))
+    res["speedup_batched_c64_vs_plain_c128"] = res["batched_complex64"]["grads_per_s"] / res["plain_complex128"]["grads_per_s"]
+    res["speedup_batched_c128_vs_plain_c128"] = res["batched_complex128"]["grads_per_s"] / res["plain_complex128"]["grads_per_s"]
+    return res
+
+
 def cpu_baseline(n):
     """The reference's kernels restated in C/OpenMP, on this host's cores.
 
@@ -397,6 +496,9 @@ def main():
     from qibojit_amd.backends.hip import HipBackend
 
     backend = HipBackend()
+    if args.only_adjoint:
+        print(json.dumps({"adjoint_24q": run_adjoint(backend)}), flush=True)
+        return
     n = args.nqubits
     print(f"# sweep complex128 n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
     main_res = run_sweep(backend, n, "complex128", args.steps, args.warmup)
@@ -414,6 +516,8 @@ def main():
         extras["qft_32q_complex64"] = run_qft(backend, 32, "complex64")
         print("# capacity 33q complex128", file=sys.stderr, flush=True)
         extras["capacity_33q_complex128"] = run_capacity(backend, 33)
+        print("# adjoint gradients 24q", file=sys.stderr, flush=True)
+        extras["adjoint_24q_8layers_batch16"] = run_adjoint(backend)
 
     name, avg_ms, avg_bytes, achieved, launches = dominant_kernel(main_res["rows"])
     roofline = {
