@@ -220,6 +220,17 @@ int qj_rot_pass(const void* psi_in, void* psi_out, const void* lam_in,
 int qj_tfim_batch(const void* psi, void* out, int dtype, int nqubits,
                   uint64_t stride, const int32_t* sel, int nrows, double g,
                   int ladder, double* partials, int ncols, void* stream);
+/* Whole ansatz circuits in LDS, one workgroup per circuit at a time
+ * (persistent: workgroup w takes circuits w, w + nblocks, ...).  desc is a
+ * device array of ncirc x {nqubits, nlayers, offset lo, offset hi}: circuit c
+ * has nlayers layers of [RZ RY fused per bit][ladder] with matrices at
+ * mats + offset, laid out [layer][bit][2x2] (bit p = qubit n-1-p), and
+ * energies[c] receives its TFIM energy <H> (FP64).  nqmax bounds the qubit
+ * count (2^nqmax amplitudes of LDS, at most 64 KiB: 13 qubits complex64,
+ * 12 complex128); circuits outside the bounds get NaN. */
+int qj_ansatz_lds(int dtype, const int32_t* desc, int ncirc, const void* mats,
+                  uint64_t mats_len, double g, double* energies, int nqmax,
+                  int nblocks, void* stream);
 /* out[r] = sum_{c < ncols} in[r * ncols + c] in FP64, fixed order. */
 int qj_reduce_rows(const double* in, uint64_t nrows, uint64_t ncols,
                    double* out, void* stream);

@@ -287,6 +287,143 @@ __global__ void __launch_bounds__(kRotBlock) k_reduce_rows(const double* __restr
   if (threadIdx.x == 0) out[r] = acc;
 }
 
+// ------------------------------------------------ whole circuit in LDS --
+// Persistent kernel for small ansatz circuits: workgroup w runs circuits
+// w, w + gridDim.x, ... entirely in LDS (2^n amplitudes, n <= nqmax) and
+// writes only their energies.  Per layer: rotations on interleaved groups
+// of up to 4 bits (group i = bits i, i+G, i+2G, i+3G), then the ladder as an
+// in-place gather through registers, then after the last layer the TFIM
+// energy in FP64.  desc[c] = {nqubits, nlayers, matrix offset lo, hi}; the
+// circuit's matrices are [layer][bit][2x2] (bit p = qubit n-1-p).  A circuit
+// whose descriptor does not fit (n outside [2, nqmax], no layers, matrices
+// past mats_len) gets a NaN energy and is not run.
+constexpr int kLdsBlock = 512;
+// amplitudes per thread in the ladder gather: 64 KiB of LDS / 512 threads
+template <typename R>
+constexpr int lds_per_thread() {
+  return 65536 / (int)sizeof(cplx<R>) / kLdsBlock;
+}
+
+template <typename R, int K>
+__device__ __forceinline__ void lds_group(cplx<R>* st, int n, int i, int G, const cplx<R>* ml) {
+  int p[K];
+#pragma unroll
+  for (int A = 0; A < K; ++A) p[A] = i + A * G;
+  const unsigned ngroups = 1u << (n - K);
+  for (unsigned g = threadIdx.x; g < ngroups; g += kLdsBlock) {
+    unsigned idx = g;
+#pragma unroll
+    for (int A = 0; A < K; ++A) {
+      const unsigned lo = idx & ((1u << p[A]) - 1);
+      idx = ((idx ^ lo) << 1) | lo;
+    }
+    cplx<R> x[1 << K];
+#pragma unroll
+    for (int r = 0; r < (1 << K); ++r) {
+      unsigned off = 0;
+#pragma unroll
+      for (int A = 0; A < K; ++A)
+        if (r & (1 << A)) off |= 1u << p[A];
+      x[r] = st[idx + off];
+    }
+    if constexpr (K >= 1) rot_apply<R, K, 0, false>(x, ml + p[0] * 4);
+    if constexpr (K >= 2) rot_apply<R, K, (K >= 2 ? 1 : 0), false>(x, ml + p[K >= 2 ? 1 : 0] * 4);
+    if constexpr (K >= 3) rot_apply<R, K, (K >= 3 ? 2 : 0), false>(x, ml + p[K >= 3 ? 2 : 0] * 4);
+    if constexpr (K >= 4) rot_apply<R, K, (K >= 4 ? 3 : 0), false>(x, ml + p[K >= 4 ? 3 : 0] * 4);
+#pragma unroll
+    for (int r = 0; r < (1 << K); ++r) {
+      unsigned off = 0;
+#pragma unroll
+      for (int A = 0; A < K; ++A)
+        if (r & (1 << A)) off |= 1u << p[A];
+      st[idx + off] = x[r];
+    }
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kLdsBlock)
+    k_ansatz_lds(const int32_t* __restrict__ desc, int ncirc, const cplx<R>* __restrict__ mats, u64 mats_len,
+                 double g, double* __restrict__ energies, int nqmax) {
+  // the state is the only LDS (dynamic, exactly 2^nqmax amplitudes); the
+  // energy's wave partials reuse its first bytes once it is no longer read
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cplx<R>* st = reinterpret_cast<cplx<R>*>(lds_raw);
+  double* wsum = reinterpret_cast<double*>(lds_raw);
+  for (int c = blockIdx.x; c < ncirc; c += gridDim.x) {
+    const int n = desc[4 * c], L = desc[4 * c + 1];
+    const u64 moff = (u64)(uint32_t)desc[4 * c + 2] | ((u64)(uint32_t)desc[4 * c + 3] << 32);
+    const bool ok = n >= 2 && n <= nqmax && L >= 1 && moff <= mats_len && (u64)L * n * 4 <= mats_len - moff;
+    if (!ok) {
+      if (threadIdx.x == 0) energies[c] = __builtin_nan("");
+      continue;  // uniform per workgroup
+    }
+    const cplx<R>* m = mats + moff;
+    const unsigned dim = 1u << n;
+    for (unsigned k = threadIdx.x; k < dim; k += kLdsBlock) st[k] = mk<R>(R(k == 0 ? 1 : 0), R(0));
+    __syncthreads();
+    const int G = (n + 3) / 4;
+    for (int l = 0; l < L; ++l) {
+      const cplx<R>* ml = m + (u64)l * n * 4;
+      for (int i = 0; i < G; ++i) {
+        const int K = (n - 1 - i) / G + 1;  // bits i + A*G < n
+        switch (K) {
+          case 1: lds_group<R, 1>(st, n, i, G, ml); break;
+          case 2: lds_group<R, 2>(st, n, i, G, ml); break;
+          case 3: lds_group<R, 3>(st, n, i, G, ml); break;
+          default: lds_group<R, 4>(st, n, i, G, ml); break;
+        }
+        __syncthreads();
+      }
+      // ladder: x[k] = st[k ^ (k >> 1)], gathered to registers first
+      constexpr int E = lds_per_thread<R>();
+      R xr[E], xi[E];  // split re/im: whole-struct copies out of LDS stay in scratch
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const unsigned k = threadIdx.x + j * kLdsBlock;
+        const unsigned kk = k < dim ? k : 0;  // in-bounds read; only k < dim is stored
+        const unsigned src = kk ^ (kk >> 1);
+        xr[j] = st[src].re;
+        xi[j] = st[src].im;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const unsigned k = threadIdx.x + j * kLdsBlock;
+        if (k < dim) st[k] = mk<R>(xr[j], xi[j]);
+      }
+      __syncthreads();
+    }
+    const unsigned zzmask = (1u << (n - 1)) - 1;
+    double acc = 0.0;
+    for (unsigned k = threadIdx.x; k < dim; k += kLdsBlock) {
+      const cplx<R> a = st[k];
+      const double ar = a.re, ai = a.im;
+      const double zz = (double)(n - 1) - 2.0 * (double)__popc((k ^ (k >> 1)) & zzmask);
+      double xs = 0.0;
+      for (int q = 0; q < n; ++q)
+        if (!(k & (1u << q))) {
+          const cplx<R> b = st[k ^ (1u << q)];
+          xs += ar * (double)b.re + ai * (double)b.im;
+        }
+      acc += -zz * (ar * ar + ai * ai) - 2.0 * g * xs;
+    }
+    // block sum over 512 threads
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __syncthreads();  // every thread is done reading st
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < kLdsBlock / 64; ++w) t += wsum[w];
+      energies[c] = t;
+    }
+    __syncthreads();
+  }
+}
+
 template <typename R, int K, bool ADJ>
 static void launch_rot(bool vec, dim3 gd, hipStream_t st, const cplx<R>* pin, cplx<R>* pout, const cplx<R>* lin,
                        cplx<R>* lout, const int32_t* sel, const cplx<R>* mats, double* partials, u64 ngroups,
@@ -425,6 +562,28 @@ int qj_tfim_batch(const void* psi, void* out, int dtype, int nqubits, uint64_t s
   else
     return set_error(QJ_EVALUE, "tfim_batch: dtype must be complex64/complex128");
   return check_launch("tfim_batch launch");
+}
+
+int qj_ansatz_lds(int dtype, const int32_t* desc, int ncirc, const void* mats, uint64_t mats_len, double g,
+                  double* energies, int nqmax, int nblocks, void* stream) {
+  if (!desc || !mats || !energies) return set_error(QJ_EVALUE, "ansatz_lds: null pointer");
+  if (ncirc < 0) return set_error(QJ_EVALUE, "ansatz_lds: negative circuit count");
+  if (ncirc == 0) return QJ_OK;
+  if (nblocks < 1 || nblocks > (1 << 20)) return set_error(QJ_EVALUE, "ansatz_lds: bad workgroup count %d", nblocks);
+  const size_t amp = dtype == QJ_COMPLEX64 ? 8 : dtype == QJ_COMPLEX128 ? 16 : 0;
+  if (!amp) return set_error(QJ_EVALUE, "ansatz_lds: dtype must be complex64/complex128");
+  if (nqmax < 2 || nqmax > 13 || (amp << nqmax) > 65536)
+    return set_error(QJ_EVALUE, "ansatz_lds: %d qubits do not fit one workgroup's LDS", nqmax);
+  const size_t lds = amp << nqmax;
+  hipStream_t st = (hipStream_t)stream;
+  dim3 gd((unsigned)nblocks);
+  if (dtype == QJ_COMPLEX64)
+    k_ansatz_lds<float><<<gd, dim3(kLdsBlock), lds, st>>>(desc, ncirc, (const cplx<float>*)mats, mats_len, g,
+                                                         energies, nqmax);
+  else
+    k_ansatz_lds<double><<<gd, dim3(kLdsBlock), lds, st>>>(desc, ncirc, (const cplx<double>*)mats, mats_len, g,
+                                                          energies, nqmax);
+  return check_launch("ansatz_lds launch");
 }
 
 int qj_reduce_rows(const double* in, uint64_t nrows, uint64_t ncols, double* out, void* stream) {
