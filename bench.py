@@ -47,6 +47,7 @@ def parse():
     p.add_argument("--only-sweep", action="store_true", help="sweep only (for profiler runs)")
     p.add_argument("--copy-probe", action="store_true", help="plain 4 GiB copy (counter scale reference)")
     p.add_argument("--only-adjoint", action="store_true", help="adjoint-gradient benchmark only")
+    p.add_argument("--only-batch", action="store_true", help="many-circuit ansatz benchmark only")
     return p.parse_args()
 
 
@@ -433,6 +434,72 @@ def run_adjoint(backend, n=24, layers=8, batch=16, steps=3, g=1.0, seed=2024):
     return res
 
 
+def run_batch(backend, ncirc=1024, steps=5, g=1.0, seed=4096):
+    """TFIM energies of `ncirc` ansatz circuits, qubits U[10,22], layers
+    U[4,12], angles U[-pi, pi), complex64: autotuned plan, one captured graph
+    per batch, against a loop of plain per-circuit launches."""
+    import torch
+
+    from qibojit_amd.batch import AnsatzBatch, autotune, plain_energies
+
+    rng = np.random.default_rng(seed)
+    shapes = [(int(rng.integers(10, 23)), int(rng.integers(4, 13))) for _ in range(ncirc)]
+    params = [rng.uniform(-np.pi, np.pi, 2 * n * nl) for n, nl in shapes]
+    res = {"circuits": ncirc, "qubits": [10, 22], "layers": [4, 12], "dtype": "complex64",
+           "gates": int(sum(nl * (3 * n - 1) for n, nl in shapes))}
+    t0 = time.perf_counter()
+    plan, timings = autotune(shapes, params, reps=2)
+    res["autotune_seconds"] = time.perf_counter() - t0
+    res["plan"] = {str(n): list(c) for n, c in sorted(plan.items())}
+    res["autotune_ms"] = {str(n): {"/".join(map(str, c)): round(t * 1e3, 3) for c, t in tv.items()}
+                          for n, tv in sorted(timings.items())}
+    batch = AnsatzBatch(shapes, dtype="complex64", g=g, plan=plan)
+    batch.set_parameters(params)
+    batch.capture()
+    batch.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.replay()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    e64 = batch.energies().cpu().numpy()
+    t0 = time.perf_counter()
+    batch.run()
+    torch.cuda.synchronize()
+    res["eager_seconds"] = time.perf_counter() - t0
+    res["graph_seconds_per_batch"] = dt
+    res["graph_circuits_per_s"] = ncirc / dt
+    del batch
+    torch.cuda.empty_cache()
+    # FP64-state reference of every circuit (same runner, complex128)
+    ref = AnsatzBatch(shapes, dtype="complex128", g=g)
+    ref.set_parameters(params)
+    ref.run()
+    e128 = ref.energies().cpu().numpy()
+    del ref
+    torch.cuda.empty_cache()
+    # plain loop: one launch per gate, one expectation per circuit
+    backend.set_dtype("complex64")
+    plain_energies(backend, shapes[:2], params[:2], g)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e_plain = plain_energies(backend, shapes, params, g)
+    torch.cuda.synchronize()
+    res["plain_loop_seconds"] = time.perf_counter() - t0
+    res["plain_circuits_per_s"] = ncirc / res["plain_loop_seconds"]
+    backend.set_dtype("complex128")
+    sub = list(range(0, ncirc, 16))
+    e_plain128 = plain_energies(backend, [shapes[i] for i in sub], [params[i] for i in sub], g)
+    res["speedup_graph_vs_plain_loop"] = res["plain_loop_seconds"] / dt
+    res["max_energy_error"] = {
+        "batched_c64_vs_batched_c128_all": float(np.max(np.abs(e64 - e128))),
+        "batched_c64_vs_plain_c64_all": float(np.max(np.abs(e64 - e_plain))),
+        "batched_c128_vs_plain_c128_every_16th": float(np.max(np.abs(e128[sub] - e_plain128))),
+    }
+    return res
+
+
 def cpu_baseline(n):
     """The reference's kernels restated in C/OpenMP, on this host's cores.
 
@@ -499,6 +566,9 @@ def main():
     if args.only_adjoint:
         print(json.dumps({"adjoint_24q": run_adjoint(backend)}), flush=True)
         return
+    if args.only_batch:
+        print(json.dumps({"batch_1024": run_batch(backend)}), flush=True)
+        return
     n = args.nqubits
     print(f"# sweep complex128 n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
     main_res = run_sweep(backend, n, "complex128", args.steps, args.warmup)
@@ -518,6 +588,8 @@ def main():
         extras["capacity_33q_complex128"] = run_capacity(backend, 33)
         print("# adjoint gradients 24q", file=sys.stderr, flush=True)
         extras["adjoint_24q_8layers_batch16"] = run_adjoint(backend)
+        print("# 1024 ansatz circuits", file=sys.stderr, flush=True)
+        extras["batch_1024_ansatz_complex64"] = run_batch(backend)
 
     name, avg_ms, avg_bytes, achieved, launches = dominant_kernel(main_res["rows"])
     roofline = {
