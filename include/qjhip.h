@@ -127,6 +127,17 @@ int qj_apply_multi_qubit(void* state, int dtype, int nqubits,
                          const int32_t* qubits, int nqubits_sorted,
                          const void* gate, void* stream);
 
+/* A dense 5-target block (32x32, row-major complex64, no controls) applied
+ * on FP32 matrix cores: the state viewed as 32 x 2^(n-5) is multiplied by
+ * the block as a real 64x64 GEMM with v_mfma_f32_32x32x2f32.  Same matrix
+ * convention as qj_apply_multi_qubit (target_bits[0] is the row index's most
+ * significant bit); complex64 only, 10 <= nqubits.  It is the matrix-core
+ * form of apply_five_qubit_gate_kernel (gates.py:336-411) for the fused
+ * 5-qubit blocks of the fusion executor (cpu.py:535-537). */
+int qj_apply_block_mfma(void* state, int dtype, int nqubits,
+                        const int32_t* target_bits, int ntargets,
+                        const void* gate, void* stream);
+
 /* ---- fused passes (LDS tiles) ----
  * One pass streams the state through LDS in tiles of 2^(low_bits + nhigh)
  * amplitudes: a tile holds every combination of the `low_bits` lowest bits

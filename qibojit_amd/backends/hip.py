@@ -102,6 +102,8 @@ class HipBackend(Backend):
         self.nthreads = len(os.sched_getaffinity(0))
         # execute_circuit runs unitary segments through the fused tile executor
         self.fusion = True
+        # uncontrolled 5-target complex64 blocks run on FP32 matrix cores
+        self.mfma = True
         self._custom_matrices_c128 = CustomMatrices("complex128")
 
     # ------------------------------------------------------------ config --
@@ -411,7 +413,14 @@ class HipBackend(Backend):
             qubits = sorted(nqubits - q - 1 for q in targets)
         mat = self._host_matrix(gate, state.dtype)
         tptr, tkeep, nt = i32_array([nqubits - t - 1 for t in targets])
-        qptr, qkeep, nq = i32_array(self._seq(qubits))
+        qseq = self._seq(qubits)
+        if self.mfma and state.dtype == torch.complex64 and nt == 5 and len(qseq) == 5 and nqubits >= 10:
+            # uncontrolled 5-qubit block: FP32 matrix-core GEMM
+            call("qj_apply_block_mfma", _ptr(state), _NATIVE_CODES[state.dtype], nqubits, tptr, nt,
+                 mat.ctypes.data_as(ctypes.c_void_p), self._stream())
+            del tkeep
+            return state
+        qptr, qkeep, nq = i32_array(qseq)
         call(
             "qj_apply_multi_qubit",
             _ptr(state),

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libqjhip.so")
 ARCH = os.environ.get("QJ_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["qj_runtime.hip", "qj_gates.hip", "qj_ops.hip", "qj_tile.hip", "qj_group.hip"]
+SOURCES = ["qj_runtime.hip", "qj_gates.hip", "qj_ops.hip", "qj_tile.hip", "qj_group.hip", "qj_mfma.hip"]
 
 CFLAGS = [
     f"--offload-arch={ARCH}",

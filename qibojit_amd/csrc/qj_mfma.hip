@@ -1,0 +1,140 @@
+// Fused 5-qubit blocks on complex64 states as FP32 matrix-core GEMMs.
+//
+// A 5-target block multiplies every group of 32 amplitudes (the targets'
+// combinations for one assignment of the other bits) by the 32x32 complex
+// matrix U: the state is a 32 x 2^(n-5) complex matrix X (rows = target
+// index, columns = groups) and the block computes U X.  Written over the
+// reals, [Yr; Yi] = [[Ur, -Ui], [Ui, Ur]] [Xr; Xi]: a 64x64 by 64 x ncols
+// GEMM, which v_mfma_f32_32x32x2f32 (32x32 outputs, K = 2 per instruction,
+// FP32 products and sums) runs as 2 row tiles x 32 K-steps per 32 columns.
+//
+// Lane layout of v_mfma_f32_32x32x2f32 for lane l, i = l % 32, h = l / 32:
+//   A: A[i][h]    B: B[h][i]    D: register r holds D[8(r/4) + 4h + r%4][i].
+// Each wave owns 32 columns: lane (i, h) loads the amplitudes of rows
+// c = 2j + h (j < 16) of column i, which are exactly its B operands (K-step
+// s < 16 reads Re X[2s + h], s >= 16 reads Im X[2(s-16) + h]); the real
+// 64x64 matrix stays in registers as the A operands for the whole kernel.
+// The outputs of row tile 0 are Re Y and of row tile 1 Im Y for the same
+// rows, so each lane stores 16 complete amplitudes of its column.  Reads of
+// a column finish before its writes (the MFMAs consume every lane's loads),
+// so the update is in place.
+//
+// Matrix convention as qj_apply_multi_qubit: bit (4 - u) of the row index is
+// target target_bits[u] (the first target is the most significant bit).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "qj_common.h"
+
+namespace qj {
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int kMfmaBlock = 256;  // 4 waves, 32 columns each per step
+constexpr u64 kMfmaMaxGrid = 8192;
+
+__global__ void __launch_bounds__(kMfmaBlock)
+    k_block5_mfma(cplx<float>* __restrict__ st, u64 ncols, Ins ins, Offs32 off, const float* __restrict__ w) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, i = lane & 31;
+  float a0[32], a1[32];  // A operands of row tiles 0 (Re rows) and 1 (Im rows)
+#pragma unroll
+  for (int s = 0; s < 32; ++s) {
+    a0[s] = w[i * 64 + 2 * s + h];
+    a1[s] = w[(32 + i) * 64 + 2 * s + h];
+  }
+  const u64 step = (u64)gridDim.x * (kMfmaBlock / 64) * 32;
+  for (u64 c0 = ((u64)blockIdx.x * (kMfmaBlock / 64) + wave) * 32; c0 < ncols; c0 += step) {
+    const u64 base = insert_bits(c0 + i, ins);
+    float xr[16], xi[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const cplx<float> v = st[base + off.o[2 * j + h]];
+      xr[j] = v.re;
+      xi[j] = v.im;
+    }
+    v16f d0 = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    v16f d1 = d0;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[s], xr[s], d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[s], xr[s], d1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[16 + s], xi[s], d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[16 + s], xi[s], d1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int m = 8 * (r >> 2) + 4 * h + (r & 3);
+      st[base + off.o[m]] = mk<float>(d0[r], d1[r]);
+    }
+  }
+}
+
+static int block5_impl(void* state, int nqubits, const int32_t* target_bits, const void* gate, hipStream_t s) {
+  if (nqubits < 10 || nqubits > 40)
+    return set_error(QJ_EVALUE, "apply_block_mfma: needs 10..40 qubits (32-column tiles), got %d", nqubits);
+  Ins ins;
+  std::memset(&ins, 0, sizeof(ins));
+  int pos[5];
+  for (int u = 0; u < 5; ++u) {
+    if (target_bits[u] < 0 || target_bits[u] >= nqubits)
+      return set_error(QJ_EVALUE, "apply_block_mfma: target bit %d out of range", target_bits[u]);
+    for (int v = 0; v < u; ++v)
+      if (target_bits[v] == target_bits[u]) return set_error(QJ_EVALUE, "apply_block_mfma: repeated target");
+    pos[u] = target_bits[u];
+  }
+  std::sort(pos, pos + 5);
+  ins.npos = 5;
+  for (int u = 0; u < 5; ++u) ins.pos[u] = pos[u];
+  Offs32 off;
+  for (int r = 0; r < 32; ++r) {
+    u64 o = 0;
+    for (int u = 0; u < 5; ++u)
+      if ((r >> (4 - u)) & 1) o |= u64(1) << target_bits[u];
+    off.o[r] = o;
+  }
+  // real 64x64 form of the complex matrix, row-major
+  const cplx<float>* u = static_cast<const cplx<float>*>(gate);
+  std::vector<float> w(64 * 64);
+  for (int r = 0; r < 32; ++r)
+    for (int c = 0; c < 32; ++c) {
+      const cplx<float> z = u[r * 32 + c];
+      w[r * 64 + c] = z.re;
+      w[r * 64 + 32 + c] = -z.im;
+      w[(32 + r) * 64 + c] = z.im;
+      w[(32 + r) * 64 + 32 + c] = z.re;
+    }
+  void* dw = nullptr;
+  int ticket = -1;
+  int rc = ring_upload(w.data(), w.size() * sizeof(float), s, &dw, &ticket);
+  if (rc) return rc;
+  const u64 ncols = u64(1) << (nqubits - 5);
+  const u64 tasks = ncols / 32;  // wave steps
+  u64 grid = (tasks + (kMfmaBlock / 64) - 1) / (kMfmaBlock / 64);
+  if (grid > kMfmaMaxGrid) grid = kMfmaMaxGrid;
+  k_block5_mfma<<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(static_cast<cplx<float>*>(state), ncols, ins,
+                                                                 off, static_cast<const float*>(dw));
+  rc = check_launch("apply_block_mfma launch");
+  ring_release(ticket, s);
+  return rc;
+}
+
+}  // namespace qj
+
+using namespace qj;
+
+extern "C" {
+
+int qj_apply_block_mfma(void* state, int dtype, int nqubits, const int32_t* target_bits, int ntargets,
+                        const void* gate, void* stream) {
+  if (!state || !target_bits || !gate) return set_error(QJ_EVALUE, "apply_block_mfma: null pointer");
+  if (dtype != QJ_COMPLEX64) return set_error(QJ_EVALUE, "apply_block_mfma: complex64 only (FP32 matrix cores)");
+  if (ntargets != 5) return set_error(QJ_EVALUE, "apply_block_mfma: exactly 5 targets, got %d", ntargets);
+  return block5_impl(state, nqubits, target_bits, gate, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -90,3 +90,89 @@ def fused_matrix(custom_matrix_of, fgate):
         g = embed(block, gate.target_qubits, gate.control_qubits, u)
         m = g @ m
     return m
+
+
+def plan_blocks(gate_qubits, max_qubits=5):
+    """Group a gate sequence into blocks of at most ``max_qubits`` qubits.
+
+    ``gate_qubits[i]`` is the qubit tuple (targets and controls) of gate i.
+    A gate may join the open block only when every earlier gate on each of
+    its qubits is already placed (in an earlier block or this one), so the
+    blocks, applied in order, equal the sequence.  The block grows first by
+    gates inside its qubits, then by the ready gate adding the fewest new
+    qubits (earliest first); it closes when no ready gate fits.  Unlike
+    qibo's ``Circuit.fuse`` (one owner block per qubit), blocks may take
+    gates from several earlier chains, which is what lets a brickwork of
+    2-qubit gates fuse into 5-qubit blocks.  Returns a list of
+    (sorted block qubits, [gate indices in order]).
+    """
+    per_q = {}
+    for i, qs in enumerate(gate_qubits):
+        if len(set(qs)) > max_qubits:
+            raise ValueError(f"Gate {i} acts on more than {max_qubits} qubits.")
+        for q in qs:
+            per_q.setdefault(q, []).append(i)
+    head = {q: 0 for q in per_q}
+    placed = [False] * len(gate_qubits)
+
+    def ready(i):
+        return all(per_q[q][head[q]] == i for q in gate_qubits[i])
+
+    def place(i):
+        placed[i] = True
+        for q in gate_qubits[i]:
+            head[q] += 1
+
+    blocks = []
+    nxt = 0
+    while nxt < len(gate_qubits):
+        while nxt < len(gate_qubits) and placed[nxt]:
+            nxt += 1
+        if nxt == len(gate_qubits):
+            break
+        block, bq = [nxt], set(gate_qubits[nxt])
+        place(nxt)
+        while True:
+            cands = sorted({per_q[q][head[q]] for q in per_q if head[q] < len(per_q[q])})
+            inside = [i for i in cands if set(gate_qubits[i]) <= bq and ready(i)]
+            if inside:
+                for i in inside:
+                    if ready(i):
+                        place(i)
+                        block.append(i)
+                continue
+            best = None
+            for i in cands:
+                new = set(gate_qubits[i]) - bq
+                if len(bq) + len(new) <= max_qubits and ready(i):
+                    key = (len(new), i)
+                    if best is None or key < best[0]:
+                        best = (key, i)
+            if best is None:
+                break
+            i = best[1]
+            place(i)
+            block.append(i)
+            bq |= set(gate_qubits[i])
+        blocks.append((sorted(bq), block))
+    return blocks
+
+
+def fuse_blocks(circuit, max_qubits=5):
+    """A circuit whose unitary gates are grouped by :func:`plan_blocks` into
+    FusedGates of at most ``max_qubits`` qubits (one dense pass each);
+    single-gate blocks keep the gate itself.  Every gate must be unitary."""
+    from qibojit_amd._qibo import Circuit, FusedGate  # pylint: disable=import-outside-toplevel
+
+    queue = list(circuit.queue)
+    blocks = plan_blocks([tuple(g.qubits) for g in queue], max_qubits)
+    out = Circuit(circuit.nqubits)
+    for qubits, idx in blocks:
+        if len(idx) == 1:
+            out.add(queue[idx[0]])
+            continue
+        fgate = FusedGate(*qubits)
+        for i in idx:
+            fgate.append(queue[i])
+        out.add(fgate)
+    return out

@@ -145,3 +145,32 @@ def test_diagonal_gates_become_phase_ops():
     psi = gm.random_state(3, np.random.default_rng(1))
     got, _ = _run_planned([gates.RZ(1, 0.3)], psi, 3, 1, 2)
     np.testing.assert_allclose(got, er.apply_gate(psi, gm.RZ(0.3), [1]), atol=1e-15)
+
+
+def test_plan_blocks_respects_dependencies_and_width():
+    import itertools
+
+    from qibojit_amd.fusion import plan_blocks
+
+    rng = np.random.default_rng(5)
+    n = 9
+    seq = [tuple(rng.choice(n, size=int(rng.integers(1, 3)), replace=False).tolist()) for _ in range(200)]
+    blocks = plan_blocks(seq, 5)
+    order = [i for _, idx in blocks for i in idx]
+    assert sorted(order) == list(range(len(seq)))
+    pos = {i: k for k, i in enumerate(order)}
+    # every pair of gates sharing a qubit keeps its relative order
+    for i, j in itertools.combinations(range(len(seq)), 2):
+        if set(seq[i]) & set(seq[j]):
+            assert pos[i] < pos[j]
+    for qubits, idx in blocks:
+        assert len(qubits) <= 5 and set().union(*(set(seq[i]) for i in idx)) == set(qubits)
+
+
+def test_plan_blocks_fuses_a_brickwork():
+    from qibojit_amd.fusion import plan_blocks
+
+    seq = [(q, q + 1) for d in range(40) for q in range(d % 2, 31, 2)]
+    blocks = plan_blocks(seq, 5)
+    assert len(blocks) <= len(seq) / 4
+    assert max(len(q) for q, _ in blocks) == 5
