@@ -48,6 +48,7 @@ def parse():
     p.add_argument("--copy-probe", action="store_true", help="plain 4 GiB copy (counter scale reference)")
     p.add_argument("--only-adjoint", action="store_true", help="adjoint-gradient benchmark only")
     p.add_argument("--only-batch", action="store_true", help="many-circuit ansatz benchmark only")
+    p.add_argument("--only-precision", action="store_true", help="complex64 vs complex128 runs and the MFMA kernel")
     return p.parse_args()
 
 
@@ -271,6 +272,145 @@ def run_qft(backend, n, dtype, fused=False):
                     "GBps_passes": passes_bytes / sec / 1e9})
     else:
         out.update({"fused": False, "algorithmic_GB": total / 1e9, "GBps": total / sec / 1e9})
+    return out
+
+
+MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
+
+
+def brickwork(n, depth, seed):
+    """Depth-`depth` brickwork of seeded 2-qubit Haar unitaries: layer d acts
+    on pairs (q, q+1) with q = d mod 2, d mod 2 + 2, ..."""
+    from qibojit_amd._qibo import Circuit, gates
+
+    rng = np.random.default_rng(seed)
+    c = Circuit(n)
+    for d in range(depth):
+        for q in range(d % 2, n - 1, 2):
+            z = (rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))) / np.sqrt(2)
+            qm, r = np.linalg.qr(z)
+            c.add(gates.Unitary(qm * (np.diag(r) / np.abs(np.diag(r))), q, q + 1))
+    return c
+
+
+def _timed(fn):
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t0
+
+
+def run_mfma_kernel(backend, n=32, launches=5, seed=11):
+    """One fused 5-qubit block on matrix cores, timed alone: 2^(n-5) columns
+    of a 32x32 complex GEMM per launch.  Algorithmic bytes per launch: one
+    read and one write of the complex64 state; flops: 8 * 32 * 32 per column."""
+    import torch
+
+    from qibojit_amd._qibo import gates
+
+    backend.set_dtype("complex64")
+    rng = np.random.default_rng(seed)
+    z = (rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))) / np.sqrt(2)
+    u, _ = np.linalg.qr(z)
+    state = torch.full((2**n,), 2 ** (-n / 2), dtype=torch.complex64, device="cuda")
+    res = {}
+    for label, targets in (("mid", (10, 11, 12, 13, 14)), ("low", (27, 28, 29, 30, 31))):
+        gate = gates.Unitary(u, *targets)
+        for mfma in (True, False):
+            backend.mfma = mfma
+            backend.apply_gate(gate, state, n)  # warm-up
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(launches + 1)]
+            ev[0].record()
+            for k in range(launches):
+                backend.apply_gate(gate, state, n)
+                ev[k + 1].record()
+            torch.cuda.synchronize()
+            ms = float(np.mean([ev[k].elapsed_time(ev[k + 1]) for k in range(launches)]))
+            nbytes = 2 * 8 * 2**n
+            flops = 8 * 32 * 32 * 2 ** (n - 5)
+            res[f"{label}_{'mfma' if mfma else 'valu'}"] = {
+                "targets": list(targets), "ms": ms, "GBps": nbytes / ms / 1e6, "TFLOPs": flops / ms / 1e9,
+                "hbm_frac": nbytes / ms / 1e6 / HBM_PEAK_GBS,
+                "mfma_util": flops / ms / 1e9 / MFMA_FP32_PEAK_TFLOPS if mfma else None,
+            }
+    backend.mfma = True
+    del state
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_precision(backend, n=32, depth=40, seed=5):
+    """complex64 results against complex128 runs of the same circuits on the
+    GPU: (a) QFT from a seeded state (fused tile executor in both
+    precisions), (b) depth-`depth` brickwork of 2-qubit Haar unitaries from
+    |0...0>: complex64 fused into 5-qubit blocks on matrix cores, complex128
+    gate by gate."""
+    import torch
+
+    from qibojit_amd.fusion import fuse_blocks
+
+    out = {}
+    # (a) QFT
+    from qibojit_amd._qibo import Circuit, gates
+
+    qft = Circuit(n)
+    for i in range(n):
+        qft.add(gates.H(i))
+        for j in range(i + 1, n):
+            qft.add(gates.CU1(j, i, np.pi / 2 ** (j - i)))
+    for i in range(n // 2):
+        qft.add(gates.SWAP(i, n - i - 1))
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    ref = torch.randn(2**n, dtype=torch.complex128, device="cuda", generator=gen)
+    ref /= torch.linalg.vector_norm(ref)
+    st = ref.to(torch.complex64)
+    backend.set_dtype("complex128")
+    ref, t128 = _timed(lambda: backend.apply_gates_fused(qft.queue, ref, n))
+    backend.set_dtype("complex64")
+    st, t64 = _timed(lambda: backend.apply_gates_fused(qft.queue, st, n))
+    cmp = backend.compare_states(st, ref)
+    out["qft"] = {"gates": len(qft.queue), "seconds_c64_fused": t64, "seconds_c128_fused": t128,
+                  "infidelity": cmp["infidelity"], "max_abs_error": cmp["max_abs_diff"],
+                  "norm2_c64": cmp["norm2_a"]}
+    del st, ref
+    torch.cuda.empty_cache()
+    # (b) brickwork
+    circ = brickwork(n, depth, seed)
+    fused = fuse_blocks(circ, 5)
+    sizes = {}
+    for g in fused.queue:
+        sizes[len(g.qubits)] = sizes.get(len(g.qubits), 0) + 1
+    backend.set_dtype("complex64")
+
+    def run(queue):
+        s = backend.zero_state(n)
+        for g in queue:
+            s = backend.apply_gate(g, s, n)
+        return s
+
+    st, t_mfma = _timed(lambda: run(fused.queue))
+    backend.set_dtype("complex128")
+    ref, t_ref = _timed(lambda: run(circ.queue))
+    cmp = backend.compare_states(st, ref)
+    del st
+    torch.cuda.empty_cache()
+    backend.set_dtype("complex64")
+    backend.mfma = False
+    st_valu, t_valu = _timed(lambda: run(fused.queue))
+    backend.mfma = True
+    del st_valu
+    torch.cuda.empty_cache()
+    out["brickwork"] = {
+        "nqubits": n, "depth": depth, "gates": len(circ.queue), "fused_blocks": len(fused.queue),
+        "block_sizes": {str(k): v for k, v in sorted(sizes.items())},
+        "seconds_c64_fused_mfma": t_mfma, "seconds_c64_fused_valu": t_valu, "seconds_c128_unfused": t_ref,
+        "infidelity": cmp["infidelity"], "max_abs_error": cmp["max_abs_diff"], "norm2_c64": cmp["norm2_a"],
+    }
+    del ref
+    torch.cuda.empty_cache()
     return out
 
 
@@ -569,6 +709,10 @@ def main():
     if args.only_batch:
         print(json.dumps({"batch_1024": run_batch(backend)}), flush=True)
         return
+    if args.only_precision:
+        print(json.dumps({"mfma_kernel_32q": run_mfma_kernel(backend), "precision_32q": run_precision(backend)},
+                         default=str), flush=True)
+        return
     n = args.nqubits
     print(f"# sweep complex128 n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
     main_res = run_sweep(backend, n, "complex128", args.steps, args.warmup)
@@ -586,6 +730,9 @@ def main():
         extras["qft_32q_complex64"] = run_qft(backend, 32, "complex64")
         print("# capacity 33q complex128", file=sys.stderr, flush=True)
         extras["capacity_33q_complex128"] = run_capacity(backend, 33)
+        print("# precision 32q complex64 vs complex128", file=sys.stderr, flush=True)
+        extras["mfma_kernel_32q_complex64"] = run_mfma_kernel(backend)
+        extras["precision_32q_complex64"] = run_precision(backend)
         print("# adjoint gradients 24q", file=sys.stderr, flush=True)
         extras["adjoint_24q_8layers_batch16"] = run_adjoint(backend)
         print("# 1024 ansatz circuits", file=sys.stderr, flush=True)

@@ -288,6 +288,12 @@ int qj_measure_frequencies(int64_t* frequencies, const void* probs,
 
 /* ---- reductions (FP64 accumulation; synchronise `stream`) ---- */
 
+/* Compare two states of n amplitudes, each complex64 or complex128 (FP64
+ * sums, host output): out = {Re <a|b>, Im <a|b>, <a|a>, <b|b>,
+ * max_i |a_i - b_i|}.  For precision reports (infidelity of a complex64 run
+ * against a complex128 one); no reference counterpart. */
+int qj_compare_states(const void* a, int dtype_a, const void* b, int dtype_b,
+                      uint64_t n, double* out, void* stream);
 /* sum |state_i|^2 over `nelements` amplitudes. */
 int qj_norm_squared(const void* state, int dtype, uint64_t nelements,
                     double* out, void* stream);

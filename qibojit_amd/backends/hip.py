@@ -570,6 +570,24 @@ class HipBackend(Backend):
         call("qj_vdot", _ptr(a), _ptr(b), _NATIVE_CODES[a.dtype], a.numel(), out, self._stream())
         return complex(out[0], out[1])
 
+    def compare_states(self, a, b):
+        """FP64 comparison of two states of any complex precision: overlap,
+        norms, infidelity 1 - |<a|b>|^2 / (<a|a><b|b>) and max |a_i - b_i|."""
+        a, b = self._as_state(a), self._as_state(b)
+        if a.numel() != b.numel():
+            raise ValueError("compare_states needs two states of equal size.")
+        out = (ctypes.c_double * 5)()
+        call("qj_compare_states", _ptr(a), _NATIVE_CODES[a.dtype], _ptr(b), _NATIVE_CODES[b.dtype], a.numel(), out,
+             self._stream())
+        overlap = complex(out[0], out[1])
+        return {
+            "overlap": overlap,
+            "norm2_a": out[2],
+            "norm2_b": out[3],
+            "infidelity": 1.0 - abs(overlap) ** 2 / (out[2] * out[3]),
+            "max_abs_diff": out[4],
+        }
+
     def expectation_tfim(self, state, nqubits, g=1.0):
         """<psi| -sum Z_i Z_{i+1} - g sum X_i |psi> (open chain), FP64."""
         state = self._as_state(state)

@@ -386,6 +386,43 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Two states of possibly different precision: partial sums of Re/Im <a|b>,
+// |a|^2, |b|^2 and the largest |a_i - b_i| per workgroup, in FP64.
+template <typename RA, typename RB>
+__global__ void __launch_bounds__(kBlock)
+    k_compare(const cplx<RA>* __restrict__ a, const cplx<RB>* __restrict__ b, u64 n, double* __restrict__ partials) {
+  __shared__ double wsum[kBlock / 64];
+  double re = 0.0, im = 0.0, na = 0.0, nb = 0.0, dmax = 0.0;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double xr = a[i].re, xi = a[i].im, yr = b[i].re, yi = b[i].im;
+    re += xr * yr + xi * yi;
+    im += xr * yi - xi * yr;
+    na += xr * xr + xi * xi;
+    nb += yr * yr + yi * yi;
+    const double dr = xr - yr, di = xi - yi;
+    dmax = fmax(dmax, dr * dr + di * di);
+  }
+  re = block_sum(re, wsum);
+  im = block_sum(im, wsum);
+  na = block_sum(na, wsum);
+  nb = block_sum(nb, wsum);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = dmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) m = fmax(m, wsum[w]);
+    const u64 g = gridDim.x, k = blockIdx.x;
+    partials[k] = re;
+    partials[g + k] = im;
+    partials[2 * g + k] = na;
+    partials[3 * g + k] = nb;
+    partials[4 * g + k] = sqrt(m);
+  }
+}
+
 // <psi|H|psi>, H = -sum Z_i Z_{i+1} - g sum X_i on an open chain of n qubits.
 // Diagonal: sum_i z_i z_{i+1} = (n-1) - 2 popcount((k ^ (k >> 1)) & (2^(n-1)-1)).
 // X_b term: sum_k Re(conj(psi_k) psi_{k^2^b}) = 2 sum_{bit b of k = 0} Re(...).
@@ -717,6 +754,47 @@ int qj_vdot(const void* a, const void* b, int dtype, uint64_t n, double* out, vo
   rc = check_launch("vdot launch");
   if (rc) return rc;
   return reduce_to_host(2, grid, grid, 1, out, s, sc);
+}
+
+int qj_compare_states(const void* a, int dtype_a, const void* b, int dtype_b, uint64_t n, double* out,
+                      void* stream) {
+  if (!a || !b || !out) return set_error(QJ_EVALUE, "compare_states: null pointer");
+  if (!is_complex(dtype_a) || !is_complex(dtype_b)) return set_error(QJ_EVALUE, "compare_states: complex states only");
+  if (n == 0) return set_error(QJ_EVALUE, "compare_states: empty states");
+  hipStream_t s = (hipStream_t)stream;
+  DeviceScratch* sc = nullptr;
+  int rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  const u64 grid = std::min<u64>(kRedBlocks, grid_for(n, kBlock));
+  const dim3 gd((unsigned)grid), bd(kBlock);
+  const bool a64 = dtype_a == QJ_COMPLEX64, b64 = dtype_b == QJ_COMPLEX64;
+  if (a64 && b64)
+    k_compare<float, float><<<gd, bd, 0, s>>>((const cplx<float>*)a, (const cplx<float>*)b, n, sc->partials);
+  else if (a64)
+    k_compare<float, double><<<gd, bd, 0, s>>>((const cplx<float>*)a, (const cplx<double>*)b, n, sc->partials);
+  else if (b64)
+    k_compare<double, float><<<gd, bd, 0, s>>>((const cplx<double>*)a, (const cplx<float>*)b, n, sc->partials);
+  else
+    k_compare<double, double><<<gd, bd, 0, s>>>((const cplx<double>*)a, (const cplx<double>*)b, n, sc->partials);
+  rc = check_launch("compare launch");
+  if (rc) return rc;
+  std::vector<double> host(5 * grid);
+  rc = hip_status(hipMemcpyAsync(host.data(), sc->partials, host.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+                  "compare readback");
+  if (rc) return rc;
+  rc = hip_status(hipStreamSynchronize(s), "compare sync");
+  if (rc) return rc;
+  double v[5] = {0, 0, 0, 0, 0};
+  for (u64 k = 0; k < grid; ++k) {
+    for (int j = 0; j < 4; ++j) v[j] += host[j * grid + k];
+    v[4] = std::max(v[4], host[4 * grid + k]);
+  }
+  out[0] = v[0];
+  out[1] = v[1];
+  out[2] = v[2];
+  out[3] = v[3];
+  out[4] = v[4];
+  return QJ_OK;
 }
 
 int qj_expectation_tfim(const void* state, int dtype, int nqubits, double g, double* out,

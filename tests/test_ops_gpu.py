@@ -169,3 +169,21 @@ def test_collapse_large_state_idempotent(backend):
     assert abs(backend.norm_squared(out) - 1) < 1e-5
     p2 = backend.to_numpy(backend.calculate_probabilities(out, [0, 7, n - 1], n))
     np.testing.assert_allclose(p2, np.eye(8)[5], atol=1e-5)
+
+
+def test_compare_states_mixed_precision(backend, rng):
+    from oracle.gate_matrices import random_state
+
+    a = random_state(9, rng)
+    b = a + 1e-3 * random_state(9, rng)
+    res = backend.compare_states(backend.cast(a.astype(np.complex64), dtype="complex64"), backend.cast(b))
+    a64 = a.astype(np.complex64).astype(np.complex128)
+    ov = np.vdot(a64, b)
+    assert abs(res["overlap"] - ov) < 1e-12
+    assert res["norm2_a"] == pytest.approx(np.vdot(a64, a64).real, abs=1e-12)
+    assert res["norm2_b"] == pytest.approx(np.vdot(b, b).real, abs=1e-12)
+    assert res["max_abs_diff"] == pytest.approx(np.max(np.abs(a64 - b)), rel=1e-12)
+    want_inf = 1 - abs(ov) ** 2 / (np.vdot(a64, a64).real * np.vdot(b, b).real)
+    assert res["infidelity"] == pytest.approx(want_inf, abs=1e-12)
+    same = backend.compare_states(backend.cast(a), backend.cast(a))
+    assert same["max_abs_diff"] == 0.0 and abs(same["infidelity"]) < 1e-14
