@@ -215,6 +215,169 @@ __global__ void __launch_bounds__(kBlock)
 // Validates the sorted qubit list and builds the insertion spec: every
 // position gets a 0 inserted, control positions (those not in `targets`) are
 // then set to 1.
+// ---- gates whose targets all lie in the 6 lowest bits: in-wave exchange --
+// A wave owns a chunk of 64 consecutive amplitudes, one per lane, so every
+// load and store of the wave is one contiguous run (the pair kernels above
+// read such gates with strided per-lane accesses).  Lane l holds the
+// amplitude of chunk-local index l; its group partners (other values of the
+// target bits) live in lanes l ^ xmask[d] and arrive by cross-lane shuffles.
+// A butterfly of conditional swaps orders the group by row index
+// (x[c] = y[c ^ r]) so that the new amplitude sum_c U[r][c] x[c] is summed in
+// ascending c, the reference's order.  Controls above bit 5 select the
+// chunks (insert_bits on the chunk index); controls below bit 6 mask the
+// stores of lanes whose control bits are not all 1 (every lane still joins
+// the shuffles).
+constexpr int kWaveBits = 6;
+constexpr int kWaveIlp = 2;
+
+template <int K>
+struct WaveSpec {
+  int tb[K];           // target bit of row-index bit K-1-u (first target = MSB)
+  int xmask[1 << K];   // lane xor of group member d
+  int cmask, cones;    // low control bits of the lane index
+};
+
+template <typename R>
+__device__ __forceinline__ cplx<R> shfl_xor_c(cplx<R> a, int mask) {
+  return mk<R>(__shfl_xor(a.re, mask, 64), __shfl_xor(a.im, mask, 64));
+}
+
+template <typename R, int K>
+__global__ void __launch_bounds__(kBlock)
+    k_wave(cplx<R>* __restrict__ st, u64 nchunks, Ins ins, WaveSpec<K> ws, const cplx<R>* __restrict__ mat) {
+  constexpr int N = 1 << K;
+  const int lane = threadIdx.x & 63;
+  int r = 0;
+#pragma unroll
+  for (int u = 0; u < K; ++u) r |= ((lane >> ws.tb[u]) & 1) << (K - 1 - u);
+  cplx<R> row[N];
+#pragma unroll
+  for (int c = 0; c < N; ++c) row[c] = mat[r * N + c];
+  const bool store = (lane & ws.cmask) == ws.cones;
+  const u64 wave = (u64)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const u64 nwaves = (u64)gridDim.x * (kBlock / 64);
+  for (u64 c0 = wave * kWaveIlp; c0 < nchunks; c0 += nwaves * kWaveIlp) {  // uniform per wave
+    u64 idx[kWaveIlp];
+    cplx<R> a[kWaveIlp];
+#pragma unroll
+    for (int i = 0; i < kWaveIlp; ++i) {
+      const u64 c = c0 + i < nchunks ? c0 + i : c0;
+      idx[i] = (insert_bits(c, ins) << kWaveBits) | (u64)lane;
+      a[i] = st[idx[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < kWaveIlp; ++i) {
+      cplx<R> y[N];
+      y[0] = a[i];
+#pragma unroll
+      for (int d = 1; d < N; ++d) y[d] = shfl_xor_c(a[i], ws.xmask[d]);
+      // x[c] = y[c ^ r]: swap across row bit b wherever r has it set
+#pragma unroll
+      for (int b = 0; b < K; ++b) {
+        const bool sw = (r >> b) & 1;
+#pragma unroll
+        for (int d = 0; d < N; ++d) {
+          if (d & (1 << b)) continue;
+          const cplx<R> lo = y[d], hi = y[d | (1 << b)];
+          y[d] = sw ? hi : lo;
+          y[d | (1 << b)] = sw ? lo : hi;
+        }
+      }
+      cplx<R> acc = cmul(row[0], y[0]);
+#pragma unroll
+      for (int c = 1; c < N; ++c) acc = cadd(acc, cmul(row[c], y[c]));
+      if (store && c0 + i < nchunks) st[idx[i]] = acc;
+    }
+  }
+}
+
+// Dense target block of a kernel family (the reference's encodings).
+template <typename R>
+static void dense_one(int kind, const cplx<R>* g, cplx<R>* d) {
+  const cplx<R> z = mk<R>(0, 0), o = mk<R>(1, 0);
+  switch (kind) {
+    case QJ_APPLY_GATE: for (int i = 0; i < 4; ++i) d[i] = g[i]; break;
+    case QJ_APPLY_X: d[0] = z; d[1] = o; d[2] = o; d[3] = z; break;
+    case QJ_APPLY_Y: d[0] = z; d[1] = mk<R>(0, -1); d[2] = mk<R>(0, 1); d[3] = z; break;
+    case QJ_APPLY_Z: d[0] = o; d[1] = z; d[2] = z; d[3] = mk<R>(-1, 0); break;
+    default: d[0] = o; d[1] = z; d[2] = z; d[3] = g[0]; break;  // z_pow
+  }
+}
+
+template <typename R>
+static void dense_two(int kind, const cplx<R>* g, cplx<R>* d) {
+  for (int i = 0; i < 16; ++i) d[i] = mk<R>(0, 0);
+  if (kind == QJ_APPLY_TWO_QUBIT_GATE) {
+    for (int i = 0; i < 16; ++i) d[i] = g[i];
+  } else if (kind == QJ_APPLY_SWAP) {
+    d[0] = d[6] = d[9] = d[15] = mk<R>(1, 0);
+  } else {  // fsim: 2x2 on |01>,|10>, phase on |11>
+    d[0] = mk<R>(1, 0);
+    d[5] = g[0];
+    d[6] = g[1];
+    d[9] = g[2];
+    d[10] = g[3];
+    d[15] = g[4];
+  }
+}
+
+// Applies a dense 2^k block with the in-wave kernel when every target is
+// below bit 6 and the state has at least one chunk; returns 1 if handled.
+template <typename R>
+static int try_wave(cplx<R>* st, int nqubits, const int* tb, int k, const Ins& gate_ins, const cplx<R>* dense,
+                    hipStream_t s, int* rc) {
+  // three targets: seven shuffles per amplitude cost more than the strided
+  // register kernel loses, so only one and two targets come here
+  if (k < 1 || k > 2 || nqubits < kWaveBits) return 0;
+  for (int u = 0; u < k; ++u)
+    if (tb[u] >= kWaveBits) return 0;
+  Ins ins;
+  std::memset(&ins, 0, sizeof(ins));
+  int cmask = 0, cones = 0, nh = 0;
+  for (int j = 0; j < gate_ins.npos; ++j) {
+    const int p = gate_ins.pos[j];
+    bool target = false;
+    for (int u = 0; u < k; ++u) target |= (tb[u] == p);
+    if (target) continue;
+    if (p < kWaveBits) {
+      cmask |= 1 << p;
+      cones |= 1 << p;
+    } else {
+      ins.pos[nh++] = p - kWaveBits;
+      ins.ones |= u64(1) << (p - kWaveBits);
+    }
+  }
+  ins.npos = nh;
+  const u64 nchunks = u64(1) << (nqubits - kWaveBits - nh);
+  const int N = 1 << k;
+  void* dmat = nullptr;
+  int ticket = -1;
+  *rc = ring_upload(dense, (size_t)N * N * sizeof(cplx<R>), s, &dmat, &ticket);
+  if (*rc) return 1;
+  const u64 waves = (nchunks + kWaveIlp - 1) / kWaveIlp;
+  const u64 grid = grid_for(waves, kBlock / 64);
+  const cplx<R>* dm = static_cast<const cplx<R>*>(dmat);
+#define QJ_WAVE(KK)                                                                 \
+  {                                                                                 \
+    WaveSpec<KK> ws;                                                                \
+    for (int u = 0; u < KK; ++u) ws.tb[u] = tb[u];                                  \
+    for (int d = 0; d < (1 << KK); ++d) {                                           \
+      int x = 0;                                                                    \
+      for (int u = 0; u < KK; ++u)                                                  \
+        if ((d >> (KK - 1 - u)) & 1) x |= 1 << tb[u];                               \
+      ws.xmask[d] = x;                                                              \
+    }                                                                               \
+    ws.cmask = cmask;                                                               \
+    ws.cones = cones;                                                               \
+    k_wave<R, KK><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, nchunks, ins, ws, dm); \
+  }
+  if (k == 1) QJ_WAVE(1) else QJ_WAVE(2)
+#undef QJ_WAVE
+  *rc = check_launch("wave gate launch");
+  ring_release(ticket, s);
+  return 1;
+}
+
 static int build_spec(int nqubits, const int* targets, int ntargets, const int32_t* qubits,
                       int nq, Ins* ins, const char* who) {
   if (nqubits < 1 || nqubits > 62)
@@ -275,6 +438,12 @@ static int one_qubit_impl(void* state, int nqubits, int kind, int m, const int32
   } else if (kind != QJ_APPLY_X && kind != QJ_APPLY_Y && kind != QJ_APPLY_Z) {
     return set_error(QJ_EVALUE, "apply_one_qubit: unknown kind %d", kind);
   }
+  {
+    cplx<R> dense[4];
+    dense_one<R>(kind, g.m, dense);
+    int wrc = QJ_OK;
+    if (try_wave<R>(static_cast<cplx<R>*>(state), nqubits, &m, 1, ins, dense, s, &wrc)) return wrc;
+  }
   const u64 npairs = u64(1) << (nqubits - ins.npos);
   // complex64 lanes move two amplitudes (16 B) when bit 0 is free
   const bool vec = sizeof(R) == 4 && ins.pos[0] >= 1;
@@ -316,6 +485,12 @@ static int two_qubit_impl(void* state, int nqubits, int kind, int bit1, int bit2
     std::memcpy(g.m, gate, (kind == QJ_APPLY_FSIM ? 5 : 16) * sizeof(cplx<R>));
   } else if (kind != QJ_APPLY_SWAP) {
     return set_error(QJ_EVALUE, "apply_two_qubit: unknown kind %d", kind);
+  }
+  {
+    cplx<R> dense[16];
+    dense_two<R>(kind, g.m, dense);
+    int wrc = QJ_OK;
+    if (try_wave<R>(static_cast<cplx<R>*>(state), nqubits, tb, 2, ins, dense, s, &wrc)) return wrc;
   }
   const u64 nquads = u64(1) << (nqubits - ins.npos);
   const bool vec = sizeof(R) == 4 && ins.pos[0] >= 1;
@@ -366,6 +541,7 @@ static int multi_qubit_impl(void* state, int nqubits, const int32_t* target_bits
   const int K = ntargets, N = 1 << K;
   const u64 ngroups = u64(1) << (nqubits - ins.npos);
   cplx<R>* st = static_cast<cplx<R>*>(state);
+
   const size_t mat_bytes = (size_t)N * N * sizeof(cplx<R>);
   if (K <= 5) {
     void* dmat = nullptr;
