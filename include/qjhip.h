@@ -271,6 +271,14 @@ int qj_collapse_state(void* state, int dtype, int nqubits,
 int qj_calculate_probabilities(const void* state, int dtype, int nqubits,
                                const int32_t* qubits, int nmeasured,
                                void* probs, int prob_dtype, void* stream);
+/* Marginal probabilities of a density matrix (2^n x 2^n, row-major): the
+ * diagonal Re rho_kk summed over the unmeasured qubits, same output order
+ * and dtypes as qj_calculate_probabilities.  Replaces qibo's density-matrix
+ * branch of calculate_probabilities, which CupyBackend.calculate_probabilities
+ * forwards to (gpu.py:751-768). */
+int qj_calculate_probabilities_dm(const void* state, int dtype, int nqubits,
+                                  const int32_t* qubits, int nmeasured,
+                                  void* probs, int prob_dtype, void* stream);
 
 /* Metropolis frequency sampler (ops.measure_frequencies, ops.py:82-108):
  * `nchains` chains; chain c re-seeds an MT19937 generator with

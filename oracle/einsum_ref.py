@@ -104,6 +104,40 @@ def probabilities(state, qubits):
     return p.reshape(-1)
 
 
+def projector(nqubits, qubits, results):
+    """Diagonal 0/1 projector onto ``qubits`` = ``results`` (qubit q = bit n-1-q)."""
+    idx = np.arange(2**nqubits)
+    keep = np.ones(idx.size, dtype=bool)
+    for q, r in zip(qubits, results):
+        keep &= ((idx >> (nqubits - 1 - q)) & 1) == r
+    return np.diag(keep.astype(np.complex128))
+
+
+def collapse_density_matrix(rho, qubits, results, normalize=True):
+    """P rho P, divided by its trace when ``normalize`` (qibo's density-matrix
+    collapse: every entry whose row or column disagrees with the outcome is
+    zeroed)."""
+    rho = np.asarray(rho, dtype=np.complex128)
+    n = int(np.log2(rho.shape[0]))
+    p = projector(n, qubits, results)
+    out = p @ rho @ p
+    if normalize:
+        out = out / np.trace(out).real
+    return out
+
+
+def probabilities_density_matrix(rho, qubits):
+    """Marginal of the diagonal of ``rho`` over ``qubits`` (first = MSB)."""
+    rho = np.asarray(rho, dtype=np.complex128)
+    n = int(np.log2(rho.shape[0]))
+    p = np.real(np.diag(rho)).reshape((2,) * n)
+    others = tuple(q for q in range(n) if q not in qubits)
+    p = p.sum(axis=others)
+    remaining = sorted(qubits)
+    p = p.transpose([remaining.index(q) for q in qubits])
+    return p.reshape(-1)
+
+
 def tfim_matrix(nqubits, g=1.0):
     """Dense H = -sum Z_i Z_{i+1} - g sum X_i (open chain)."""
     X = np.array([[0, 1], [1, 0]], dtype=complex)

@@ -47,6 +47,7 @@ SIGNATURES = {
     "qj_apply_tile_pass": [_vp, _i32, _i32, _i32, _pi32, _i32, _vp, _i32, _vp, _u64, _vp],
     "qj_collapse_state": [_vp, _i32, _i32, _pi32, _i32, _i64, _i32, _pdbl, _vp],
     "qj_calculate_probabilities": [_vp, _i32, _i32, _pi32, _i32, _vp, _i32, _vp],
+    "qj_calculate_probabilities_dm": [_vp, _i32, _i32, _pi32, _i32, _vp, _i32, _vp],
     "qj_measure_frequencies": [_vp, _vp, _i32, _i64, _i64, _pu32, _i32, _vp],
     "qj_norm_squared": [_vp, _i32, _u64, _pdbl, _vp],
     "qj_vdot": [_vp, _vp, _i32, _u64, _pdbl, _vp],

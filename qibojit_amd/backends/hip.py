@@ -449,9 +449,13 @@ class HipBackend(Backend):
 
     # ------------------------------------------------------- measurement --
     def collapse_state(self, state, qubits, shot, nqubits, normalize=True, density_matrix=False):
-        """Project onto outcome ``shot`` of ``qubits`` (gpu.py:616-644, cpu.py:541-563)."""
-        if density_matrix or len(tuple(state.shape)) == 2:
-            raise_error(NotImplementedError, "Density-matrix collapse is not implemented by the HIP backend yet.")
+        """Project onto outcome ``shot`` of ``qubits`` (gpu.py:616-644, cpu.py:541-563).
+
+        Density matrices (``density_matrix=True`` or a 2-d state) follow qibo's
+        density-matrix collapse: entries whose row or column disagrees with
+        the outcome are zeroed and, with ``normalize``, the rest is divided by
+        its trace.  (The reference's CUDA path ignores ``density_matrix`` and
+        runs the state-vector kernel on the flattened matrix.)"""
         if isinstance(shot, torch.Tensor):
             shot = int(shot.reshape(-1)[0])
         elif hasattr(shot, "shape"):
@@ -460,6 +464,8 @@ class HipBackend(Backend):
             shot = int(shot[0])
         else:
             shot = int(shot)
+        if density_matrix or len(tuple(state.shape)) == 2:
+            return self._collapse_density_matrix(state, qubits, shot, nqubits, normalize)
         state = self._as_state(state)
         qptr, keep, nq = i32_array([nqubits - q - 1 for q in reversed(list(qubits))])
         call(
@@ -477,19 +483,39 @@ class HipBackend(Backend):
         del keep
         return state
 
+    def _collapse_density_matrix(self, state, qubits, shot, nqubits, normalize=True):
+        """rho -> P rho P (/ tr(P rho P)): the 2n-bit collapse keeps the
+        measured row bits and column bits equal to ``shot``."""
+        state = self._as_state(state)
+        qubits = list(qubits)
+        k = len(qubits)
+        cols = [nqubits - q - 1 for q in reversed(qubits)]
+        rows = [2 * nqubits - q - 1 for q in reversed(qubits)]
+        qptr, keep, nq = i32_array(cols + rows)
+        call("qj_collapse_state", _ptr(state), _NATIVE_CODES[state.dtype], 2 * nqubits, qptr, nq,
+             shot | (shot << k), 0, None, self._stream())
+        del keep
+        if normalize:
+            trace = float(self.calculate_probabilities(state, qubits, nqubits, density_matrix=True).sum().item())
+            alpha = (ctypes.c_double * 2)(1.0 / trace, 0.0)
+            beta = (ctypes.c_double * 2)(0.0, 0.0)
+            call("qj_axpby", _ptr(state), _ptr(state), _NATIVE_CODES[state.dtype], state.numel(), alpha, beta,
+                 self._stream())
+        return state
+
     def _collapse_statevector(self, state, qubits, shot, nqubits, normalize=True):
         return self.collapse_state(state, qubits, shot, nqubits, normalize)
 
     def calculate_probabilities(self, state, qubits, nqubits, density_matrix=False):
-        """Marginal |psi|^2 over ``qubits`` in list order, FP64 accumulation."""
-        if density_matrix or len(tuple(state.shape)) == 2:
-            raise_error(NotImplementedError, "Density-matrix probabilities are not implemented by the HIP backend yet.")
+        """Marginal |psi|^2 over ``qubits`` in list order, FP64 accumulation;
+        for a density matrix the marginal of its diagonal."""
+        dm = density_matrix or len(tuple(state.shape)) == 2
         state = self._as_state(state)
         rdt = _REAL_OF[state.dtype]
         probs = torch.empty(1 << len(qubits), dtype=rdt, device=self._torch_device)
         qptr, keep, nq = i32_array(list(qubits))
         call(
-            "qj_calculate_probabilities",
+            "qj_calculate_probabilities_dm" if dm else "qj_calculate_probabilities",
             _ptr(state),
             _NATIVE_CODES[state.dtype],
             nqubits,

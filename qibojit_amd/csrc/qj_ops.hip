@@ -100,9 +100,16 @@ __global__ void __launch_bounds__(kBlock)
 // Few outcomes (2^k <= 32): each thread keeps the 2^k partial sums in
 // registers while it walks the free index; workgroup partials are then
 // summed by k_finalize.
+// Weight of basis index i: |psi_i|^2 for a state vector (dstride = 0), or
+// Re rho_ii of a density matrix stored row-major (dstride = 2^n + 1).
+template <typename R>
+__device__ __forceinline__ double prob_at(const cplx<R>* st, u64 i, u64 dstride) {
+  return dstride ? (double)st[i * dstride].re : abs2(st[i]);
+}
+
 template <typename R, int NB>
 __global__ void __launch_bounds__(kBlock)
-    k_probs_bins(const cplx<R>* __restrict__ st, u64 nfree, Ins fins, Offs32 hoff,
+    k_probs_bins(const cplx<R>* __restrict__ st, u64 nfree, Ins fins, Offs32 hoff, u64 dstride,
                  double* __restrict__ partials) {
   __shared__ double wsum[kBlock / 64];
   double acc[NB];
@@ -112,7 +119,7 @@ __global__ void __launch_bounds__(kBlock)
   for (u64 g = (u64)blockIdx.x * kBlock + threadIdx.x; g < nfree; g += stride) {
     const u64 base = insert_bits(g, fins);
 #pragma unroll
-    for (int h = 0; h < NB; ++h) acc[h] += abs2(st[base + hoff.o[h]]);
+    for (int h = 0; h < NB; ++h) acc[h] += prob_at(st, base + hoff.o[h], dstride);
   }
 #pragma unroll
   for (int h = 0; h < NB; ++h) {
@@ -125,7 +132,7 @@ __global__ void __launch_bounds__(kBlock)
 template <typename R, typename T>
 __global__ void __launch_bounds__(kBlock)
     k_probs_wide(const cplx<R>* __restrict__ st, u64 nbins, u64 nfree, u64 nchunks, Ins fins,
-                 Ins obits, double* __restrict__ partials, T* __restrict__ out) {
+                 Ins obits, u64 dstride, double* __restrict__ partials, T* __restrict__ out) {
   const u64 total = nbins * nchunks;
   const u64 per = (nfree + nchunks - 1) / nchunks;
   const u64 stride = (u64)gridDim.x * kBlock;
@@ -137,7 +144,7 @@ __global__ void __launch_bounds__(kBlock)
       if ((h >> (obits.npos - 1 - j)) & 1) off |= u64(1) << obits.pos[j];
     const u64 g0 = c * per, g1 = (g0 + per < nfree) ? g0 + per : nfree;
     double acc = 0.0;
-    for (u64 g = g0; g < g1; ++g) acc += abs2(st[insert_bits(g, fins) | off]);
+    for (u64 g = g0; g < g1; ++g) acc += prob_at(st, insert_bits(g, fins) | off, dstride);
     if (nchunks == 1) out[h] = (T)acc;
     else partials[c * nbins + h] = acc;
   }
@@ -538,7 +545,10 @@ static int collapse_impl(void* state, int nqubits, const int32_t* qk, int nm, in
 
 template <typename R, typename T>
 static int probs_impl(const void* state, int nqubits, const int32_t* qubits, int nm, T* out,
-                      hipStream_t s) {
+                      hipStream_t s, bool density_matrix = false) {
+  if (density_matrix && (nqubits < 1 || nqubits > 31))
+    return set_error(QJ_EVALUE, "calculate_probabilities: bad density-matrix nqubits %d", nqubits);
+  const u64 dstride = density_matrix ? (u64(1) << nqubits) + 1 : 0;
   if (nqubits < 1 || nqubits > 62) return set_error(QJ_EVALUE, "calculate_probabilities: bad nqubits");
   if (nm < 1 || nm > nqubits || nm > kMaxPos)
     return set_error(QJ_EVALUE, "calculate_probabilities: bad number of qubits %d", nm);
@@ -579,11 +589,11 @@ static int probs_impl(const void* state, int nqubits, const int32_t* qubits, int
     const u64 grid = std::min<u64>(kRedBlocks, grid_for(nfree, kBlock));
     dim3 gd((unsigned)grid), bd(kBlock);
     switch (nm) {
-      case 1: k_probs_bins<R, 2><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, sc->partials); break;
-      case 2: k_probs_bins<R, 4><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, sc->partials); break;
-      case 3: k_probs_bins<R, 8><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, sc->partials); break;
-      case 4: k_probs_bins<R, 16><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, sc->partials); break;
-      default: k_probs_bins<R, 32><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, sc->partials); break;
+      case 1: k_probs_bins<R, 2><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, dstride, sc->partials); break;
+      case 2: k_probs_bins<R, 4><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, dstride, sc->partials); break;
+      case 3: k_probs_bins<R, 8><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, dstride, sc->partials); break;
+      case 4: k_probs_bins<R, 16><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, dstride, sc->partials); break;
+      default: k_probs_bins<R, 32><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, dstride, sc->partials); break;
     }
     rc = check_launch("probabilities launch");
     if (rc) return rc;
@@ -596,7 +606,7 @@ static int probs_impl(const void* state, int nqubits, const int32_t* qubits, int
   if (nchunks < 1) nchunks = 1;
   const u64 items = nbins * nchunks;
   k_probs_wide<R, T><<<dim3((unsigned)grid_for(items, kBlock)), dim3(kBlock), 0, s>>>(
-      st, nbins, nfree, nchunks, fins, obits, sc->partials, out);
+      st, nbins, nfree, nchunks, fins, obits, dstride, sc->partials, out);
   rc = check_launch("probabilities launch");
   if (rc || nchunks == 1) return rc;
   k_finalize<T><<<dim3((unsigned)nbins), dim3(kBlock), 0, s>>>(sc->partials, nchunks, 1, nbins, out);
@@ -691,6 +701,21 @@ int qj_collapse_state(void* state, int dtype, int nqubits, const int32_t* qubits
   if (dtype == QJ_COMPLEX128)
     return collapse_impl<double>(state, nqubits, qubits_kernel, nmeasured, result, normalize, norm_out, s);
   return set_error(QJ_EVALUE, "collapse_state: dtype must be complex64/complex128");
+}
+
+int qj_calculate_probabilities_dm(const void* state, int dtype, int nqubits, const int32_t* qubits,
+                                  int nmeasured, void* probs, int prob_dtype, void* stream) {
+  if (!state || !qubits || !probs) return set_error(QJ_EVALUE, "calculate_probabilities: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == QJ_COMPLEX64 && prob_dtype == QJ_FLOAT32)
+    return probs_impl<float, float>(state, nqubits, qubits, nmeasured, (float*)probs, s, true);
+  if (dtype == QJ_COMPLEX64 && prob_dtype == QJ_FLOAT64)
+    return probs_impl<float, double>(state, nqubits, qubits, nmeasured, (double*)probs, s, true);
+  if (dtype == QJ_COMPLEX128 && prob_dtype == QJ_FLOAT32)
+    return probs_impl<double, float>(state, nqubits, qubits, nmeasured, (float*)probs, s, true);
+  if (dtype == QJ_COMPLEX128 && prob_dtype == QJ_FLOAT64)
+    return probs_impl<double, double>(state, nqubits, qubits, nmeasured, (double*)probs, s, true);
+  return set_error(QJ_EVALUE, "calculate_probabilities: unsupported dtype pair");
 }
 
 int qj_calculate_probabilities(const void* state, int dtype, int nqubits, const int32_t* qubits,

@@ -187,3 +187,20 @@ def test_compare_states_mixed_precision(backend, rng):
     assert res["infidelity"] == pytest.approx(want_inf, abs=1e-12)
     same = backend.compare_states(backend.cast(a), backend.cast(a))
     assert same["max_abs_diff"] == 0.0 and abs(same["infidelity"]) < 1e-14
+
+
+@pytest.mark.parametrize("qubits,shot", [([1], 1), ([0, 3], 2), ([3, 1, 2], 5), ([2, 0], 0)])
+def test_density_matrix_probabilities_and_collapse(backend, dtype, qubits, shot):
+    rng = np.random.default_rng(17 + shot)
+    n = 4
+    backend.set_dtype(dtype)
+    rho = gm.random_density_matrix(n, rng)
+    probs = backend.to_numpy(backend.calculate_probabilities(backend.cast(rho.copy()), qubits, n, density_matrix=True))
+    tol = 1e-5 if dtype == "complex64" else 1e-12
+    np.testing.assert_allclose(probs, er.probabilities_density_matrix(rho, qubits), atol=tol)
+    results = [(shot >> (len(qubits) - 1 - j)) & 1 for j in range(len(qubits))]
+    for normalize in (True, False):
+        got = backend.to_numpy(backend.collapse_state(backend.cast(rho.copy()), qubits, shot, n, normalize=normalize,
+                                                      density_matrix=True))
+        want = er.collapse_density_matrix(rho, qubits, results, normalize=normalize)
+        np.testing.assert_allclose(got, want, atol=tol)

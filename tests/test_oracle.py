@@ -335,3 +335,24 @@ def test_circuit_fuse_preserves_action():
             else:
                 psi = port.PortBackend(4).apply_gate(g, psi, n)
         np.testing.assert_allclose(psi, ref, atol=1e-12)
+
+
+def test_density_matrix_collapse_and_probabilities_pinned_by_pure_states():
+    """For rho = |psi><psi| the density-matrix forms reduce to the
+    state-vector closed forms; for a mixture they are linear."""
+    rng = np.random.default_rng(3)
+    n, qubits, results = 4, [2, 0], [1, 0]
+    psi = gm.random_state(n, rng)
+    rho = np.outer(psi, psi.conj())
+    np.testing.assert_allclose(er.probabilities_density_matrix(rho, qubits), er.probabilities(psi, qubits),
+                               atol=1e-14)
+    c = er.collapse(psi, qubits, results)
+    np.testing.assert_allclose(er.collapse_density_matrix(rho, qubits, results), np.outer(c, c.conj()), atol=1e-14)
+    phi = gm.random_state(n, rng)
+    mix = 0.3 * rho + 0.7 * np.outer(phi, phi.conj())
+    np.testing.assert_allclose(er.probabilities_density_matrix(mix, qubits),
+                               0.3 * er.probabilities(psi, qubits) + 0.7 * er.probabilities(phi, qubits), atol=1e-14)
+    un = er.collapse_density_matrix(mix, qubits, results, normalize=False)
+    cp = er.collapse(psi, qubits, results, normalize=False)
+    cf = er.collapse(phi, qubits, results, normalize=False)
+    np.testing.assert_allclose(un, 0.3 * np.outer(cp, cp.conj()) + 0.7 * np.outer(cf, cf.conj()), atol=1e-14)
