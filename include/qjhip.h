@@ -223,6 +223,19 @@ int qj_rot_pass(const void* psi_in, void* psi_out, const void* lam_in,
                 const int32_t* sel, int nrows, const int32_t* bits, int nbits,
                 int nrot, const int32_t* gens, int load, const void* mats,
                 uint64_t mstride, double* partials, int ncols, void* stream);
+/* The same pass on LDS tiles (complex64): 12 tile bits tile_bits[] (ascending;
+ * the lowest should be bits 0..3 so that loads are whole 128-byte lines) are
+ * held per workgroup, and the rotations of the tile bits set in rotmask run
+ * in three register phases with LDS transposes in between -- up to 12 bits
+ * per pass instead of QJ_ROT_MAX_BITS.  Arguments as qj_rot_pass, with
+ * matrices laid out [tile bit j][step][2x2] (mstride >= 12 * nrot * 4) and
+ * gradient slot j * nrot + step; needs 12 <= nqubits. */
+int qj_rot_tile_pass(const void* psi_in, void* psi_out, const void* lam_in,
+                     void* lam_out, int dtype, int nqubits, uint64_t stride,
+                     const int32_t* sel, int nrows, const int32_t* tile_bits,
+                     int rotmask, int nrot, const int32_t* gens, int load,
+                     const void* mats, uint64_t mstride, double* partials,
+                     int ncols, void* stream);
 /* Open-chain transverse-field Ising model over the bits,
  * H = -sum_j Z_j Z_{j+1} - g sum_q X_q.  For each row b: out = H' psi (out
  * may be NULL: energy only; it must not alias psi) and

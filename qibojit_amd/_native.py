@@ -55,6 +55,8 @@ SIGNATURES = {
     "qj_axpby": [_vp, _vp, _i32, _u64, _pdbl, _pdbl, _vp],
     "qj_rot_pass": [_vp, _vp, _vp, _vp, _i32, _i32, _u64, _vp, _i32, _pi32, _i32, _i32, _pi32, _i32,
                     _vp, _u64, _vp, _i32, _vp],
+    "qj_rot_tile_pass": [_vp, _vp, _vp, _vp, _i32, _i32, _u64, _vp, _i32, _pi32, _i32, _i32, _pi32, _i32,
+                         _vp, _u64, _vp, _i32, _vp],
     "qj_tfim_batch": [_vp, _vp, _i32, _i32, _u64, _vp, _i32, _dbl, _i32, _vp, _i32, _vp],
     "qj_reduce_rows": [_vp, _u64, _u64, _vp, _vp],
     "qj_compare_states": [_vp, _i32, _vp, _i32, _u64, _pdbl, _vp],

@@ -112,6 +112,50 @@ def interleaved_groups(nbits, k):
     return [tuple(range(i, nbits, ngroups)) for i in range(ngroups)]
 
 
+TILE_BITS = 12
+TILE_LOW = 4  # at least bits 0..3 in every tile: 16 complex64 = one 128-byte line
+TILE_LOW_MAX = 6  # bits 0..5: a wave's load is one 512-byte run
+
+
+class PassSpec:
+    """One pass of a layer: ``kind`` "rot" (register group, bits = rotated
+    bits) or "tile" (12 LDS tile bits, of which ``rot`` are rotated)."""
+
+    __slots__ = ("kind", "bits", "rot")
+
+    def __init__(self, kind, bits, rot):
+        self.kind, self.bits, self.rot = kind, tuple(bits), tuple(rot)
+
+    def __repr__(self):
+        return f"PassSpec({self.kind!r}, {self.bits}, rot={self.rot})"
+
+
+def register_passes(nbits, k):
+    return [PassSpec("rot", g, g) for g in interleaved_groups(nbits, k)]
+
+
+def tile_passes(nbits, tile=TILE_BITS, low=None):
+    """Cover bits 0..nbits-1 with LDS tile passes: the first tile is bits
+    0..tile-1, every later one keeps bits 0..low-1 (contiguous loads) and
+    rotates up to tile-low new bits, padded with already-rotated bits.
+    Without ``low``, the widest low block in [TILE_LOW, TILE_LOW_MAX] that
+    needs no extra pass."""
+    if nbits < tile:
+        raise ValueError(f"Tile passes need at least {tile} qubits.")
+    if low is None:
+        count = {w: 1 + -(-(nbits - tile) // (tile - w)) for w in range(TILE_LOW, TILE_LOW_MAX + 1)}
+        best = min(count.values())
+        low = max(w for w, c in count.items() if c == best)
+    passes = [PassSpec("tile", range(tile), range(tile))]
+    done = tile
+    while done < nbits:
+        new = list(range(done, min(nbits, done + tile - low)))
+        fill = [b for b in range(low, tile)][: tile - low - len(new)]
+        passes.append(PassSpec("tile", sorted(list(range(low)) + fill + new), new))
+        done += len(new)
+    return passes
+
+
 def _cols(ngroups, limit):
     """Workgroups per state: one group per thread, at most ``limit``."""
     want = max(1, -(-ngroups // ROT_BLOCK))
@@ -123,11 +167,16 @@ class AnsatzEngine:
 
     States of the batch live back to back in two ping-pong buffers (a pass
     that loads through the ladder permutation writes the other buffer);
-    gradients add a second pair for lambda.
+    gradients add a second pair for lambda.  With ``tiles`` (default for
+    complex64 and n >= 12) every forward pass is an LDS tile pass of 12 bits,
+    else a register pass of ``k_forward`` bits; ``tiles_adjoint`` does the
+    same for the adjoint passes (register passes of ``k_adjoint`` bits by
+    default: with psi and lambda the tile kernel holds 217 VGPRs and runs
+    at half the register kernel's bandwidth).
     """
 
     def __init__(self, ansatz, batch, dtype="complex64", g=1.0, k_forward=5, k_adjoint=4, device=None,
-                 adjoint_cols=256, tfim_cols=256, stream=None):
+                 adjoint_cols=256, tfim_cols=256, stream=None, tiles=None, tiles_adjoint=False):
         self.ansatz = ansatz
         self.batch = int(batch)
         if not 1 <= self.batch <= 65535:
@@ -138,26 +187,49 @@ class AnsatzEngine:
         n = ansatz.nqubits
         self.k_forward = int(min(k_forward, ROT_MAX_BITS, n))
         self.k_adjoint = int(min(k_adjoint, ROT_MAX_ADJ_BITS, n))
+        if tiles is None:
+            tiles = self.tdtype == torch.complex64 and n >= TILE_BITS
+        if (tiles or tiles_adjoint) and (self.tdtype != torch.complex64 or n < TILE_BITS):
+            raise ValueError("Tile passes need complex64 and at least 12 qubits.")
+        self.tiles = bool(tiles)
+        self.tiles_adjoint = bool(tiles_adjoint)
         self.device = torch.device("cuda", 0) if device is None else torch.device(device)
         self.stream = stream
         self.dim = 1 << n
-        self.groups_f = interleaved_groups(n, self.k_forward)
-        self.groups_a = interleaved_groups(n, self.k_adjoint)
-        self.cols_f = [_cols(self.dim >> len(b), 1 << 23) for b in self.groups_f]
+        self.passes_f = tile_passes(n) if self.tiles else register_passes(n, self.k_forward)
+        self.passes_a = tile_passes(n) if self.tiles_adjoint else register_passes(n, self.k_adjoint)
+        self.width_f = max(len(ps.bits) for ps in self.passes_f)
+        self.width_a = max(len(ps.bits) for ps in self.passes_a)
+        self.cols_f = [self._pass_cols(ps, 1 << 23) for ps in self.passes_f]
         # one column count for every adjoint pass keeps each pass's partials contiguous
-        self.cols_a = _cols(self.dim >> self.k_adjoint, adjoint_cols)
+        self.cols_a = min(self._pass_cols(ps, adjoint_cols) for ps in self.passes_a)
         self.tfim_cols = _cols(self.dim, tfim_cols)
         self.psi = torch.empty((2, self.batch, self.dim), dtype=self.tdtype, device=self.device)
         self.lam = None
         self.cur = 0
         self._forward_key = None
-        self._bits_f = [i32_array(b) for b in self.groups_f]
-        self._bits_a = [i32_array(b) for b in self.groups_a]
-        self._gens = [i32_array([GEN_Y, GEN_Z] * len(b)) for b in self.groups_a]
+        self._bits_f = [i32_array(ps.bits) for ps in self.passes_f]
+        self._bits_a = [i32_array(ps.bits) for ps in self.passes_a]
+        self._gens = [i32_array([g for b in ps.bits for g in ((GEN_Y, GEN_Z) if b in ps.rot else (0, 0))])
+                      for ps in self.passes_a]
         self.tfim_partials = torch.zeros((self.batch, self.tfim_cols), dtype=torch.float64, device=self.device)
         self.energies_dev = torch.zeros(self.batch, dtype=torch.float64, device=self.device)
 
+    # kept for callers that inspect the register grouping
+    @property
+    def groups_f(self):
+        return [ps.bits for ps in self.passes_f]
+
+    @property
+    def groups_a(self):
+        return [ps.bits for ps in self.passes_a]
+
     # ---------------------------------------------------------- helpers --
+    def _pass_cols(self, ps, limit):
+        if ps.kind == "tile":
+            return int(min(self.dim >> TILE_BITS, limit, MAX_COLS))
+        return _cols(self.dim >> len(ps.bits), limit)
+
     def _stream(self):
         s = self.stream if self.stream is not None else torch.cuda.current_stream(self.device)
         return ctypes.c_void_p(s.cuda_stream)
@@ -173,34 +245,43 @@ class AnsatzEngine:
         return p
 
     def _forward_mats(self, p):
-        """[L, G, B, k, 1, 4]: RZ @ RY per bit, groups padded to k bits."""
+        """[L, P, B, W, 1, 4]: RZ @ RY per rotated bit at its place in the pass."""
         ang = self.ansatz.angles(p)  # (B, L, 2, n)
         fused = rz_matrices(ang[:, :, 1]) @ ry_matrices(ang[:, :, 0])  # (B, L, n, 2, 2)
-        return self._pack(fused[:, :, :, None], self.groups_f, self.k_forward)
+        return self._pack(fused[:, :, :, None], self.passes_f, self.width_f)
 
     def _adjoint_mats(self, p):
-        """[L, G, B, k, 2, 4]: step 0 RY, step 1 RZ per bit."""
+        """[L, P, B, W, 2, 4]: step 0 RY, step 1 RZ per rotated bit."""
         ang = self.ansatz.angles(p)
         steps = np.stack([ry_matrices(ang[:, :, 0]), rz_matrices(ang[:, :, 1])], axis=3)  # (B, L, n, 2, 2, 2)
-        return self._pack(steps, self.groups_a, self.k_adjoint)
+        return self._pack(steps, self.passes_a, self.width_a)
 
-    def _pack(self, per_qubit, groups, k):
+    def _pack(self, per_qubit, passes, width):
         b, nl, _n, nsteps = per_qubit.shape[:4]
-        out = np.zeros((nl, len(groups), b, k, nsteps, 4), dtype=_NP[self.tdtype])
-        for i, bits in enumerate(groups):
-            for a, bit in enumerate(bits):
-                q = self._qubit_of(bit)
-                out[:, i, :, a] = per_qubit[:, :, q].reshape(b, nl, nsteps, 4).transpose(1, 0, 2, 3)
+        out = np.zeros((nl, len(passes), b, width, nsteps, 4), dtype=_NP[self.tdtype])
+        for i, ps in enumerate(passes):
+            for a, bit in enumerate(ps.bits):
+                if bit in ps.rot:
+                    q = self._qubit_of(bit)
+                    out[:, i, :, a] = per_qubit[:, :, q].reshape(b, nl, nsteps, 4).transpose(1, 0, 2, 3)
         return torch.from_numpy(out).to(self.device, non_blocking=False)
 
-    def _rot(self, src, dst, lsrc, ldst, bits, nrot, gens, load, mats, mstride, partials, ncols):
+    def _run_pass(self, ps, src, dst, lsrc, ldst, bits, nrot, gens, load, mats, mstride, partials, ncols):
         bits_ptr, _keep, nbits = bits
         gens_ptr = gens[0] if gens is not None else None
-        call(
-            "qj_rot_pass", _vp(src), _vp(dst), _vp(lsrc), _vp(ldst), self.code, self.ansatz.nqubits,
-            self.dim, None, self.batch, bits_ptr, nbits, nrot, gens_ptr, load, _vp(mats), mstride,
-            _vp(partials), ncols, self._stream(),
-        )
+        if ps.kind == "tile":
+            rotmask = sum(1 << a for a, b in enumerate(ps.bits) if b in ps.rot)
+            call(
+                "qj_rot_tile_pass", _vp(src), _vp(dst), _vp(lsrc), _vp(ldst), self.code, self.ansatz.nqubits,
+                self.dim, None, self.batch, bits_ptr, rotmask, nrot, gens_ptr, load, _vp(mats), mstride,
+                _vp(partials), ncols, self._stream(),
+            )
+        else:
+            call(
+                "qj_rot_pass", _vp(src), _vp(dst), _vp(lsrc), _vp(ldst), self.code, self.ansatz.nqubits,
+                self.dim, None, self.batch, bits_ptr, nbits, nrot, gens_ptr, load, _vp(mats), mstride,
+                _vp(partials), ncols, self._stream(),
+            )
 
     def _reduce(self, partials, out):
         rows = partials.numel() // partials.shape[-1]
@@ -215,7 +296,7 @@ class AnsatzEngine:
         psi = self.psi
         cur = 0
         for layer in range(self.ansatz.nlayers):
-            for i, bits in enumerate(self.groups_f):
+            for i, ps in enumerate(self.passes_f):
                 if layer == 0 and i == 0:
                     src, dst, load = None, psi[cur], LOAD_ZERO
                 elif i == 0:
@@ -223,9 +304,8 @@ class AnsatzEngine:
                     cur = 1 - cur
                 else:
                     src, dst, load = psi[cur], psi[cur], LOAD_PLAIN
-                k = self.k_forward
-                self._rot(src, dst, None, None, self._bits_f[i], 1, None, load, mats[layer, i], k * 4, None,
-                          self.cols_f[i])
+                self._run_pass(ps, src, dst, None, None, self._bits_f[i], 1, None, load, mats[layer, i],
+                               self.width_f * 4, None, self.cols_f[i])
         self.cur = cur
         self._forward_key = p.tobytes()
         self._mats_keepalive = mats
@@ -261,16 +341,16 @@ class AnsatzEngine:
         if self.lam is None:
             self.lam = torch.empty_like(self.psi)
         mats = self._adjoint_mats(p)
-        nl, ga, ka = self.ansatz.nlayers, len(self.groups_a), self.k_adjoint
+        nl, npass, width = self.ansatz.nlayers, len(self.passes_a), self.width_a
         ncols = self.cols_a
-        partials = torch.zeros((nl, ga, ka * 2, self.batch, ncols), dtype=torch.float64, device=self.device)
+        partials = torch.zeros((nl, npass, width * 2, self.batch, ncols), dtype=torch.float64, device=self.device)
         # lambda = L^+ H L psi on the pre-ladder states
         lcur = 0
         self._tfim(self.lam[lcur])
         psi, lam = self.psi, self.lam
         cur = self.cur
         for layer in reversed(range(nl)):
-            for i, bits in enumerate(self.groups_a):
+            for i, ps in enumerate(self.passes_a):
                 if i == 0 and layer < nl - 1:
                     load = LOAD_UNLADDER
                     src, dst, lsrc, ldst = psi[cur], psi[1 - cur], lam[lcur], lam[1 - lcur]
@@ -278,8 +358,8 @@ class AnsatzEngine:
                 else:
                     load = LOAD_PLAIN
                     src, dst, lsrc, ldst = psi[cur], psi[cur], lam[lcur], lam[lcur]
-                self._rot(src, dst, lsrc, ldst, self._bits_a[i], 2, self._gens[i], load, mats[layer, i], ka * 2 * 4,
-                          partials[layer, i], ncols)
+                self._run_pass(ps, src, dst, lsrc, ldst, self._bits_a[i], 2, self._gens[i], load, mats[layer, i],
+                               width * 2 * 4, partials[layer, i], ncols)
         self.cur = cur
         self._forward_key = None  # psi now holds the initial states again
         sums = torch.empty(partials.shape[:-1], dtype=torch.float64, device=self.device)
@@ -287,12 +367,14 @@ class AnsatzEngine:
         return self.energies_dev.clone(), self._scatter(sums)
 
     def _scatter(self, sums):
-        """(L, G, k*2, B) slot sums -> (B, P) gradients."""
+        """(L, P, W*2, B) slot sums -> (B, P) gradients."""
         n = self.ansatz.nqubits
-        index = np.full((self.ansatz.nlayers, len(self.groups_a), self.k_adjoint * 2), -1, dtype=np.int64)
+        index = np.full((self.ansatz.nlayers, len(self.passes_a), self.width_a * 2), -1, dtype=np.int64)
         for layer in range(self.ansatz.nlayers):
-            for i, bits in enumerate(self.groups_a):
-                for a, bit in enumerate(bits):
+            for i, ps in enumerate(self.passes_a):
+                for a, bit in enumerate(ps.bits):
+                    if bit not in ps.rot:
+                        continue
                     q = n - 1 - bit
                     for j in range(2):
                         index[layer, i, a * 2 + j] = self.ansatz.index(layer, j, q)

@@ -12,6 +12,8 @@ paths, chosen per n by :func:`autotune` (or given as a plan):
   ``qj_rot_pass`` over the circuits still running at that layer (the ladder
   is folded into the next layer's first pass), then one ``qj_tfim_batch``
   and ``qj_reduce_rows`` give the FP64 energies.
+* ``("tile",)``: as ``pass`` with LDS tile passes (``qj_rot_tile_pass``,
+  12 bits each, complex64, n >= 12).
 
 All inputs live in static device buffers, so :meth:`AnsatzBatch.capture`
 records the whole sequence once into a ``torch.cuda.CUDAGraph`` (a hipGraph
@@ -35,10 +37,12 @@ from qibojit_amd.ansatz import (
     LOAD_PLAIN,
     LOAD_ZERO,
     ROT_BLOCK,
+    TILE_BITS,
     HardwareEfficientAnsatz,
-    interleaved_groups,
+    register_passes,
     ry_matrices,
     rz_matrices,
+    tile_passes,
 )
 
 _CODES = {torch.complex64: _native.QJ_COMPLEX64, torch.complex128: _native.QJ_COMPLEX128}
@@ -59,7 +63,8 @@ def _fused_layers(n, nlayers, params):
 
 
 class _PassGroup:
-    """Device buffers and launch list of the pass path for one qubit count."""
+    """Device buffers and launch list of the pass path for one qubit count:
+    register passes of k bits, or (k = None) LDS tile passes."""
 
     def __init__(self, n, members, layers, k, tdtype, device, tfim_cols=64):
         self.n, self.members, self.k = n, members, k
@@ -68,10 +73,12 @@ class _PassGroup:
         self.cnt = cnt
         self.dim = 1 << n
         self.lmax = int(self.layers.max())
-        self.groups = interleaved_groups(n, k)
-        self.bits = [i32_array(b) for b in self.groups]
+        self.passes = tile_passes(n) if k is None else register_passes(n, k)
+        self.width = max(len(ps.bits) for ps in self.passes)
+        self.bits = [i32_array(ps.bits) for ps in self.passes]
+        self.rotmask = [sum(1 << a for a, b in enumerate(ps.bits) if b in ps.rot) for ps in self.passes]
         self.psi = torch.empty((2, cnt, self.dim), dtype=tdtype, device=device)
-        self.mats = torch.zeros((self.lmax, len(self.groups), cnt, k, 1, 4), dtype=tdtype, device=device)
+        self.mats = torch.zeros((self.lmax, len(self.passes), cnt, self.width, 1, 4), dtype=tdtype, device=device)
         self.sel = []
         for layer in range(self.lmax):
             active = np.nonzero(self.layers > layer)[0]
@@ -82,15 +89,17 @@ class _PassGroup:
         want = -(-self.dim // ROT_BLOCK)
         self.tfim_cols = int(min(tfim_cols, want))
         self.partials = torch.zeros((cnt, self.tfim_cols), dtype=torch.float64, device=device)
-        self.cols = [int(min(-(-(self.dim >> len(b)) // ROT_BLOCK), 65535)) for b in self.groups]
+        self.cols = [int(min(self.dim >> TILE_BITS, 65535)) if ps.kind == "tile"
+                     else int(min(-(-(self.dim >> len(ps.bits)) // ROT_BLOCK), 65535)) for ps in self.passes]
 
     def pack(self, fused):
         """fused: list over members of (L_c, n, 4) -> device matrices."""
         host = np.zeros(tuple(self.mats.shape), dtype=_NP[self.mats.dtype])
         for s, m in enumerate(fused):
-            for i, bits in enumerate(self.groups):
-                for a, bit in enumerate(bits):
-                    host[: m.shape[0], i, s, a, 0] = m[:, bit]
+            for i, ps in enumerate(self.passes):
+                for a, bit in enumerate(ps.bits):
+                    if bit in ps.rot:
+                        host[: m.shape[0], i, s, a, 0] = m[:, bit]
         self.mats.copy_(torch.from_numpy(host))
 
     def launch(self, code, g, stream, energies_out):
@@ -104,9 +113,14 @@ class _PassGroup:
                     src, dst, load = psi[(layer - 1) % 2], psi[layer % 2], LOAD_LADDER
                 else:
                     src, dst, load = psi[layer % 2], psi[layer % 2], LOAD_PLAIN
-                call("qj_rot_pass", _vp(src), _vp(dst), None, None, code, self.n, self.dim, _vp(sel),
-                     self.nsel[layer], bits_ptr, nbits, 1, None, load, _vp(self.mats[layer, i]), self.k * 4, None,
-                     self.cols[i], stream)
+                if self.passes[i].kind == "tile":
+                    call("qj_rot_tile_pass", _vp(src), _vp(dst), None, None, code, self.n, self.dim, _vp(sel),
+                         self.nsel[layer], bits_ptr, self.rotmask[i], 1, None, load, _vp(self.mats[layer, i]),
+                         self.width * 4, None, self.cols[i], stream)
+                else:
+                    call("qj_rot_pass", _vp(src), _vp(dst), None, None, code, self.n, self.dim, _vp(sel),
+                         self.nsel[layer], bits_ptr, nbits, 1, None, load, _vp(self.mats[layer, i]),
+                         self.width * 4, None, self.cols[i], stream)
         call("qj_tfim_batch", _vp(psi), None, code, self.n, self.dim, _vp(self.tfim_sel), self.cnt, g, 1,
              _vp(self.partials), self.tfim_cols, stream)
         call("qj_reduce_rows", _vp(self.partials), self.cnt, self.tfim_cols, _vp(energies_out), stream)
@@ -134,6 +148,8 @@ class AnsatzBatch:
                 raise ValueError(f"No plan for {n} qubits.")
             if self.plan[n][0] == "lds" and not LDS_MIN_QUBITS <= n <= LDS_MAX_QUBITS[self.tdtype]:
                 raise ValueError(f"{n} qubits do not fit the LDS path.")
+            if self.plan[n][0] == "tile" and (n < TILE_BITS or self.tdtype != torch.complex64):
+                raise ValueError(f"The tile path needs complex64 and at least {TILE_BITS} qubits (got {n}).")
         self.energies_sorted = torch.zeros(ncirc, dtype=torch.float64, device=self.device)
         # LDS path: sorted positions 0..nlds-1 (smallest n first)
         sorted_n = nq[self.order]
@@ -163,8 +179,8 @@ class AnsatzBatch:
         pos = self.nlds
         for n in sorted({int(v) for v in sorted_n[self.nlds:]}):
             members = [c for c in self.order[self.nlds:] if self.shapes[c][0] == n]
-            grp = _PassGroup(n, members, [self.shapes[c][1] for c in members], self.plan[n][1], self.tdtype,
-                             self.device)
+            k = None if self.plan[n][0] == "tile" else self.plan[n][1]
+            grp = _PassGroup(n, members, [self.shapes[c][1] for c in members], k, self.tdtype, self.device)
             grp.offset = pos
             pos += len(members)
             self.pass_groups.append(grp)
@@ -243,6 +259,8 @@ def autotune(shapes, params, dtype="complex64", g=1.0, reps=3, ks=(3, 4, 5)):
         choices = [("pass", k) for k in ks if k <= n]
         if LDS_MIN_QUBITS <= n <= LDS_MAX_QUBITS[tdtype]:
             choices.insert(0, ("lds",))
+        if n >= TILE_BITS and tdtype == torch.complex64:
+            choices.append(("tile",))
         timings[n] = {}
         for choice in choices:
             b = AnsatzBatch(sub_shapes, dtype=dtype, g=g, plan={n: choice})

@@ -287,6 +287,158 @@ __global__ void __launch_bounds__(kRotBlock) k_reduce_rows(const double* __restr
   if (threadIdx.x == 0) out[r] = acc;
 }
 
+// -------------------------------------------------- LDS-tiled passes ----
+// complex64 only.  A workgroup of 256 threads owns tiles of 2^12 amplitudes
+// (tile bit j = global bit tpos[j], ascending; the lowest tile bits should be
+// the lowest global bits so that loads are whole lines).  Each thread holds
+// 16 amplitudes; three register phases cover the tile bits 8..11, 4..7 and
+// 0..3, with LDS transposes in between:
+//   phase A: thread bits t[0..7] = tid,          registers t[8..11]
+//   phase B: thread bits t[0..3], t[8..11],      registers t[4..7]
+//   phase C: thread bits t[4..11],               registers t[0..3]
+// Loads and stores use the phase-A layout (lanes = t[0..5]).  LDS address
+// p(t) = t ^ ((t >> 4) & 31) spreads every phase's 64 lanes over all banks.
+// Tile bits outside `rotmask` are carried but not rotated (they were done by
+// an earlier pass of the layer).
+constexpr int kTileBits = 12;
+constexpr int kTileBlock = 256;
+
+struct RotTileArgs {
+  Ins ins;                 // tile positions, for insert_bits of the tile id
+  u64 offr[16];            // offset of register r in phase A (tile bits 8..11)
+  int tpos[kTileBits];     // global bit of tile bit j
+  int rotmask;             // tile bits rotated by this pass
+  int gen[kTileBits][QJ_ROT_MAX_STEPS];
+  int nrot, load;
+  u64 stride, mstride;
+  int nsel;
+};
+
+__device__ __forceinline__ unsigned tile_swz(unsigned t) { return t ^ ((t >> 4) & 31u); }
+
+template <int PH>
+__device__ __forceinline__ unsigned tile_t(unsigned tid, int r) {
+  if (PH == 0) return tid | ((unsigned)r << 8);
+  if (PH == 1) return (tid & 15u) | ((unsigned)r << 4) | ((tid >> 4) << 8);
+  return (unsigned)r | (tid << 4);
+}
+
+// Rotations of the phase's four register bits (tile bits B0..B0+3); NROT
+// steps per bit (compile time, so the FP64 accumulators stay in registers).
+template <bool ADJ, int NROT, int B0, int A>
+__device__ __forceinline__ void tile_rot(cplx<float> (&x)[16], cplx<float> (&y)[16], const cplx<float>* m,
+                                         const RotTileArgs& a, double (&acc)[kTileBits * NROT]) {
+  if constexpr (A < 4) {
+    constexpr int j = B0 + A;
+    if ((a.rotmask >> j) & 1) {
+      if (!ADJ) {
+#pragma unroll
+        for (int s = 0; s < NROT; ++s) rot_apply<float, 4, A, false>(x, m + (j * NROT + s) * 4);
+      } else {
+#pragma unroll
+        for (int s = NROT - 1; s >= 0; --s) {
+          const int g = a.gen[j][s];
+          if (g) acc[j * NROT + s] += rot_gen<float, 4, A>(x, y, g);
+          rot_apply<float, 4, A, true>(x, m + (j * NROT + s) * 4);
+          rot_apply<float, 4, A, true>(y, m + (j * NROT + s) * 4);
+        }
+      }
+    }
+    tile_rot<ADJ, NROT, B0, A + 1>(x, y, m, a, acc);
+  }
+}
+
+template <int PH>
+__device__ __forceinline__ void lds_put(cplx<float>* sm, const cplx<float> (&x)[16]) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) sm[tile_swz(tile_t<PH>(threadIdx.x, r))] = x[r];
+}
+template <int PH>
+__device__ __forceinline__ void lds_get(const cplx<float>* sm, cplx<float> (&x)[16]) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) x[r] = sm[tile_swz(tile_t<PH>(threadIdx.x, r))];
+}
+
+// Moves x from layout P to layout Q through the LDS buffer (and y after it,
+// through the same buffer, in adjoint mode).
+template <bool ADJ, int P, int Q>
+__device__ __forceinline__ void tile_transpose(cplx<float>* sm, cplx<float> (&x)[16], cplx<float> (&y)[16]) {
+  lds_put<P>(sm, x);
+  __syncthreads();
+  lds_get<Q>(sm, x);
+  if (ADJ) {
+    __syncthreads();
+    lds_put<P>(sm, y);
+    __syncthreads();
+    lds_get<Q>(sm, y);
+  }
+  __syncthreads();  // the buffer is free again
+}
+
+template <bool ADJ, int NROT>
+__global__ void __launch_bounds__(kTileBlock)
+    k_rot_tile(const cplx<float>* psi_in, cplx<float>* psi_out, const cplx<float>* lam_in, cplx<float>* lam_out,
+               const int32_t* __restrict__ sel, const cplx<float>* __restrict__ mats,
+               double* __restrict__ partials, u64 ntiles, RotTileArgs a) {
+  // one 32 KiB tile buffer (the kernel's only LDS); the gradient reduction
+  // reuses it after the last tile
+  extern __shared__ __align__(16) unsigned char tile_raw[];
+  cplx<float>* sm = reinterpret_cast<cplx<float>*>(tile_raw);
+  double* wsum = reinterpret_cast<double*>(tile_raw);
+  const int b = blockIdx.y;
+  const u64 s = sel ? (u64)sel[b] : (u64)b;
+  const cplx<float>* pin = psi_in + s * a.stride;
+  cplx<float>* pout = psi_out + s * a.stride;
+  const cplx<float>* lin = ADJ ? lam_in + s * a.stride : nullptr;
+  cplx<float>* lout = ADJ ? lam_out + s * a.stride : nullptr;
+  const cplx<float>* m = mats + s * a.mstride;
+  double acc[kTileBits * NROT];
+#pragma unroll
+  for (int i = 0; i < kTileBits * NROT; ++i) acc[i] = 0.0;
+  // offset of this thread's tile bits 0..7 (phase-A thread bits)
+  u64 offt = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if ((threadIdx.x >> j) & 1) offt |= u64(1) << a.tpos[j];
+  for (u64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const u64 base = insert_bits(tile, a.ins) + offt;
+    cplx<float> x[16], y[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const u64 gidx = base + a.offr[r];
+      if (a.load == QJ_LOAD_ZERO) {
+        x[r] = mk<float>(gidx == 0 ? 1.f : 0.f, 0.f);
+      } else {
+        const u64 src = load_index(gidx, a.load);
+        x[r] = pin[src];
+        if (ADJ) y[r] = lin[src];
+      }
+    }
+    tile_rot<ADJ, NROT, 8, 0>(x, y, m, a, acc);
+    tile_transpose<ADJ, 0, 1>(sm, x, y);
+    tile_rot<ADJ, NROT, 4, 0>(x, y, m, a, acc);
+    tile_transpose<ADJ, 1, 2>(sm, x, y);
+    tile_rot<ADJ, NROT, 0, 0>(x, y, m, a, acc);
+    tile_transpose<ADJ, 2, 0>(sm, x, y);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      pout[base + a.offr[r]] = x[r];
+      if (ADJ) lout[base + a.offr[r]] = y[r];
+    }
+  }
+  if (ADJ) {
+    const u64 cols = gridDim.x, rows = (u64)a.nsel;
+#pragma unroll
+    for (int j = 0; j < kTileBits; ++j)
+#pragma unroll
+      for (int st = 0; st < NROT; ++st)
+        if (((a.rotmask >> j) & 1) && a.gen[j][st]) {
+          const double t = rot_block_sum(acc[j * NROT + st], wsum);
+          if (threadIdx.x == 0) partials[((u64)(j * NROT + st) * rows + b) * cols + blockIdx.x] = t;
+        }
+  }
+}
+
 // ------------------------------------------------ whole circuit in LDS --
 // Persistent kernel for small ansatz circuits: workgroup w runs circuits
 // w, w + gridDim.x, ... entirely in LDS (2^n amplitudes, n <= nqmax) and
@@ -541,6 +693,85 @@ int qj_rot_pass(const void* psi_in, void* psi_out, const void* lam_in, void* lam
     return rot_pass_impl<double>(psi_in, psi_out, lam_in, lam_out, nqubits, stride, sel, nrows, bits, nbits, nrot,
                                  gens, load, mats, mstride, partials, ncols, st);
   return set_error(QJ_EVALUE, "rot_pass: dtype must be complex64/complex128");
+}
+
+int qj_rot_tile_pass(const void* psi_in, void* psi_out, const void* lam_in, void* lam_out, int dtype, int nqubits,
+                     uint64_t stride, const int32_t* sel, int nrows, const int32_t* tile_bits, int rotmask, int nrot,
+                     const int32_t* gens, int load, const void* mats, uint64_t mstride, double* partials, int ncols,
+                     void* stream) {
+  const char* what = "rot_tile_pass";
+  if ((!psi_in && load != QJ_LOAD_ZERO) || !psi_out || !tile_bits || !mats)
+    return set_error(QJ_EVALUE, "%s: null pointer", what);
+  if (dtype != QJ_COMPLEX64) return set_error(QJ_EVALUE, "%s: complex64 only", what);
+  if (nqubits < kTileBits || nqubits > 40) return set_error(QJ_EVALUE, "%s: needs 12..40 qubits, got %d", what, nqubits);
+  int rc = check_rows(nrows, sel, stride, nqubits, what);
+  if (rc) return rc;
+  if (nrot < 1 || nrot > QJ_ROT_MAX_STEPS) return set_error(QJ_EVALUE, "%s: bad step count %d", what, nrot);
+  if (rotmask <= 0 || rotmask >= (1 << kTileBits)) return set_error(QJ_EVALUE, "%s: bad rotation mask", what);
+  if (mstride < (u64)(kTileBits * nrot * 4) && !(sel == nullptr && nrows == 1))
+    return set_error(QJ_EVALUE, "%s: matrix stride below %d", what, kTileBits * nrot * 4);
+  if (load < QJ_LOAD_PLAIN || load > QJ_LOAD_ZERO) return set_error(QJ_EVALUE, "%s: bad load mode %d", what, load);
+  if (ncols < 1 || ncols > kRotMaxCols) return set_error(QJ_EVALUE, "%s: bad column count %d", what, ncols);
+  const bool adj = lam_in != nullptr;
+  if (adj != (lam_out != nullptr)) return set_error(QJ_EVALUE, "%s: lam_in and lam_out go together", what);
+  if (load == QJ_LOAD_ZERO && adj) return set_error(QJ_EVALUE, "%s: zero load is forward only", what);
+  if ((load == QJ_LOAD_LADDER || load == QJ_LOAD_UNLADDER) && (psi_in == psi_out || (adj && lam_in == lam_out)))
+    return set_error(QJ_EVALUE, "%s: a permuted load needs distinct input and output", what);
+  for (int j = 0; j < kTileBits; ++j) {
+    if (tile_bits[j] < 0 || tile_bits[j] >= nqubits) return set_error(QJ_EVALUE, "%s: bit %d out of range", what, tile_bits[j]);
+    if (j && tile_bits[j] <= tile_bits[j - 1]) return set_error(QJ_EVALUE, "%s: tile bits must be ascending and unique", what);
+  }
+  RotTileArgs a;
+  std::memset(&a, 0, sizeof(a));
+  bool any_gen = false;
+  if (gens) {
+    for (int j = 0; j < kTileBits; ++j)
+      for (int st = 0; st < nrot; ++st) {
+        const int g = gens[j * nrot + st];
+        if (g < 0 || g > 3) return set_error(QJ_EVALUE, "%s: bad generator %d", what, g);
+        a.gen[j][st] = g;
+        any_gen |= g != 0 && ((rotmask >> j) & 1);
+      }
+  }
+  if (adj && any_gen && !partials) return set_error(QJ_EVALUE, "%s: gradients need a partials buffer", what);
+  a.ins.npos = kTileBits;
+  for (int j = 0; j < kTileBits; ++j) {
+    a.ins.pos[j] = tile_bits[j];
+    a.tpos[j] = tile_bits[j];
+  }
+  for (int r = 0; r < 16; ++r) {
+    u64 o = 0;
+    for (int u = 0; u < 4; ++u)
+      if ((r >> u) & 1) o |= u64(1) << tile_bits[8 + u];
+    a.offr[r] = o;
+  }
+  a.rotmask = rotmask;
+  a.nrot = nrot;
+  a.load = load;
+  a.stride = stride;
+  a.mstride = mstride;
+  a.nsel = nrows;
+  const u64 ntiles = (u64(1) << nqubits) >> kTileBits;
+  dim3 gd((unsigned)ncols, (unsigned)nrows);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t lds = (size_t)8 << kTileBits;  // one complex64 tile
+  const cplx<float>* m = static_cast<const cplx<float>*>(mats);
+  const cplx<float>* pi = (const cplx<float>*)psi_in;
+  cplx<float>* po = (cplx<float>*)psi_out;
+  const cplx<float>* li = (const cplx<float>*)lam_in;
+  cplx<float>* lo = (cplx<float>*)lam_out;
+#define QJ_TILE(ADJ, NR) k_rot_tile<ADJ, NR><<<gd, dim3(kTileBlock), lds, st>>>(pi, po, li, lo, sel, m, partials, ntiles, a)
+  if (adj) {
+    if (nrot == 1) QJ_TILE(true, 1);
+    else if (nrot == 2) QJ_TILE(true, 2);
+    else QJ_TILE(true, 3);
+  } else {
+    if (nrot == 1) QJ_TILE(false, 1);
+    else if (nrot == 2) QJ_TILE(false, 2);
+    else QJ_TILE(false, 3);
+  }
+#undef QJ_TILE
+  return check_launch("rot_tile_pass launch");
 }
 
 int qj_tfim_batch(const void* psi, void* out, int dtype, int nqubits, uint64_t stride, const int32_t* sel,

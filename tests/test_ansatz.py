@@ -195,3 +195,49 @@ def test_tfim_and_reduce_argument_errors():
         _native.call("qj_tfim_batch", _fake(), None, _native.QJ_COMPLEX64, 1, 64, None, 1, 1.0, 0, _fake(), 1, None)
     with pytest.raises(ValueError, match="columns"):
         _native.call("qj_reduce_rows", _fake(), 3, 0, _fake(), None)
+
+
+@pytest.mark.parametrize("n", [12, 13, 19, 20, 24, 30])
+def test_tile_passes_cover_every_bit_once(n):
+    from qibojit_amd.ansatz import TILE_BITS, TILE_LOW, tile_passes
+
+    passes = tile_passes(n)
+    rotated = [b for ps in passes for b in ps.rot]
+    assert sorted(rotated) == list(range(n))
+    for ps in passes:
+        assert len(ps.bits) == TILE_BITS and list(ps.bits) == sorted(set(ps.bits))
+        assert set(ps.rot) <= set(ps.bits) and max(ps.bits) < n
+        assert set(range(TILE_LOW)) <= set(ps.bits)  # whole-line loads
+    assert len(passes) == 1 + -(-(n - TILE_BITS) // (TILE_BITS - TILE_LOW))  # no pass wasted on wider runs
+
+
+def test_tile_engine_packs_and_scatters(rng):
+    import torch
+
+    ansatz = HardwareEfficientAnsatz(14, 2)
+    eng = AnsatzEngine(ansatz, 2, dtype="complex64", device="cpu", tiles_adjoint=True)
+    assert eng.tiles and all(ps.kind == "tile" for ps in eng.passes_f + eng.passes_a)
+    p = ansatz.random_parameters(rng, 2)
+    adj = eng._adjoint_mats(p).numpy()
+    ang = p.reshape(2, 2, 2, 14)
+    for layer in range(2):
+        for i, ps in enumerate(eng.passes_a):
+            for a, bit in enumerate(ps.bits):
+                m = adj[layer, i, 1, a]
+                if bit in ps.rot:
+                    np.testing.assert_allclose(m[0].reshape(2, 2), RY(ang[1, layer, 0, 13 - bit]), atol=1e-7)
+                    np.testing.assert_allclose(m[1].reshape(2, 2), RZ(ang[1, layer, 1, 13 - bit]), atol=1e-7)
+                else:
+                    assert not m.any()
+    sums = torch.full((2, len(eng.passes_a), eng.width_a * 2, 2), np.nan, dtype=torch.float64)
+    for layer in range(2):
+        for i, ps in enumerate(eng.passes_a):
+            for a, bit in enumerate(ps.bits):
+                if bit in ps.rot:
+                    for j in range(2):
+                        for b in range(2):
+                            sums[layer, i, a * 2 + j, b] = 1000 * b + ansatz.index(layer, j, 13 - bit)
+    grads = eng._scatter(sums).numpy()
+    np.testing.assert_array_equal(grads, np.arange(ansatz.nparams)[None, :] + 1000 * np.arange(2)[:, None])
+    with pytest.raises(ValueError, match="complex64"):
+        AnsatzEngine(ansatz, 2, dtype="complex128", device="cpu", tiles=True)

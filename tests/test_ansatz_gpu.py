@@ -229,3 +229,100 @@ def test_tfim_batch_energy_and_apply(torch_mod, dtype, rng, ladder):
         assert np.max(np.abs(_np(out)[s] - hx)) < tol
         assert abs(_np(sums)[s] - e) < ENERGY[dtype]
         assert abs(_np(sums2)[s] - e) < ENERGY[dtype]
+
+
+@pytest.mark.parametrize("n,layers,batch", [(12, 2, 2), (14, 3, 3), (21, 1, 1)])
+def test_tile_engine_energies_states_and_gradients(torch_mod, rng, n, layers, batch):
+    """LDS tile passes (complex64, n >= 12) against the oracle and against
+    the register passes."""
+    from qibojit_amd.ansatz import AnsatzEngine, HardwareEfficientAnsatz
+
+    ansatz = HardwareEfficientAnsatz(n, layers)
+    p = ansatz.random_parameters(rng, batch)
+    tile = AnsatzEngine(ansatz, batch, dtype="complex64", tiles=True, tiles_adjoint=True)
+    reg = AnsatzEngine(ansatz, batch, dtype="complex64", tiles=False)
+    e_t, g_t = tile.gradients(p)
+    e_r, g_r = reg.gradients(p)
+    np.testing.assert_allclose(_np(e_t), _np(e_r), atol=ENERGY["complex64"])
+    assert np.max(np.abs(_np(g_t) - _np(g_r))) < GRAD["complex64"]
+    tile.energies(p)
+    states = _np(tile.final_states())
+    if n <= 14:
+        for b in range(batch):
+            want = ansatz_ref.hea_state(n, layers, p[b])
+            assert np.max(np.abs(states[b] - want)) < AMP["complex64"]
+        ps = ansatz_ref.parameter_shift(n, layers, p[0], 1.0, indices=range(0, ansatz.nparams, 5))
+        assert np.max(np.abs(_np(g_t)[0][::5] - ps)) < GRAD["complex64"]
+
+
+@pytest.mark.parametrize("load", [0, 1, 2])
+def test_rot_tile_pass_against_numpy(torch_mod, rng, load):
+    """Tile pass with a partial rotation mask, two steps, every load mode,
+    forward and adjoint (gradient partials and both output states)."""
+    from qibojit_amd import _native
+
+    torch = torch_mod
+    n, nstates, nrot = 14, 3, 2
+    tile_bits = [0, 1, 2, 3, 5, 6, 8, 9, 10, 11, 12, 13]
+    rot = [1, 3, 4, 6, 7, 9, 11]  # tile indices rotated
+    rotmask = sum(1 << j for j in rot)
+    host = np.stack([random_state(n, rng) for _ in range(nstates)]).astype(np.complex64)
+    lamh = np.stack([random_state(n, rng) for _ in range(nstates)]).astype(np.complex64)
+    mats = np.zeros((nstates, 12, nrot, 2, 2), dtype=np.complex128)
+    for s in range(nstates):
+        for j in range(12):
+            for st in range(nrot):
+                mats[s, j, st] = haar_unitary(2, rng)
+    gens = [g for j in range(12) for g in ((2, 3) if j in rot else (0, 0))]
+    idx = np.arange(1 << n)
+
+    def vp(t):
+        return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+    bits_ptr, _k1, _ = _native.i32_array(tile_bits)
+    gptr, _k2, _ = _native.i32_array(gens)
+    dm = torch.from_numpy(mats.astype(np.complex64).reshape(nstates, -1)).cuda()
+
+    def permuted(x):
+        if load == 1:
+            return x[idx ^ (idx >> 1)]
+        if load == 2:
+            return x[np.argsort(idx ^ (idx >> 1))]
+        return x
+
+    # forward
+    src = torch.from_numpy(host).cuda()
+    dst = src.clone() if load == 0 else torch.zeros_like(src)
+    _native.call("qj_rot_tile_pass", vp(src), vp(dst), None, None, _native.QJ_COMPLEX64, n, 1 << n, None, nstates,
+                 bits_ptr, rotmask, nrot, None, load, vp(dm), 12 * nrot * 4, None, 2, None)
+    got = _np(dst)
+    for s in range(nstates):
+        x = permuted(host[s].astype(np.complex128))
+        for j in rot:
+            for st in range(nrot):
+                x = einsum_ref.apply_gate(x, mats[s, j, st], [n - 1 - tile_bits[j]], nqubits=n)
+        assert np.max(np.abs(got[s] - x)) < AMP["complex64"]
+    # adjoint
+    src = torch.from_numpy(host).cuda()
+    lsrc = torch.from_numpy(lamh).cuda()
+    dst = src.clone() if load == 0 else torch.zeros_like(src)
+    ldst = lsrc.clone() if load == 0 else torch.zeros_like(lsrc)
+    ncols = 3
+    part = torch.zeros((12 * nrot, nstates, ncols), dtype=torch.float64).cuda()
+    _native.call("qj_rot_tile_pass", vp(src), vp(dst), vp(lsrc), vp(ldst), _native.QJ_COMPLEX64, n, 1 << n, None,
+                 nstates, bits_ptr, rotmask, nrot, gptr, load, vp(dm), 12 * nrot * 4, vp(part), ncols, None)
+    got, lgot, grad = _np(dst), _np(ldst), _np(part).sum(axis=-1)
+    paulis = {2: np.array([[0, -1j], [1j, 0]]), 3: np.diag([1.0, -1.0])}
+    for s in range(nstates):
+        x = permuted(host[s].astype(np.complex128))
+        y = permuted(lamh[s].astype(np.complex128))
+        for j in rot:
+            t = [n - 1 - tile_bits[j]]
+            for st in reversed(range(nrot)):
+                gx = einsum_ref.apply_gate(x, paulis[(2, 3)[st]], t, nqubits=n)
+                assert abs(grad[j * nrot + st, s] - np.vdot(y, gx).imag) < 1e-4
+                u = mats[s, j, st].conj().T
+                x = einsum_ref.apply_gate(x, u, t, nqubits=n)
+                y = einsum_ref.apply_gate(y, u, t, nqubits=n)
+        assert np.max(np.abs(got[s] - x)) < AMP["complex64"]
+        assert np.max(np.abs(lgot[s] - y)) < AMP["complex64"]

@@ -20,14 +20,18 @@ def _shapes_and_params(rng, shapes):
 SHAPES = [(3, 2), (9, 1), (5, 3), (14, 2), (12, 4), (2, 1), (15, 1), (9, 3), (14, 1), (13, 2), (4, 5)]
 
 
-@pytest.mark.parametrize("mode", ["default", "all_pass_k2", "all_pass_k5", "mixed"])
+@pytest.mark.parametrize("mode", ["default", "all_pass_k2", "all_pass_k5", "mixed", "tiles"])
 def test_batch_energies(hip_backend, dtype, rng, mode):
     from qibojit_amd.batch import AnsatzBatch
 
     params, want = _shapes_and_params(rng, SHAPES)
     ns = sorted({n for n, _ in SHAPES})
+    if mode == "tiles" and dtype != "complex64":
+        pytest.skip("tile passes are complex64")
     if mode == "default":
         plan = None
+    elif mode == "tiles":
+        plan = {n: ("tile",) if n >= 12 else ("lds",) for n in ns}
     elif mode.startswith("all_pass"):
         k = int(mode[-1])
         plan = {n: ("pass", min(k, n)) for n in ns}
