@@ -22,6 +22,7 @@
 // Matrix convention as qj_apply_multi_qubit: bit (4 - u) of the row index is
 // target target_bits[u] (the first target is the most significant bit).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -74,6 +75,60 @@ __global__ void __launch_bounds__(kMfmaBlock)
   }
 }
 
+// The same GEMM with v_mfma_f32_16x16x4f32 (16x16 outputs, K = 4): lane l,
+// n = l % 16, q = l / 16 supplies A[n][q], B[q][n] and holds D[4q + r][n] in
+// register r.  A wave owns 16 columns; lane (n, q) loads rows c = 4j + q
+// (j < 8) of column n -- its B operands for K-steps j (Re) and 8 + j (Im).
+// Four row tiles of 16: tiles 0/1 give Re Y of rows 0..31, tiles 2/3 Im Y of
+// the same rows.  Half the operand registers of the 32x32x2 form.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(kMfmaBlock)
+    k_block5_mfma16(cplx<float>* __restrict__ st, u64 ncols, Ins ins, Offs32 off, const float* __restrict__ w) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane >> 4, n = lane & 15;
+  float a[4][16];  // a[t][s] = W[16t + n][4s + q]
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int s = 0; s < 16; ++s) a[t][s] = w[(16 * t + n) * 64 + 4 * s + q];
+  const u64 step = (u64)gridDim.x * (kMfmaBlock / 64) * 16;
+  for (u64 c0 = ((u64)blockIdx.x * (kMfmaBlock / 64) + wave) * 16; c0 < ncols; c0 += step) {
+    const u64 base = insert_bits(c0 + n, ins);
+    float xr[8], xi[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const cplx<float> v = st[base + off.o[4 * j + q]];
+      xr[j] = v.re;
+      xi[j] = v.im;
+    }
+    v4f d[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) d[t] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) d[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t][s], xr[s], d[t], 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) d[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t][8 + s], xi[s], d[t], 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      st[base + off.o[4 * q + r]] = mk<float>(d[0][r], d[2][r]);
+      st[base + off.o[16 + 4 * q + r]] = mk<float>(d[1][r], d[3][r]);
+    }
+  }
+}
+
+static int mfma_shape() {
+  static const int shape = [] {
+    const char* e = std::getenv("QJ_MFMA_SHAPE");
+    return (e && std::atoi(e) == 32) ? 32 : 16;
+  }();
+  return shape;
+}
+
 static int block5_impl(void* state, int nqubits, const int32_t* target_bits, const void* gate, hipStream_t s) {
   if (nqubits < 10 || nqubits > 40)
     return set_error(QJ_EVALUE, "apply_block_mfma: needs 10..40 qubits (32-column tiles), got %d", nqubits);
@@ -113,11 +168,16 @@ static int block5_impl(void* state, int nqubits, const int32_t* target_bits, con
   int rc = ring_upload(w.data(), w.size() * sizeof(float), s, &dw, &ticket);
   if (rc) return rc;
   const u64 ncols = u64(1) << (nqubits - 5);
-  const u64 tasks = ncols / 32;  // wave steps
+  const int cols_per_wave = mfma_shape();
+  const u64 tasks = ncols / cols_per_wave;  // wave steps
   u64 grid = (tasks + (kMfmaBlock / 64) - 1) / (kMfmaBlock / 64);
   if (grid > kMfmaMaxGrid) grid = kMfmaMaxGrid;
-  k_block5_mfma<<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(static_cast<cplx<float>*>(state), ncols, ins,
-                                                                 off, static_cast<const float*>(dw));
+  if (cols_per_wave == 32)
+    k_block5_mfma<<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(static_cast<cplx<float>*>(state), ncols, ins,
+                                                                   off, static_cast<const float*>(dw));
+  else
+    k_block5_mfma16<<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(static_cast<cplx<float>*>(state), ncols,
+                                                                     ins, off, static_cast<const float*>(dw));
   rc = check_launch("apply_block_mfma launch");
   ring_release(ticket, s);
   return rc;
