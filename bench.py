@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale from a plain 4 GiB copy):
 # profiles/sweep_30q_c128_hbm_traffic.md.  Keyed by (kernel, nqubits).
-PROFILED_TRAFFIC = {("k_wave<double,1>", 30): 31.496e9, ("k_one_qubit<double,0,1>", 30): 28.633e9}
+PROFILED_TRAFFIC = {("k_wave<double,1,1>", 30): 31.496e9, ("k_one_qubit<double,0,1>", 30): 28.633e9}
 
 
 def parse():
@@ -114,7 +114,9 @@ def kernel_name(backend, gate, n, dtype):
     k = len(gate.target_qubits)
     name = gate.__class__.__name__
     if k <= 2 and n >= 6 and all(n - 1 - t < 6 for t in gate.target_qubits):
-        return f"k_wave<{r},{k}>"  # targets inside a 64-amplitude chunk: in-wave exchange
+        ilp = os.environ.get("QJ_WAVE_ILP", "1")
+        ilp = ilp if ilp in ("2", "4") else "1"
+        return f"k_wave<{r},{k},{ilp}>"  # targets inside a 64-amplitude chunk: in-wave exchange
     if k == 1:
         kind = {"apply_x": 1, "apply_y": 2, "apply_z": 3, "apply_z_pow": 4}.get(GATE_OPS.get(name), 0)
         return f"k_one_qubit<{r},{kind},{v}>"

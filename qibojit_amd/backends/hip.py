@@ -102,7 +102,7 @@ class HipBackend(Backend):
         self.nthreads = len(os.sched_getaffinity(0))
         # execute_circuit runs unitary segments through the fused tile executor
         self.fusion = True
-        # uncontrolled 5-target complex64 blocks run on FP32 matrix cores
+        # uncontrolled 4/5-target complex64 blocks run on FP32 matrix cores
         self.mfma = True
         self._custom_matrices_c128 = CustomMatrices("complex128")
 
@@ -412,14 +412,21 @@ class HipBackend(Backend):
         if qubits is None:
             qubits = sorted(nqubits - q - 1 for q in targets)
         mat = self._host_matrix(gate, state.dtype)
-        tptr, tkeep, nt = i32_array([nqubits - t - 1 for t in targets])
         qseq = self._seq(qubits)
-        if self.mfma and state.dtype == torch.complex64 and nt == 5 and len(qseq) == 5 and nqubits >= 10:
-            # uncontrolled 5-qubit block: FP32 matrix-core GEMM
+        if self.mfma and state.dtype == torch.complex64 and len(targets) in (4, 5) and len(qseq) == len(targets) \
+                and nqubits >= 10:
+            # uncontrolled 4/5-qubit block: FP32 matrix-core GEMM; a 4-qubit
+            # block gets a spectator qubit (U x I on the highest free bit)
+            if len(targets) == 4:
+                spectator = min(q for q in range(nqubits) if q not in targets)
+                targets = targets + [spectator]
+                mat = np.ascontiguousarray(np.kron(mat, np.eye(2, dtype=mat.dtype)))
+            tptr, tkeep, nt = i32_array([nqubits - t - 1 for t in targets])
             call("qj_apply_block_mfma", _ptr(state), _NATIVE_CODES[state.dtype], nqubits, tptr, nt,
                  mat.ctypes.data_as(ctypes.c_void_p), self._stream())
             del tkeep
             return state
+        tptr, tkeep, nt = i32_array([nqubits - t - 1 for t in targets])
         qptr, qkeep, nq = i32_array(qseq)
         call(
             "qj_apply_multi_qubit",

@@ -75,3 +75,18 @@ def test_fused_brickwork_matches_unfused(backend, port_backend, rng, dtype_):
         want = port_backend.apply_gate(g, want, n)
     tol = 1e-5 if dtype_ == "complex64" else 1e-12
     assert np.max(np.abs(got - want)) < tol
+
+
+@pytest.mark.parametrize("targets", [(0, 1, 2, 3), (5, 2, 9, 7), (8, 9, 10, 11)])
+def test_four_qubit_blocks_on_matrix_cores(backend, rng, targets):
+    """A 4-qubit block runs as U x I with a spectator qubit on the MFMA kernel."""
+    from qibojit_amd._qibo import gates
+
+    n = 12
+    backend.set_dtype("complex64")
+    u = haar_unitary(16, rng)
+    state = random_state(n, rng).astype(np.complex64)
+    gate = gates.Unitary(u, *targets)
+    got = backend.to_numpy(backend.apply_gate(gate, backend.cast(state.copy()), n))
+    want = einsum_ref.apply_gate(state.astype(np.complex128), u, list(targets), nqubits=n)
+    assert np.max(np.abs(got - want)) < 1e-5
