@@ -292,6 +292,51 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ---- three targets on bits {0, 1, 2}, uncontrolled: LDS-staged chunks --
+// Each thread of the register kernel would own 8 consecutive amplitudes, so
+// its loads stride 128 B across a wave.  Here a block moves 2048 consecutive
+// amplitudes with coalesced loads into LDS, each thread then takes one
+// 8-amplitude group (2048 / 8 = 256 groups), applies the 8x8 matrix in
+// registers (rows summed in ascending column order), writes the group back
+// to LDS, and the block stores the chunk coalesced.  LDS slot s lives at
+// s ^ ((s >> 3) & 7): both the coalesced and the per-group accesses then hit
+// distinct 16-byte bank groups within every 8 lanes.
+constexpr int kLow3Chunk = 2048;
+
+template <typename R>
+constexpr size_t mat_bytes_of(int n) {
+  return (size_t)n * n * sizeof(cplx<R>);
+}
+
+__device__ __forceinline__ unsigned low3_swz(unsigned s) { return s ^ ((s >> 3) & 7u); }
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_multi_low3(cplx<R>* __restrict__ st, u64 nchunks, Offs32 roff, const cplx<R>* __restrict__ mat) {
+  __shared__ cplx<R> buf[kLow3Chunk];
+  const unsigned t = threadIdx.x;
+  for (u64 c = blockIdx.x; c < nchunks; c += gridDim.x) {  // uniform per block
+    cplx<R>* p = st + c * kLow3Chunk;
+#pragma unroll
+    for (int i = 0; i < kLow3Chunk / kBlock; ++i) buf[low3_swz(i * kBlock + t)] = p[i * kBlock + t];
+    __syncthreads();
+    cplx<R> x[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) x[r] = buf[low3_swz(8 * t + (unsigned)roff.o[r])];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      cplx<R> acc = cmul(mat[r * 8], x[0]);
+#pragma unroll
+      for (int k = 1; k < 8; ++k) acc = cadd(acc, cmul(mat[r * 8 + k], x[k]));
+      buf[low3_swz(8 * t + (unsigned)roff.o[r])] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kLow3Chunk / kBlock; ++i) p[i * kBlock + t] = buf[low3_swz(i * kBlock + t)];
+    __syncthreads();
+  }
+}
+
 // Dense target block of a kernel family (the reference's encodings).
 template <typename R>
 static void dense_one(int kind, const cplx<R>* g, cplx<R>* d) {
@@ -549,6 +594,30 @@ static int multi_qubit_impl(void* state, int nqubits, const int32_t* target_bits
   const int K = ntargets, N = 1 << K;
   const u64 ngroups = u64(1) << (nqubits - ins.npos);
   cplx<R>* st = static_cast<cplx<R>*>(state);
+  if (K == 3 && ins.npos == 3 && nqubits >= 11) {
+    bool low = true;
+    for (int u = 0; u < 3; ++u) low &= tb[u] < 3;
+    if (low) {
+      Offs32 roff;
+      std::memset(&roff, 0, sizeof(roff));
+      for (int r = 0; r < 8; ++r) {
+        u64 o = 0;
+        for (int u = 0; u < 3; ++u)
+          if ((r >> (2 - u)) & 1) o |= u64(1) << tb[u];
+        roff.o[r] = o;
+      }
+      void* dmat = nullptr;
+      int ticket = -1;
+      rc = ring_upload(gate, mat_bytes_of<R>(8), s, &dmat, &ticket);
+      if (rc) return rc;
+      const u64 nchunks = (u64(1) << nqubits) / kLow3Chunk;
+      k_multi_low3<R><<<dim3((unsigned)grid_for(nchunks, 1)), dim3(kBlock), 0, s>>>(
+          st, nchunks, roff, static_cast<const cplx<R>*>(dmat));
+      rc = check_launch("apply_multi_qubit launch");
+      ring_release(ticket, s);
+      return rc;
+    }
+  }
 
   const size_t mat_bytes = (size_t)N * N * sizeof(cplx<R>);
   if (K <= 5) {
