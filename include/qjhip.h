@@ -261,6 +261,12 @@ int qj_reduce_rows(const double* in, uint64_t nrows, uint64_t ncols,
 
 /* ---- measurement ---- */
 
+/* Every one of the n amplitudes set to re + i im (stream-ordered): the plus
+ * state 2^(-n/2) and its density matrix 2^-n.  Replaces qibo's plus_state
+ * (NumpyBackend), which NumbaBackend inherits. */
+int qj_fill_state(void* state, int dtype, uint64_t n, double re, double im,
+                  void* stream);
+
 /* Collapse onto outcome `result` of the measured bits `qubits_kernel`
  * (= [n - q - 1 for q in reversed(measured qubits)], cpu.py:550-552); bit j of
  * `result` is the value of bit qubits_kernel[j] (ops.py:33-40; for ascending

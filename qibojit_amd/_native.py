@@ -59,6 +59,7 @@ SIGNATURES = {
                          _vp, _u64, _vp, _i32, _vp],
     "qj_tfim_batch": [_vp, _vp, _i32, _i32, _u64, _vp, _i32, _dbl, _i32, _vp, _i32, _vp],
     "qj_reduce_rows": [_vp, _u64, _u64, _vp, _vp],
+    "qj_fill_state": [_vp, _i32, _u64, _dbl, _dbl, _vp],
     "qj_compare_states": [_vp, _i32, _vp, _i32, _u64, _pdbl, _vp],
     "qj_apply_block_mfma": [_vp, _i32, _i32, _pi32, _i32, _vp, _vp],
     "qj_ansatz_lds": [_i32, _vp, _i32, _vp, _u64, _dbl, _vp, _i32, _i32, _vp],

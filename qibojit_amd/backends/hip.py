@@ -197,6 +197,18 @@ class HipBackend(Backend):
         call("qj_initial_state", _ptr(state), _NATIVE_CODES[tdt], numel, self._stream())
         return state.reshape(n, n) if density_matrix else state
 
+    def plus_state(self, nqubits, density_matrix=False, dtype=None):
+        """|+...+> (all amplitudes 2^(-n/2)), or its density matrix (all 2^-n)."""
+        if dtype is None:
+            dtype = self.dtype
+        tdt = _TORCH_DTYPES[_dtype_name(dtype)]
+        n = 1 << nqubits
+        numel = n * n if density_matrix else n
+        state = torch.empty(numel, dtype=tdt, device=self._torch_device)
+        value = 1.0 / n if density_matrix else 1.0 / math.sqrt(n)
+        call("qj_fill_state", _ptr(state), _NATIVE_CODES[tdt], numel, value, 0.0, self._stream())
+        return state.reshape(n, n) if density_matrix else state
+
     # ---------------------------------------------------- gate dispatch --
     def apply_gate(self, gate, state, nqubits, inverse=False):
         density_matrix = len(tuple(state.shape)) == 2

@@ -61,6 +61,14 @@ __global__ void __launch_bounds__(kBlock) k_initial_state(cplx<R>* __restrict__ 
     st[i] = mk<R>(i == 0 ? R(1) : R(0), R(0));
 }
 
+// Every amplitude set to `value` (the plus state |+...+> and its density
+// matrix are uniform).
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_fill(cplx<R>* __restrict__ st, u64 n, cplx<R> value) {
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) st[i] = value;
+}
+
 // --------------------------------------------------------------- collapse --
 // Every amplitude i splits into (free bits, measured bits); it is kept iff
 // (i & mask) == keep.  Pass 1 zeroes the rest (write-only) and accumulates
@@ -779,6 +787,20 @@ int qj_vdot(const void* a, const void* b, int dtype, uint64_t n, double* out, vo
   rc = check_launch("vdot launch");
   if (rc) return rc;
   return reduce_to_host(2, grid, grid, 1, out, s, sc);
+}
+
+int qj_fill_state(void* state, int dtype, uint64_t n, double re, double im, void* stream) {
+  if (!state) return set_error(QJ_EVALUE, "fill_state: null state");
+  if (n == 0) return QJ_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const u64 grid = grid_for(n, kBlock);
+  if (dtype == QJ_COMPLEX64)
+    k_fill<float><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((cplx<float>*)state, n, mk<float>((float)re, (float)im));
+  else if (dtype == QJ_COMPLEX128)
+    k_fill<double><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((cplx<double>*)state, n, mk<double>(re, im));
+  else
+    return set_error(QJ_EVALUE, "fill_state: dtype must be complex");
+  return check_launch("fill launch");
 }
 
 int qj_compare_states(const void* a, int dtype_a, const void* b, int dtype_b, uint64_t n, double* out,
