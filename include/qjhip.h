@@ -211,7 +211,7 @@ int qj_apply_tile_pass(void* state, int dtype, int nqubits, int low_bits,
  * U^+ to both.  For exp(-i theta G / 2) and lam = U_{>k}^+ H psi_L this is
  * d<psi_L|H|psi_L>/d theta. */
 #define QJ_ROT_MAX_BITS 5
-#define QJ_ROT_MAX_ADJ_BITS 4 /* with lam: psi and lam groups in registers */
+#define QJ_ROT_MAX_ADJ_BITS 5 /* with lam (psi and lam in registers): 5 for complex64, 4 for complex128 */
 #define QJ_ROT_MAX_STEPS 3
 #define QJ_LOAD_PLAIN 0
 #define QJ_LOAD_LADDER 1

@@ -35,7 +35,7 @@ from qibojit_amd._qibo import Circuit, gates
 LOAD_PLAIN, LOAD_LADDER, LOAD_UNLADDER, LOAD_ZERO = 0, 1, 2, 3
 GEN_X, GEN_Y, GEN_Z = 1, 2, 3
 ROT_MAX_BITS = 5
-ROT_MAX_ADJ_BITS = 4
+ROT_MAX_ADJ_BITS = {torch.complex64: 5, torch.complex128: 4}
 ROT_BLOCK = 256
 MAX_COLS = 65535
 
@@ -186,7 +186,7 @@ class AnsatzEngine:
         self.g = float(g)
         n = ansatz.nqubits
         self.k_forward = int(min(k_forward, ROT_MAX_BITS, n))
-        self.k_adjoint = int(min(k_adjoint, ROT_MAX_ADJ_BITS, n))
+        self.k_adjoint = int(min(k_adjoint, ROT_MAX_ADJ_BITS[self.tdtype], n))
         if tiles is None:
             tiles = self.tdtype == torch.complex64 and n >= TILE_BITS
         if (tiles or tiles_adjoint) and (self.tdtype != torch.complex64 or n < TILE_BITS):

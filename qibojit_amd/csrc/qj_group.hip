@@ -596,8 +596,12 @@ static void launch_rot_k(int k, bool vec, dim3 gd, hipStream_t st, const cplx<R>
     case 3: launch_rot<R, 3, ADJ>(vec, gd, st, pin, pout, lin, lout, sel, mats, partials, ngroups, a); break;
     case 4: launch_rot<R, 4, ADJ>(vec, gd, st, pin, pout, lin, lout, sel, mats, partials, ngroups, a); break;
     default:
-      // five bits of psi and lam exceed the register file in adjoint mode
-      if constexpr (!ADJ) launch_rot<R, 5, ADJ>(vec, gd, st, pin, pout, lin, lout, sel, mats, partials, ngroups, a);
+      // five bits of psi and lam: complex64 only, and only the 8-byte form
+      // (the 16-byte form spills)
+      if constexpr (!ADJ)
+        launch_rot<R, 5, ADJ>(vec, gd, st, pin, pout, lin, lout, sel, mats, partials, ngroups, a);
+      else if constexpr (sizeof(R) == 4)
+        k_rot<R, 5, true, 1><<<gd, dim3(kRotBlock), 0, st>>>(pin, pout, lin, lout, sel, mats, partials, ngroups, a);
       break;
   }
 }
@@ -626,7 +630,8 @@ static int rot_pass_impl(const void* psi_in, void* psi_out, const void* lam_in, 
   if (ncols < 1 || ncols > kRotMaxCols) return set_error(QJ_EVALUE, "%s: bad column count %d", what, ncols);
   const bool adj = lam_in != nullptr;
   if (adj != (lam_out != nullptr)) return set_error(QJ_EVALUE, "%s: lam_in and lam_out go together", what);
-  if (adj && k > QJ_ROT_MAX_ADJ_BITS) return set_error(QJ_EVALUE, "%s: adjoint groups hold at most %d bits", what, QJ_ROT_MAX_ADJ_BITS);
+  const int max_adj = sizeof(R) == 4 ? QJ_ROT_MAX_ADJ_BITS : QJ_ROT_MAX_ADJ_BITS - 1;
+  if (adj && k > max_adj) return set_error(QJ_EVALUE, "%s: adjoint groups hold at most %d bits", what, max_adj);
   if (load == QJ_LOAD_ZERO && adj) return set_error(QJ_EVALUE, "%s: zero load is forward only", what);
   if ((load == QJ_LOAD_LADDER || load == QJ_LOAD_UNLADDER) && (psi_in == psi_out || (adj && lam_in == lam_out)))
     return set_error(QJ_EVALUE, "%s: a permuted load needs distinct input and output", what);

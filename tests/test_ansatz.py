@@ -177,7 +177,8 @@ def _rot(**kw):
         (dict(mstride=3), "matrix stride"),
         (dict(load=1, psi_out=_fake()), "distinct"),
         (dict(lam_in=_fake()), "together"),
-        (dict(lam_in=_fake(), lam_out=_fake(), bits=[0, 1, 2, 3, 4], mstride=20), "at most 4"),
+        (dict(lam_in=_fake(), lam_out=_fake(), bits=[0, 1, 2, 3, 4], mstride=20, dtype=_native.QJ_COMPLEX128),
+         "at most 4"),
         (dict(lam_in=_fake(), lam_out=_fake(), nrot=2, mstride=24, gens=[2, 3, 2, 3, 2, 3]), "partials"),
         (dict(nrot=2, mstride=24, gens=[2, 3, 2, 9, 2, 3]), "generator"),
         (dict(dtype=_native.QJ_FLOAT32), "dtype"),

@@ -44,7 +44,7 @@ def test_engine_states_and_energies(torch_mod, dtype, rng, n, layers, batch, kf)
         assert abs(energies[b] - einsum_ref.tfim_energy(want, 0.8)) < ENERGY[dtype]
 
 
-@pytest.mark.parametrize("n,layers,batch,ka", [(6, 2, 2, 4), (7, 2, 3, 3), (5, 3, 1, 2), (8, 1, 2, 1)])
+@pytest.mark.parametrize("n,layers,batch,ka", [(6, 2, 2, 4), (7, 2, 3, 3), (5, 3, 1, 2), (8, 1, 2, 1), (11, 2, 2, 5)])
 def test_engine_gradients_match_parameter_shift(torch_mod, dtype, rng, n, layers, batch, ka):
     from qibojit_amd.ansatz import AnsatzEngine, HardwareEfficientAnsatz
 
