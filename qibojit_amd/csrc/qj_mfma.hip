@@ -121,10 +121,104 @@ __global__ void __launch_bounds__(kMfmaBlock)
   }
 }
 
+// 3M form (default): three real 32x32 products instead of the 64x64 one,
+//   P1 = Ur Xr,  P2 = Ui Xi,  P3 = (Ur + Ui)(Xr + Xi),
+//   Re Y = P1 - P2,  Im Y = P3 - P1 - P2,
+// with v_mfma_f32_16x16x4f32: 3 products x 2 row tiles x 8 K-steps = 48
+// MFMAs per 16 columns (64 for the real 64x64 form).  A operands per lane:
+// ur[t][s] = Ur[16t + n][4s + q] (and ui, us = ur + ui); lane (n, q) loads
+// rows 4j + q of column n (its B operands for K-step j) and holds outputs
+// rows 16t + 4q + r of column n.
+// STAGE (every target below bit 9): a wave's 16 columns are one contiguous
+// run of 512 amplitudes, moved through LDS with coalesced loads and stores
+// (the direct loads of low targets are strided); slot a sits at a + a / 32
+// so the column-wise reads spread over the banks.  The loop is uniform per
+// block (ncols is a multiple of 64 there) so the block barriers are safe.
+constexpr int kStageAmps = 512;
+constexpr int kStageSlots = kStageAmps + kStageAmps / 32;
+
+__device__ __forceinline__ unsigned stage_slot(unsigned a) { return a + (a >> 5); }
+
+template <bool STAGE>
+__global__ void __launch_bounds__(kMfmaBlock)
+    k_block5_mfma3m(cplx<float>* __restrict__ st, u64 ncols, Ins ins, Offs32 off, const float* __restrict__ u3) {
+  __shared__ cplx<float> stage[STAGE ? kMfmaBlock / 64 : 1][STAGE ? kStageSlots : 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane >> 4, n = lane & 15;
+  float ur[2][8], ui[2][8], us[2][8];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int e = (16 * t + n) * 32 + 4 * s + q;
+      ur[t][s] = u3[e];
+      ui[t][s] = u3[1024 + e];
+      us[t][s] = u3[2048 + e];
+    }
+  // chunk-local index of this lane's column (targets inserted as zeros)
+  const unsigned colbase = STAGE ? (unsigned)insert_bits((u64)n, ins) : 0u;
+  const u64 step = (u64)gridDim.x * (kMfmaBlock / 64) * 16;
+  for (u64 c0 = ((u64)blockIdx.x * (kMfmaBlock / 64) + wave) * 16; c0 < ncols; c0 += step) {
+    const u64 base = insert_bits(c0 + (STAGE ? 0 : n), ins);
+    float xr[8], xi[8], xs[8];
+    if (STAGE) {
+#pragma unroll
+      for (int k = 0; k < kStageAmps / 64; ++k) stage[wave][stage_slot(64 * k + lane)] = st[base + 64 * k + lane];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const cplx<float> v = stage[wave][stage_slot(colbase + (unsigned)off.o[4 * j + q])];
+        xr[j] = v.re;
+        xi[j] = v.im;
+        xs[j] = v.re + v.im;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const cplx<float> v = st[base + off.o[4 * j + q]];
+        xr[j] = v.re;
+        xi[j] = v.im;
+        xs[j] = v.re + v.im;
+      }
+    }
+    v4f p1[2], p2[2], p3[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) p1[t] = p2[t] = p3[t] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        p1[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[t][s], xr[s], p1[t], 0, 0, 0);
+        p2[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[t][s], xi[s], p2[t], 0, 0, 0);
+        p3[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(us[t][s], xs[s], p3[t], 0, 0, 0);
+      }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const cplx<float> y = mk<float>(p1[t][r] - p2[t][r], p3[t][r] - p1[t][r] - p2[t][r]);
+        const unsigned row = (unsigned)off.o[16 * t + 4 * q + r];
+        if (STAGE)
+          stage[wave][stage_slot(colbase + row)] = y;
+        else
+          st[base + row] = y;
+      }
+    if (STAGE) {
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kStageAmps / 64; ++k) st[base + 64 * k + lane] = stage[wave][stage_slot(64 * k + lane)];
+      __syncthreads();
+    }
+  }
+}
+
+// QJ_MFMA_SHAPE: 3 (3M form, default), 16 (real 64x64 form, 16x16x4) or 32
+// (real 64x64 form, 32x32x2).
 static int mfma_shape() {
   static const int shape = [] {
     const char* e = std::getenv("QJ_MFMA_SHAPE");
-    return (e && std::atoi(e) == 32) ? 32 : 16;
+    const int v = e ? std::atoi(e) : 3;
+    return (v == 32 || v == 16) ? v : 3;  // 3: the 3M form
   }();
   return shape;
 }
@@ -152,27 +246,47 @@ static int block5_impl(void* state, int nqubits, const int32_t* target_bits, con
       if ((r >> (4 - u)) & 1) o |= u64(1) << target_bits[u];
     off.o[r] = o;
   }
-  // real 64x64 form of the complex matrix, row-major
   const cplx<float>* u = static_cast<const cplx<float>*>(gate);
-  std::vector<float> w(64 * 64);
-  for (int r = 0; r < 32; ++r)
-    for (int c = 0; c < 32; ++c) {
-      const cplx<float> z = u[r * 32 + c];
-      w[r * 64 + c] = z.re;
-      w[r * 64 + 32 + c] = -z.im;
-      w[(32 + r) * 64 + c] = z.im;
-      w[(32 + r) * 64 + 32 + c] = z.re;
+  const int shape = mfma_shape();
+  std::vector<float> w;
+  if (shape == 3) {
+    // Ur, Ui, Ur + Ui as three 32x32 row-major blocks
+    w.resize(3 * 1024);
+    for (int e = 0; e < 1024; ++e) {
+      w[e] = u[e].re;
+      w[1024 + e] = u[e].im;
+      w[2048 + e] = u[e].re + u[e].im;
     }
+  } else {
+    // real 64x64 form [[Ur, -Ui], [Ui, Ur]], row-major
+    w.resize(64 * 64);
+    for (int r = 0; r < 32; ++r)
+      for (int c = 0; c < 32; ++c) {
+        const cplx<float> z = u[r * 32 + c];
+        w[r * 64 + c] = z.re;
+        w[r * 64 + 32 + c] = -z.im;
+        w[(32 + r) * 64 + c] = z.im;
+        w[(32 + r) * 64 + 32 + c] = z.re;
+      }
+  }
   void* dw = nullptr;
   int ticket = -1;
   int rc = ring_upload(w.data(), w.size() * sizeof(float), s, &dw, &ticket);
   if (rc) return rc;
   const u64 ncols = u64(1) << (nqubits - 5);
-  const int cols_per_wave = mfma_shape();
+  const int cols_per_wave = shape == 32 ? 32 : 16;
   const u64 tasks = ncols / cols_per_wave;  // wave steps
   u64 grid = (tasks + (kMfmaBlock / 64) - 1) / (kMfmaBlock / 64);
   if (grid > kMfmaMaxGrid) grid = kMfmaMaxGrid;
-  if (cols_per_wave == 32)
+  // targets below bit 9: a wave's columns form one contiguous run (staged)
+  const bool stage = shape == 3 && pos[4] < 9 && nqubits >= 11;
+  if (stage)
+    k_block5_mfma3m<true><<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(
+        static_cast<cplx<float>*>(state), ncols, ins, off, static_cast<const float*>(dw));
+  else if (shape == 3)
+    k_block5_mfma3m<false><<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(
+        static_cast<cplx<float>*>(state), ncols, ins, off, static_cast<const float*>(dw));
+  else if (cols_per_wave == 32)
     k_block5_mfma<<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(static_cast<cplx<float>*>(state), ncols, ins,
                                                                    off, static_cast<const float*>(dw));
   else

@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("n", [10, 12, 15])
-@pytest.mark.parametrize("targets", [(0, 1, 2, 3, 4), (4, 2, 0, 3, 1), (3, 7, 9, 1, 5), (9, 8, 7, 6, 5)])
+@pytest.mark.parametrize("targets", [(0, 1, 2, 3, 4), (4, 2, 0, 3, 1), (3, 7, 9, 1, 5), (9, 8, 7, 6, 5),
+                                     (8, 1, 5, 0, 3), (6, 4, 2, 7, 5)])
 def test_mfma_block_matches_oracle(backend, rng, n, targets):
     from qibojit_amd._qibo import gates
 
