@@ -114,8 +114,9 @@ def kernel_name(backend, gate, n, dtype):
     k = len(gate.target_qubits)
     name = gate.__class__.__name__
     if k <= 2 and n >= 6 and all(n - 1 - t < 6 for t in gate.target_qubits):
-        ilp = os.environ.get("QJ_WAVE_ILP", "1")
-        ilp = ilp if ilp in ("2", "4") else "1"
+        ilp = os.environ.get("QJ_WAVE_ILP", "")
+        if ilp not in ("1", "2", "4"):
+            ilp = "2" if dtype == "complex64" else "1"
         return f"k_wave<{r},{k},{ilp}>"  # targets inside a 64-amplitude chunk: in-wave exchange
     if k == 1:
         kind = {"apply_x": 1, "apply_y": 2, "apply_z": 3, "apply_z_pow": 4}.get(GATE_OPS.get(name), 0)

@@ -400,11 +400,15 @@ static int try_wave(cplx<R>* st, int nqubits, const int* tb, int k, const Ins& g
   int ticket = -1;
   *rc = ring_upload(dense, (size_t)N * N * sizeof(cplx<R>), s, &dmat, &ticket);
   if (*rc) return 1;
-  static const int ilp = [] {
+  // chunks per wave step: QJ_WAVE_ILP overrides; by default 1 for complex128
+  // (16-byte lanes; 5.9 vs 5.6 TB/s with 2 at 30 qubits) and 2 for complex64
+  // (8-byte lanes need more loads in flight)
+  static const int ilp_env = [] {
     const char* e = std::getenv("QJ_WAVE_ILP");
-    const int v = e ? std::atoi(e) : 1;  // 1 measured fastest (5.9 vs 5.6 TB/s at 30 qubits)
-    return (v == 2 || v == 4) ? v : 1;
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 1 || v == 2 || v == 4) ? v : 0;
   }();
+  const int ilp = ilp_env ? ilp_env : (sizeof(R) == 4 ? 2 : 1);
   const u64 waves = (nchunks + ilp - 1) / ilp;
   const u64 grid = grid_for(waves, kBlock / 64);
   const cplx<R>* dm = static_cast<const cplx<R>*>(dmat);

@@ -398,6 +398,49 @@ class Channel(Gate):
         return backend.apply_channel(self, state, nqubits)
 
 
+class KrausChannel(Channel):
+    """rho -> sum_i K_i rho K_i^dagger for Kraus operators ``(qubits_i, K_i)``
+    (qibo.gates.channels.KrausChannel): coefficients 1, not unitary, so the
+    backend applies each K_i to a copy of rho."""
+
+    def __init__(self, qubits, operators):
+        super().__init__()
+        self.name = "KrausChannel"
+        if isinstance(qubits, int) or (len(qubits) and isinstance(qubits[0], (int, np.integer))):
+            qubits = [tuple(qubits) if not isinstance(qubits, int) else (qubits,)] * len(operators)
+        self.init_args = [qubits, operators]
+        self.gates = tuple(Unitary(np.asarray(m), *q) for q, m in zip(qubits, operators))
+        self.coefficients = len(self.gates) * (1.0,)
+        self.coefficient_sum = 1.0
+        self._all_unitary_operators = False
+        all_q = sorted({q for qs in qubits for q in qs})
+        self.target_qubits = tuple(all_q)
+
+
+class ReadoutErrorChannel(KrausChannel):
+    """Readout error with row-stochastic matrix P on ``qubits``
+    (qibo.gates.channels.ReadoutErrorChannel): Kraus operators
+    K_jk = sqrt(P[j, k]) |k><j|."""
+
+    def __init__(self, qubits, probabilities):
+        qubits = (qubits,) if isinstance(qubits, int) else tuple(qubits)
+        p = np.asarray(probabilities, dtype=float)
+        d = 2 ** len(qubits)
+        if p.shape != (d, d):
+            raise ValueError(f"Readout matrix of shape {p.shape} does not act on {len(qubits)} qubits.")
+        if np.any(p < 0) or not np.allclose(p.sum(axis=1), 1):
+            raise ValueError("The readout matrix must be row-stochastic.")
+        ops = []
+        for j in range(d):
+            for k in range(d):
+                op = np.zeros((d, d), dtype=complex)
+                op[k, j] = np.sqrt(p[j, k])
+                ops.append(op)
+        super().__init__(qubits, ops)
+        self.name = "ReadoutErrorChannel"
+        self.init_args = [qubits, probabilities]
+
+
 class UnitaryChannel(Channel):
     """sum_i p_i U_i rho U_i^dagger + (1 - sum p_i) rho."""
 

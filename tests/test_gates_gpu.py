@@ -290,3 +290,35 @@ def test_large_state_round_trip(backend, dtype):
         state = backend.apply_gate(inv, state, n)
     overlap = backend.vdot(backend.cast(psi0, dtype=dtype), state)
     assert abs(overlap - 1) < (1e-10 if dtype == "complex128" else 1e-4)
+
+
+def test_readout_error_channel(backend, rng):
+    """tests/test_gates.py:443-455: a readout-error channel keeps the trace."""
+    from qibojit_amd.qibo_compat.config import PRECISION_TOL
+    from qibojit_amd.qibo_compat.gates import ReadoutErrorChannel
+
+    rho = gm.random_density_matrix(1, rng)
+    p = rng.random((2, 2))
+    p /= p.sum(axis=1, keepdims=True)
+    channel = ReadoutErrorChannel(0, p)
+    out = backend.to_numpy(channel.apply(backend, backend.cast(rho.copy()), 1))
+    assert abs(np.trace(out).real - 1) < PRECISION_TOL
+    want = sum(k @ rho @ k.conj().T for k in (g.parameters[0] for g in channel.gates))
+    np.testing.assert_allclose(out, want, atol=1e-12)
+
+
+def test_kraus_channel(backend, dtype, rng):
+    """rho -> sum K rho K^+ against the dense closed form."""
+    from qibojit_amd.qibo_compat.gates import KrausChannel
+
+    backend.set_dtype(dtype)
+    n = 3
+    rho = gm.random_density_matrix(n, rng)
+    u1, u2 = gm.haar_unitary(4, rng), gm.haar_unitary(4, rng)
+    ops = [np.sqrt(0.3) * u1, np.sqrt(0.7) * u2]
+    channel = KrausChannel((0, 2), ops)
+    out = backend.to_numpy(backend.apply_channel(channel, backend.cast(rho.copy()), n))
+    full = [er.full_matrix(k, [0, 2], [], n) for k in ops]
+    want = sum(f @ rho @ f.conj().T for f in full)
+    np.testing.assert_allclose(out, want, atol=ATOL[dtype])
+    assert abs(np.trace(out).real - 1) < (1e-5 if dtype == "complex64" else 1e-12)
