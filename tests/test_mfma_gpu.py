@@ -78,7 +78,7 @@ def test_fused_brickwork_matches_unfused(backend, port_backend, rng, dtype_):
     assert np.max(np.abs(got - want)) < tol
 
 
-@pytest.mark.parametrize("targets", [(0, 1, 2, 3), (5, 2, 9, 7), (8, 9, 10, 11)])
+@pytest.mark.parametrize("targets", [(0, 1, 2, 3), (5, 2, 9, 7), (8, 9, 10, 11), (1, 2, 3, 4), (11, 6, 10, 7)])
 def test_four_qubit_blocks_on_matrix_cores(backend, rng, targets):
     """A 4-qubit block runs as U x I with a spectator qubit on the MFMA kernel."""
     from qibojit_amd._qibo import gates
