@@ -138,13 +138,13 @@ int qj_apply_block_mfma(void* state, int dtype, int nqubits,
                         const int32_t* target_bits, int ntargets,
                         const void* gate, void* stream);
 
-/* ---- fused passes (LDS tiles) ----
- * One pass streams the state through LDS in tiles of 2^(low_bits + nhigh)
+/* ---- fused passes (tiles) ----
+ * One pass streams the state through tiles of 2^(low_bits + nhigh)
  * amplitudes: a tile holds every combination of the `low_bits` lowest bits
  * (contiguous in memory) and of the `nhigh` bit positions `high_bits`
  * (ascending, all >= low_bits), for one assignment of the other bits.  The
- * `nops` operations are applied to each tile in order, in LDS, and the tile
- * is written back: the whole list costs one read and one write of the state.
+ * `nops` operations are applied to each tile in order and the tile is
+ * written back: the whole list costs one read and one write of the state.
  * This is the gate-fusion executor; it has no single counterpart in the
  * reference, which applies a fused block (qibo FusedGate via
  * matrix_fused, cpu.py:535-537) as one dense gate per pass.
@@ -154,17 +154,23 @@ int qj_apply_block_mfma(void* state, int dtype, int nqubits,
  *   QJ_TILE_DENSE: 2^k x 2^k row-major matrix (k <= 5) at `data + offset` on
  *     the tile-local target bits target[0..k-1] (target[0] = matrix MSB);
  *   QJ_TILE_PHASE: multiplies amplitude i by exp(i*angle(i)) with
- *     angle(i) = c0 + sum_j theta_j * bit_j(i), as a product of four complex
- *     tables at `data + offset`: {int32 ga, gb, la, pad; OA[2^ga], OB[2^gb],
- *     A[2^la], B[2^(tb-la)]}.  With t the tile index (the out-of-tile bits of
- *     i, ascending, packed; ga + gb = nqubits - tb) and e the tile-local
- *     index (tb = low_bits + nhigh bits, la = min(8, tb)), the factor is
- *     OA[t mod 2^ga] * OB[t >> ga] * A[e mod 2^la] * B[e >> la]; each table
- *     holds exp(i * sum of the thetas of its set bits), c0 folded into OA.
- * Both kinds apply only where every bit of `local_ctrl` (tile-local mask)
- * and of `global_ctrl` (mask of out-of-tile bit positions) is 1. */
+ *     angle(i) = c0 + sum_j theta_j * bit_j(i).  The block at
+ *     `data + offset` is {int32 ga, gb, tb, pad; OA[2^ga], OB[2^gb], W[tb]}
+ *     (complex, the state's precision).  With t the tile index (the
+ *     out-of-tile bits of i, ascending, packed; ga = (nqubits - tb + 1) / 2,
+ *     gb = nqubits - tb - ga) and e the tile-local index (tb = low_bits +
+ *     nhigh), the factor is OA[t mod 2^ga] * OB[t >> ga] * prod of W[b] over
+ *     the bits b set in e: OA and OB hold exp(i * sum of the thetas of the
+ *     out-of-tile bits set in their index), c0 folded into OA, and
+ *     W[b] = exp(i * theta of tile bit b);
+ *   QJ_TILE_SWAP: exchanges tile-local bits target[0] and target[1] (k = 2,
+ *     no controls, no data), the reference's apply_swap_kernel
+ *     (custom_operators/gates.py:196-216) inside the tile.
+ * Dense and phase ops apply only where every bit of `local_ctrl` (tile-local
+ * mask) and of `global_ctrl` (mask of out-of-tile bit positions) is 1. */
 #define QJ_TILE_DENSE 0
 #define QJ_TILE_PHASE 1
+#define QJ_TILE_SWAP 2
 #define QJ_TILE_MAX_K 5
 #define QJ_TILE_MAX_BITS 13
 
@@ -182,6 +188,13 @@ int qj_apply_tile_pass(void* state, int dtype, int nqubits, int low_bits,
                        const int32_t* high_bits, int nhigh,
                        const qj_tile_op* ops, int nops, const void* data,
                        uint64_t data_bytes, void* stream);
+
+/* Swaps of disjoint bit pairs (pairs[2j], pairs[2j+1]), j < npairs, all in one
+ * read and one write of the state: the permutation of a run of SWAP gates
+ * on distinct qubits, such as the final swaps of a QFT.  Each pair is the
+ * reference's apply_swap_kernel (custom_operators/gates.py:196-216);
+ * the reference runs one pass per pair. */
+int qj_apply_bit_swaps(void* state, int dtype, int nqubits, const int32_t* pairs, int npairs, void* stream);
 
 /* ---- layered variational circuits (batched forward, TFIM, adjoint) ----
  * No counterpart in the reference (qibojit has no differentiation or

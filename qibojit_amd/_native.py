@@ -45,6 +45,7 @@ SIGNATURES = {
     "qj_apply_two_qubit": [_vp, _i32, _i32, _i32, _i32, _i32, _pi32, _i32, _vp, _vp],
     "qj_apply_multi_qubit": [_vp, _i32, _i32, _pi32, _i32, _pi32, _i32, _vp, _vp],
     "qj_apply_tile_pass": [_vp, _i32, _i32, _i32, _pi32, _i32, _vp, _i32, _vp, _u64, _vp],
+    "qj_apply_bit_swaps": [_vp, _i32, _i32, _pi32, _i32, _vp],
     "qj_collapse_state": [_vp, _i32, _i32, _pi32, _i32, _i64, _i32, _pdbl, _vp],
     "qj_calculate_probabilities": [_vp, _i32, _i32, _pi32, _i32, _vp, _i32, _vp],
     "qj_calculate_probabilities_dm": [_vp, _i32, _i32, _pi32, _i32, _vp, _i32, _vp],

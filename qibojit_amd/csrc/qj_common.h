@@ -32,11 +32,11 @@ __host__ __device__ __forceinline__ cplx<R> mk(R a, R b) {
   return c;
 }
 template <typename R>
-__device__ __forceinline__ cplx<R> cadd(cplx<R> a, cplx<R> b) {
+__host__ __device__ __forceinline__ cplx<R> cadd(cplx<R> a, cplx<R> b) {
   return mk(a.re + b.re, a.im + b.im);
 }
 template <typename R>
-__device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
+__host__ __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
   return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
 // a*x + b*y, summed left to right as in the reference kernels

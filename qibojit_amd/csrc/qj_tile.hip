@@ -1,29 +1,44 @@
-// Fused tile passes: a list of gate operations applied to LDS-resident tiles
-// of the state, one HBM read and one HBM write per pass.
+// Fused tile passes: a list of gate operations applied to tiles of the
+// state, one HBM read and one HBM write per pass.
 //
 // A tile is 2^TB amplitudes: all combinations of the L lowest bits (one
 // contiguous run of 2^L amplitudes in memory) and of H = TB - L chosen high
 // bit positions, for one assignment of the remaining bits (the tile base,
 // whose compact index is the tile number t).  Workgroups take tiles in a
-// grid-stride loop: load (a wave reads 64 consecutive 16-byte vectors of one
-// run), apply every op with a barrier between ops, store.
+// grid-stride loop.
 //
-// Thread layout: thread `tid` of a BS-thread workgroup owns tile elements
-// e = tid + j*BS.  The low 8 bits of e are then fixed per thread and the
-// remaining bits are the same for all 64 lanes of a wave.
+// Two kernels run a pass:
+//  * k_tile_reg (qj_tile_reg.h): the tile lives in registers, 2^NRB
+//    amplitudes per thread; dense ops are register mat-vecs, phase ops are
+//    per-amplitude products, swaps relabel bits, and LDS only re-lays the
+//    tile out between segments of ops.  Used whenever the widest dense op
+//    fits the register bits (k <= 5 for complex64, k <= 4 for complex128).
+//  * k_tile_pass: the tile lives in LDS and every op is one sweep over it
+//    with a barrier after it.  It runs the passes the register kernel cannot
+//    (or every pass, with QJ_TILE_KERNEL=lds).
+//
+// LDS kernel thread layout: thread `tid` of a BS-thread workgroup owns tile
+// elements e = tid + j*BS.  The low 8 bits of e are then fixed per thread and
+// the remaining bits are the same for all 64 lanes of a wave.
 //  * Dense ops run as register mat-vecs on GB independent groups at a time
 //    (all loads, then all arithmetic, then all stores), so LDS latency of
 //    one group hides behind the others.
 //  * Phase ops multiply element e by F(t) * A(e & 255) * B(e >> 8): F from
 //    two tables over the bits of t (per tile, wave-uniform), A per thread,
 //    B wave-uniform -- one complex product per element, no trigonometry.
+//    The host expands the per-bit phase factors of the op into these tables.
 // Kernel instances are specialised on the widest dense op of the pass
 // (register budget) and on the block size.
+//
+// k_bit_swaps applies disjoint bit swaps (a qubit permutation that is its
+// own inverse, such as the final swaps of a QFT) in one read and one write:
+// a workgroup exchanges a tile with its partner tile.
 #include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "qj_common.h"
+#include "qj_tile_reg.h"
 
 namespace qj {
 
@@ -47,8 +62,9 @@ __device__ __forceinline__ u64 high_offset(int h, const TileSpec& ts) {
   return o;
 }
 
-// Phase data at op.offset: int32 ga, int32 gb, int32 la, int32 pad; then
-// complex tables OA[2^ga], OB[2^gb] over the tile index t (t & (2^ga-1), t >> ga),
+// LDS-kernel phase data at op.offset (built by the host from the per-bit
+// factors of the op): int32 ga, int32 gb, int32 la, int32 pad; then complex
+// tables OA[2^ga], OB[2^gb] over the tile index t (t & (2^ga-1), t >> ga),
 // A[2^la] over tile bits < la, B[2^(TB-la)] over tile bits >= la.
 struct PhaseHeader {
   int32_t ga, gb, la, pad;
@@ -192,6 +208,25 @@ __device__ __forceinline__ void tile_phase(cplx<R>* __restrict__ tile, const qj_
   }
 }
 
+// Swap of tile bits a and b: exchanges the elements with (bit a, bit b) =
+// (1, 0) and (0, 1).
+template <typename R, int BS>
+__device__ __forceinline__ void tile_swap(cplx<R>* __restrict__ tile, int a, int b, int TB) {
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  const int ngr = 1 << (TB - 2);
+  for (int g = threadIdx.x; g < ngr; g += BS) {
+    int e = g;
+    int l = e & ((1 << lo) - 1);
+    e = ((e ^ l) << 1) | l;
+    l = e & ((1 << hi) - 1);
+    e = ((e ^ l) << 1) | l;
+    const int e1 = e | (1 << a), e2 = e | (1 << b);
+    const cplx<R> v = tile[e1];
+    tile[e1] = tile[e2];
+    tile[e2] = v;
+  }
+}
+
 template <typename R, int V, int MAXK, int BS>
 __global__ void __launch_bounds__(BS)
     k_tile_pass(cplx<R>* __restrict__ st, u64 ntiles, Ins tins, TileSpec ts,
@@ -207,6 +242,8 @@ __global__ void __launch_bounds__(BS)
       if ((base & op.global_ctrl) != op.global_ctrl) continue;  // uniform
       if (op.kind == QJ_TILE_PHASE) {
         tile_phase<R, BS>(tile, op, data, t, ts.TB);
+      } else if (op.kind == QJ_TILE_SWAP) {
+        tile_swap<R, BS>(tile, op.target[0], op.target[1], ts.TB);
       } else {
         const cplx<R>* m = reinterpret_cast<const cplx<R>*>(data + op.offset);
         // A phase op right after a dense op without local controls (which
@@ -264,6 +301,296 @@ static void launch_tile(int bs, u64 grid, size_t lds, hipStream_t s, cplx<R>* st
   }
 }
 
+// ------------------------------------------------------ register kernel --
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ u64 uni64(u64 v) {
+  return ((u64)(uint32_t)uni((int)(v >> 32)) << 32) | (uint32_t)uni((int)v);
+}
+
+// A persistent workgroup copies the LDS part of the plan once, then loops
+// over tiles: the out-of-tile factors of the phase ops for the tile (one
+// thread each, double-buffered by tile parity), the loads, and the ops of
+// every segment; plan values read from LDS are made wave-uniform so the op
+// dispatch is scalar.  WPE: waves per SIMD the register budget must allow.
+template <typename R, int NRB, int BSMAX, int WPE>
+__global__ void __launch_bounds__(BSMAX, WPE)
+    k_tile_reg(cplx<R>* __restrict__ st, u64 ntiles, Ins tins, const char* __restrict__ plan) {
+  constexpr int NR = 1 << NRB;
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  const RegHead& h = *reinterpret_cast<const RegHead*>(plan);
+  const int TB = h.TB, NT = h.NT, nseg = h.nseg, nphase = h.nphase, ga = h.ga;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw);
+  char* lp = smem_raw + (sizeof(cplx<R>) << TB);
+  cplx<R>* fv = reinterpret_cast<cplx<R>*>(lp + h.lds_bytes);
+  const int tid = threadIdx.x;
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(plan + h.lds_off);
+    uint4* dst = reinterpret_cast<uint4*>(lp);
+    const int n16 = (int)(h.lds_bytes / 16);
+    for (int i = tid; i < n16; i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  const RegSeg* segs = reinterpret_cast<const RegSeg*>(lp + h.segs_off);
+  const RegOp* ops = reinterpret_cast<const RegOp*>(lp + h.ops_off);
+  const u64* ftab = reinterpret_cast<const u64*>(lp + h.ftab_off);
+  const char* ldata = lp + h.ldata_off;
+  u64 ld_thr, st_thr, ld_rb[NRB], st_rb[NRB];
+  reg_state_offsets<NRB>(segs[0], h.gbit_ld, NT, tid, ld_thr, ld_rb);
+  reg_state_offsets<NRB>(segs[nseg - 1], h.gbit_st, NT, tid, st_thr, st_rb);
+#pragma unroll
+  for (int j = 0; j < NRB; ++j) {
+    ld_rb[j] = uni64(ld_rb[j]);
+    st_rb[j] = uni64(st_rb[j]);
+  }
+  int parity = 0;
+  for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x, parity ^= 1) {
+    const u64 base = insert_bits(t, tins);
+    cplx<R>* f = fv + parity * nphase;
+    if (tid < nphase) f[tid] = reg_phase_outside<R>(reinterpret_cast<const cplx<R>*>(plan + ftab[tid]), ga, t);
+    // opaque per tile: keeps the compiler from holding 2 x 2^NRB 64-bit
+    // addresses live across the tile loop (it recomputes them instead)
+    __asm__ volatile("" : "+v"(ld_thr), "+v"(st_thr));
+    cplx<R> x[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) x[r] = st[base | ld_thr | reg_state_offset<NRB>(ld_rb, r)];
+    // the factors are written; every thread is done with the previous tile
+    __syncthreads();
+    RegLay<NRB> cur = reg_layout<NRB>(segs[0], NT, tid);
+    for (int s = 0; s < nseg; ++s) {
+      const RegSeg& sg = segs[s];
+      if (s > 0) {
+        // re-layout through LDS; the first barrier keeps the previous
+        // re-layout's reads ahead of these writes
+        RegLay<NRB> nxt = reg_layout<NRB>(sg, NT, tid);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NR; ++r) lds[lds_swz<R>(reg_elem<NRB>(cur, r))] = x[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NR; ++r) x[r] = lds[lds_swz<R>(reg_elem<NRB>(nxt, r))];
+        cur = nxt;
+      }
+#pragma unroll
+      for (int j = 0; j < NRB; ++j) cur.rb[j] = uni(cur.rb[j]);
+      const int o1 = uni(sg.o1);
+      for (int o = uni(sg.o0); o < o1; ++o) {
+        const RegOp& op = ops[o];
+        const u64 gc = uni64(op.gctrl);
+        if ((base & gc) != gc) continue;
+        const uint32_t lc = (uint32_t)uni((int)op.lctrl);
+        const cplx<R>* d = reinterpret_cast<const cplx<R>*>(ldata + uni((int)op.loff));
+        if (uni(op.kind) == QJ_TILE_PHASE)
+          reg_phase<R, NRB>(x, f[uni(op.fidx)], d, NT, tid, cur.ethr, lc, uni(op.mask));
+        else
+          reg_dense_any<R, NRB>(x, uni(op.mask), d, cur, lc);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) st[base | st_thr | reg_state_offset<NRB>(st_rb, r)] = x[r];
+  }
+}
+
+// ----------------------------------------------------------- bit swaps --
+struct SwapSpec {
+  TileSpec ts;             // tile geometry (low bits + high positions)
+  int8_t pl[16];           // tile bit -> tile bit of its partner
+  int8_t po[64];           // out-of-tile rank -> rank of its partner
+  int8_t tord[64];         // bit k of the work index -> out-of-tile rank
+};
+
+// Tile element e = (tid + i * kSwapBlock) * V + k splits into a thread part, an
+// iteration part and k, so its state offset and its permuted index are ORs
+// of per-thread and per-iteration values.  Every load of a tile is issued
+// before any LDS write.
+constexpr int kSwapBlock = 512;
+constexpr int kSwapIters = 4;  // (2^12 amplitudes) / (512 * 2), (2^11) / (512 * 1)
+
+template <typename R, int V>
+__global__ void __launch_bounds__(kSwapBlock)
+    k_bit_swaps(cplx<R>* __restrict__ st, u64 ntiles, Ins tins, SwapSpec sp) {
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  const int TB = sp.ts.TB, L = sp.ts.L, n = 1 << TB;
+  cplx<R>* a = reinterpret_cast<cplx<R>*>(smem_raw);
+  cplx<R>* b = a + n;
+  const int lmask = (1 << L) - 1;
+  auto state_off = [&](int e) { return (u64)(e & lmask) | high_offset(e >> L, sp.ts); };
+  auto perm = [&](int e) {
+    int p = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (j < TB && ((e >> j) & 1)) p |= 1 << sp.pl[j];
+    return p;
+  };
+  const int e_thr = threadIdx.x * V;
+  const bool active = e_thr < n;  // states smaller than one block's share
+  const u64 off_thr = state_off(e_thr);
+  const int p_thr = perm(e_thr);
+  const int p_one = perm(1);
+  for (u64 w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    // work index -> tile (natural order, or each bit next to its partner's)
+    u64 t = 0, t2 = 0;
+#pragma unroll
+    for (int k = 0; k < 64; ++k)
+      if (k < sp.ts.nout && ((w >> k) & 1)) t |= u64(1) << sp.tord[k];
+#pragma unroll
+    for (int r = 0; r < 64; ++r)
+      if (r < sp.ts.nout && ((t >> r) & 1)) t2 |= u64(1) << sp.po[r];
+    if (t2 < t) continue;  // the partner tile's workgroup moves both (uniform)
+    const bool pair = t2 != t;
+    const u64 base1 = insert_bits(t, tins), base2 = insert_bits(t2, tins);
+    cvec<R, V> v1[kSwapIters], v2[kSwapIters];
+#pragma unroll
+    for (int i = 0; i < kSwapIters; ++i) {
+      const int ei = i * kSwapBlock * V;
+      if (ei >= n || !active) continue;
+      const u64 off = off_thr | state_off(ei);
+      v1[i] = ldv<R, V>(st + base1 + off);
+      if (pair) v2[i] = ldv<R, V>(st + base2 + off);
+    }
+#pragma unroll
+    for (int i = 0; i < kSwapIters; ++i) {
+      const int e = e_thr + i * kSwapBlock * V;
+      if (i * kSwapBlock * V >= n || !active) continue;
+#pragma unroll
+      for (int k = 0; k < V; ++k) a[lds_swz<R>(e + k)] = v1[i].v[k];
+      if (pair) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) b[lds_swz<R>(e + k)] = v2[i].v[k];
+      }
+    }
+    __syncthreads();
+    // element e of tile t2 <- element pi(e) of tile t, and the reverse
+#pragma unroll
+    for (int i = 0; i < kSwapIters; ++i) {
+      const int ei = i * kSwapBlock * V;
+      if (ei >= n || !active) continue;
+      const u64 off = off_thr | state_off(ei);
+      const int p0 = p_thr | perm(ei);
+      cvec<R, V> o1, o2;
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        const int p = k ? (p0 | p_one) : p0;
+        o1.v[k] = a[lds_swz<R>(p)];
+        o2.v[k] = b[lds_swz<R>(p)];
+      }
+      stv<R, V>(st + base2 + off, o1);
+      if (pair) stv<R, V>(st + base1 + off, o2);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- host --
+// Per-bit phase blocks -> the LDS kernel's four-table blocks; dense matrices
+// copied.  `ops_out` is `ops` with new offsets into `data_out`.
+template <typename R>
+static void lds_kernel_data(const qj_tile_op* ops, int nops, const char* data, int TB, int ga, int gb,
+                            std::vector<qj_tile_op>& ops_out, std::vector<char>& data_out) {
+  const int la = TB < kTileLA ? TB : kTileLA;
+  ops_out.assign(ops, ops + nops);
+  data_out.clear();
+  auto put = [&](const void* src, size_t bytes) {
+    const u64 off = data_out.size();
+    data_out.resize(off + ((bytes + 15) & ~size_t(15)), 0);
+    std::memcpy(data_out.data() + off, src, bytes);
+    return off;
+  };
+  for (int o = 0; o < nops; ++o) {
+    const qj_tile_op& op = ops[o];
+    if (op.kind == QJ_TILE_DENSE) {
+      ops_out[o].offset = put(data + op.offset, ((size_t)1 << (2 * op.k)) * sizeof(cplx<R>));
+    } else if (op.kind == QJ_TILE_PHASE) {
+      const cplx<R>* src = reinterpret_cast<const cplx<R>*>(data + op.offset + 16);
+      const size_t na = size_t(1) << ga, nb = size_t(1) << gb;
+      const cplx<R>* w = src + na + nb;
+      std::vector<cplx<R>> blk(na + nb + (size_t(1) << la) + (size_t(1) << (TB - la)));
+      std::memcpy(blk.data(), src, (na + nb) * sizeof(cplx<R>));
+      auto prod = [&](int lo, int nbits, cplx<R>* out) {
+        for (int e = 0; e < (1 << nbits); ++e) {
+          double re = 1.0, im = 0.0;
+          for (int j = 0; j < nbits; ++j)
+            if ((e >> j) & 1) {
+              const double wr = (double)w[lo + j].re, wi = (double)w[lo + j].im;
+              const double nr = re * wr - im * wi;
+              im = re * wi + im * wr;
+              re = nr;
+            }
+          out[e] = mk((R)re, (R)im);
+        }
+      };
+      prod(0, la, blk.data() + na + nb);
+      prod(la, TB - la, blk.data() + na + nb + (size_t(1) << la));
+      const int32_t head[4] = {ga, gb, la, 0};
+      const u64 off = put(head, sizeof(head));
+      put(blk.data(), blk.size() * sizeof(cplx<R>));
+      ops_out[o].offset = off;
+    }
+  }
+}
+
+// LDS of one CU, and the share of it a register-kernel plan may take next to
+// a 64 KiB tile (two workgroups per CU).
+constexpr size_t kLdsPerCu = 160 * 1024;
+constexpr u64 kRegPlanLds = 15 * 1024;
+
+// Compute units and the largest LDS allocation of one workgroup, per device.
+static void device_limits(int* cus, size_t* max_lds) {
+  static int ncu[64] = {0};
+  static int mlds[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = -1;
+  if (dev >= 0 && !ncu[dev]) {
+    int n = 0, m = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    if (hipDeviceGetAttribute(&m, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || m <= 0) m = 65536;
+    ncu[dev] = n;
+    mlds[dev] = m;
+  }
+  *cus = dev >= 0 ? ncu[dev] : 256;
+  *max_lds = dev >= 0 ? (size_t)mlds[dev] : 65536;
+}
+
+// Register bits of the register kernel: 4 by default (QJ_TILE_NRB overrides:
+// 4/5 for complex64, 3/4 for complex128), at least the widest dense op.
+// Returns 0 when the pass needs the LDS kernel.
+static int tile_register_bits(bool dbl, int maxk) {
+  const char* kern = std::getenv("QJ_TILE_KERNEL");
+  if (kern && std::strcmp(kern, "lds") == 0) return 0;
+  const char* e = std::getenv("QJ_TILE_NRB");
+  const int v = e ? std::atoi(e) : 0;
+  const int lo = dbl ? 3 : 4, hi = dbl ? 4 : 5;
+  int nrb = (v >= lo && v <= hi) ? v : 4;
+  if (maxk > nrb) nrb = maxk;
+  return nrb <= hi ? nrb : 0;
+}
+
+template <typename R, int NRB, int BSMAX, int WPE>
+static void launch_reg(dim3 gd, dim3 bd, size_t lds, hipStream_t s, cplx<R>* st, u64 ntiles, const Ins& tins,
+                       const char* plan) {
+  // dynamic LDS above 64 KiB needs the kernel's limit raised first
+  if (lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_reg<R, NRB, BSMAX, WPE>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  k_tile_reg<R, NRB, BSMAX, WPE><<<gd, bd, lds, s>>>(st, ntiles, tins, plan);
+}
+
+template <typename R>
+static void launch_tile_reg(int nrb, int NT, u64 grid, size_t lds, hipStream_t s, cplx<R>* st, u64 ntiles,
+                            const Ins& tins, const char* plan) {
+  const dim3 gd((unsigned)grid), bd(1u << NT);
+  if constexpr (sizeof(R) == 4) {
+    if (nrb == 4)
+      launch_reg<R, 4, 512, 4>(gd, bd, lds, s, st, ntiles, tins, plan);
+    else
+      launch_reg<R, 5, 256, 2>(gd, bd, lds, s, st, ntiles, tins, plan);
+  } else {
+    if (nrb == 3)
+      launch_reg<R, 3, 512, 4>(gd, bd, lds, s, st, ntiles, tins, plan);
+    else
+      launch_reg<R, 4, 256, 2>(gd, bd, lds, s, st, ntiles, tins, plan);
+  }
+}
+
 template <typename R>
 static int tile_pass_impl(void* state, int nqubits, int L, const int32_t* high, int nhigh,
                           const qj_tile_op* ops, int nops, const void* data, u64 data_bytes,
@@ -294,59 +621,114 @@ static int tile_pass_impl(void* state, int nqubits, int L, const int32_t* high, 
   }
   u64 tile_mask = (u64(1) << L) - 1;
   for (int j = 0; j < nhigh; ++j) tile_mask |= u64(1) << high[j];
+  const int ga = (ts.nout + 1) / 2, gb = ts.nout - ga;
   int maxk = 1;
-  const int la = TB < kTileLA ? TB : kTileLA;
   for (int o = 0; o < nops; ++o) {
     const qj_tile_op& op = ops[o];
     if (op.global_ctrl & tile_mask)
       return set_error(QJ_EVALUE, "apply_tile_pass: op %d has a global control inside the tile", o);
+    if (op.global_ctrl >> nqubits)
+      return set_error(QJ_EVALUE, "apply_tile_pass: op %d global control beyond the state", o);
     if (op.local_ctrl >> TB) return set_error(QJ_EVALUE, "apply_tile_pass: op %d local control out of tile", o);
-    if (op.offset % 16) return set_error(QJ_EVALUE, "apply_tile_pass: op %d data offset not 16-aligned", o);
-    if (op.kind == QJ_TILE_DENSE) {
-      if (op.k < 1 || op.k > QJ_TILE_MAX_K) return set_error(QJ_EVALUE, "apply_tile_pass: op %d has k=%d", o, op.k);
+    if (op.kind == QJ_TILE_DENSE || op.kind == QJ_TILE_SWAP) {
+      const int k = op.kind == QJ_TILE_SWAP ? 2 : op.k;
+      if (op.kind == QJ_TILE_SWAP && (op.k != 2 || op.local_ctrl || op.global_ctrl))
+        return set_error(QJ_EVALUE, "apply_tile_pass: swap op %d needs k=2 and no controls", o);
+      if (k < 1 || k > QJ_TILE_MAX_K) return set_error(QJ_EVALUE, "apply_tile_pass: op %d has k=%d", o, k);
       u64 seen = 0;
-      for (int u = 0; u < op.k; ++u) {
+      for (int u = 0; u < k; ++u) {
         const int b = op.target[u];
         if (b < 0 || b >= TB || ((seen >> b) & 1) || ((op.local_ctrl >> b) & 1))
           return set_error(QJ_EVALUE, "apply_tile_pass: op %d target %d invalid", o, b);
         seen |= u64(1) << b;
       }
-      const u64 need = op.offset + ((u64(1) << (2 * op.k)) * sizeof(cplx<R>));
-      if (need > data_bytes) return set_error(QJ_EVALUE, "apply_tile_pass: op %d matrix outside data", o);
-      if (op.k > maxk) maxk = op.k;
+      if (op.kind == QJ_TILE_DENSE) {
+        if (op.offset % 16) return set_error(QJ_EVALUE, "apply_tile_pass: op %d data offset not 16-aligned", o);
+        const u64 need = op.offset + ((u64(1) << (2 * k)) * sizeof(cplx<R>));
+        if (need > data_bytes) return set_error(QJ_EVALUE, "apply_tile_pass: op %d matrix outside data", o);
+        if (k > maxk) maxk = k;
+      }
     } else if (op.kind == QJ_TILE_PHASE) {
-      if (op.offset + sizeof(PhaseHeader) > data_bytes)
+      if (op.offset % 16) return set_error(QJ_EVALUE, "apply_tile_pass: op %d data offset not 16-aligned", o);
+      if (op.offset + 16 > data_bytes)
         return set_error(QJ_EVALUE, "apply_tile_pass: op %d phase data outside data", o);
-      PhaseHeader hh;
-      std::memcpy(&hh, static_cast<const char*>(data) + op.offset, sizeof(hh));
-      if (hh.ga < 0 || hh.gb < 0 || hh.ga + hh.gb != ts.nout || hh.la != la)
-        return set_error(QJ_EVALUE, "apply_tile_pass: op %d phase table shape (%d,%d,%d) does not match the tile",
-                         o, hh.ga, hh.gb, hh.la);
-      const u64 need = op.offset + sizeof(PhaseHeader) +
-                       ((u64(1) << hh.ga) + (u64(1) << hh.gb) + (u64(1) << la) + (u64(1) << (TB - la))) *
-                           sizeof(cplx<R>);
+      int32_t hh[4];
+      std::memcpy(hh, static_cast<const char*>(data) + op.offset, sizeof(hh));
+      if (hh[0] != ga || hh[1] != gb || hh[2] != TB)
+        return set_error(QJ_EVALUE,
+                         "apply_tile_pass: op %d phase block shape (%d,%d,%d) does not match the tile (%d,%d,%d)",
+                         o, hh[0], hh[1], hh[2], ga, gb, TB);
+      const u64 need = op.offset + 16 + ((u64(1) << ga) + (u64(1) << gb) + (u64)TB) * sizeof(cplx<R>);
       if (need > data_bytes) return set_error(QJ_EVALUE, "apply_tile_pass: op %d phase tables outside data", o);
     } else {
       return set_error(QJ_EVALUE, "apply_tile_pass: op %d has unknown kind %d", o, op.kind);
     }
   }
-  // ops and data travel in one staging upload, ops first (64 B each, so the
-  // data block starts 16-byte aligned)
+  const u64 ntiles = u64(1) << (nqubits - TB);
+  const u64 grid = ntiles < kMaxGrid ? ntiles : kMaxGrid;
+  const size_t lds = ((size_t)1 << TB) * sizeof(cplx<R>);
+  cplx<R>* st = static_cast<cplx<R>*>(state);
+  const char* cdata = static_cast<const char*>(data);
+
+  const int nrb = tile_register_bits(sizeof(R) == 8, maxk);
+  if (nrb) {
+    RegPlanIn in;
+    std::memset(&in, 0, sizeof(in));
+    in.TB = TB;
+    in.NRB = nrb;
+    in.S = sizeof(R) == 4 ? 5 : 4;
+    in.esize = (int)sizeof(cplx<R>);
+    in.ga = ga;
+    in.gb = gb;
+    for (int b = 0; b < TB; ++b) in.gpos[b] = tins.pos[b];
+    in.ops = ops;
+    in.nops = nops;
+    in.data = cdata;
+    in.lds_cap = kRegPlanLds;
+    std::vector<char> plan;
+    RegHead hh;
+    size_t lds_reg = 0;
+    int cus = 0;
+    size_t max_lds = 0;
+    device_limits(&cus, &max_lds);
+    const bool planned = reg_plan(in, plan);
+    if (planned) {
+      std::memcpy(&hh, plan.data(), sizeof(hh));
+      lds_reg = lds + hh.lds_bytes + 2 * (size_t)hh.nphase * sizeof(cplx<R>);
+    }
+    if (planned && lds_reg <= max_lds) {
+      const int bs = 1 << (TB - nrb);
+      int per_cu = (int)(kLdsPerCu / lds_reg);
+      per_cu = std::min(per_cu, 2048 / bs);
+      per_cu = std::max(per_cu, 1);
+      const u64 rgrid = std::min<u64>(ntiles, (u64)cus * per_cu);
+      void* dplan = nullptr;
+      int ticket = -1;
+      int rc = ring_upload(plan.data(), plan.size(), s, &dplan, &ticket);
+      if (rc) return rc;
+      launch_tile_reg<R>(nrb, TB - nrb, rgrid, lds_reg, s, st, ntiles, tins, static_cast<const char*>(dplan));
+      rc = check_launch("apply_tile_pass launch");
+      ring_release(ticket, s);
+      return rc;
+    }
+  }
+
+  // LDS kernel: ops and data travel in one staging upload, ops first (64 B
+  // each, so the data block starts 16-byte aligned)
+  std::vector<qj_tile_op> lops;
+  std::vector<char> ldata;
+  lds_kernel_data<R>(ops, nops, cdata, TB, ga, gb, lops, ldata);
   const size_t ops_bytes = (size_t)nops * sizeof(qj_tile_op);
   // never empty: an empty vector's data() is null and the ring copies from it
-  std::vector<char> buf(ops_bytes + data_bytes > 16 ? ops_bytes + data_bytes : 16, 0);
-  if (ops_bytes) std::memcpy(buf.data(), ops, ops_bytes);
-  if (data_bytes) std::memcpy(buf.data() + ops_bytes, data, data_bytes);
+  std::vector<char> buf(ops_bytes + ldata.size() > 16 ? ops_bytes + ldata.size() : 16, 0);
+  if (ops_bytes) std::memcpy(buf.data(), lops.data(), ops_bytes);
+  if (!ldata.empty()) std::memcpy(buf.data() + ops_bytes, ldata.data(), ldata.size());
   void* dbuf = nullptr;
   int ticket = -1;
   int rc = ring_upload(buf.data(), buf.size(), s, &dbuf, &ticket);
   if (rc) return rc;
   const qj_tile_op* dops = static_cast<const qj_tile_op*>(dbuf);
   const char* ddata = static_cast<const char*>(dbuf) + ops_bytes;
-  const u64 ntiles = u64(1) << (nqubits - TB);
-  const u64 grid = ntiles < kMaxGrid ? ntiles : kMaxGrid;
-  const size_t lds = ((size_t)1 << TB) * sizeof(cplx<R>);
-  cplx<R>* st = static_cast<cplx<R>*>(state);
   const int bs = tile_block_size();
   constexpr int VV = 16 / (int)sizeof(cplx<R>);
   if (maxk <= 2)
@@ -358,6 +740,93 @@ static int tile_pass_impl(void* state, int nqubits, int L, const int32_t* high, 
   rc = check_launch("apply_tile_pass launch");
   ring_release(ticket, s);
   return rc;
+}
+
+template <typename R>
+static int bit_swaps_impl(void* state, int nqubits, const int32_t* pairs, int npairs, hipStream_t s) {
+  if (nqubits < 1 || nqubits > 62) return set_error(QJ_EVALUE, "apply_bit_swaps: bad nqubits %d", nqubits);
+  if (npairs < 0 || npairs > nqubits / 2) return set_error(QJ_EVALUE, "apply_bit_swaps: bad npairs %d", npairs);
+  int partner[64];
+  for (int b = 0; b < 64; ++b) partner[b] = b;
+  for (int j = 0; j < npairs; ++j) {
+    const int a = pairs[2 * j], c = pairs[2 * j + 1];
+    if (a < 0 || c < 0 || a >= nqubits || c >= nqubits || a == c || partner[a] != a || partner[c] != c)
+      return set_error(QJ_EVALUE, "apply_bit_swaps: pair %d (%d, %d) invalid or not disjoint", j, a, c);
+    partner[a] = c;
+    partner[c] = a;
+  }
+  if (npairs == 0) return QJ_OK;
+  constexpr int V = 16 / (int)sizeof(cplx<R>);
+  // tile: the 512-byte run of low bits, their partners, and further bits
+  // (unswapped ones, or both bits of a pair) up to 32 KiB of amplitudes, so
+  // that the tile set is closed under the permutation
+  const int L = std::min(sizeof(R) == 4 ? 6 : 5, nqubits);
+  const int tb_max = sizeof(R) == 4 ? 12 : 11;
+  u64 in_tile = (u64(1) << L) - 1;
+  for (int b = 0; b < L; ++b) in_tile |= u64(1) << partner[b];
+  int tb = __builtin_popcountll(in_tile);
+  for (int b = L; b < nqubits && tb < tb_max; ++b) {
+    if ((in_tile >> b) & 1) continue;
+    const int add = partner[b] == b ? 1 : 2;
+    if (tb + add > tb_max || tb + add - L > kTileMaxHighBits) continue;
+    in_tile |= (u64(1) << b) | (u64(1) << partner[b]);
+    tb += add;
+  }
+  SwapSpec sp;
+  std::memset(&sp, 0, sizeof(sp));
+  TileSpec& ts = sp.ts;
+  ts.L = L;
+  ts.TB = tb;
+  ts.H = tb - L;
+  ts.nout = nqubits - tb;
+  Ins tins;
+  std::memset(&tins, 0, sizeof(tins));
+  int local_of[64], gpos[16];
+  int nl = 0;
+  for (int b = 0; b < nqubits; ++b)
+    if ((in_tile >> b) & 1) {
+      if (b >= L) ts.hb[nl - L] = b;
+      tins.pos[nl] = b;
+      gpos[nl] = b;
+      local_of[b] = nl++;
+    }
+  tins.npos = tb;
+  for (int i = 0; i < tb; ++i) sp.pl[i] = (int8_t)local_of[partner[gpos[i]]];
+  int rank_of[64], outside[64];
+  int no = 0;
+  for (int b = 0; b < nqubits; ++b)
+    if (!((in_tile >> b) & 1)) {
+      outside[no] = b;
+      rank_of[b] = no++;
+    }
+  for (int r = 0; r < no; ++r) sp.po[r] = (int8_t)rank_of[partner[outside[r]]];
+  // work order: natural (tile index = work index), or paired (each bit
+  // next to its partner's bit; QJ_SWAP_ORDER=paired)
+  const char* order = std::getenv("QJ_SWAP_ORDER");
+  const bool paired = order && std::strcmp(order, "paired") == 0;
+  {
+    int k = 0;
+    u64 placed = 0;
+    for (int r = 0; r < no; ++r) {
+      if ((placed >> r) & 1) continue;
+      sp.tord[k++] = (int8_t)r;
+      placed |= u64(1) << r;
+      const int q = sp.po[r];
+      if (paired && !((placed >> q) & 1)) {
+        sp.tord[k++] = (int8_t)q;
+        placed |= u64(1) << q;
+      }
+    }
+  }
+  const u64 ntiles = u64(1) << ts.nout;
+  const size_t lds = 2 * ((size_t)1 << tb) * sizeof(cplx<R>);
+  int cus = 0;
+  size_t max_lds = 0;
+  device_limits(&cus, &max_lds);
+  const u64 per_cu = std::max<u64>(1, std::min<u64>(kLdsPerCu / lds, 2048 / kSwapBlock));
+  const u64 grid = std::min<u64>(ntiles, (u64)cus * per_cu);
+  k_bit_swaps<R, V><<<dim3((unsigned)grid), dim3(kSwapBlock), lds, s>>>(static_cast<cplx<R>*>(state), ntiles, tins, sp);
+  return check_launch("apply_bit_swaps launch");
 }
 
 }  // namespace qj
@@ -377,4 +846,14 @@ extern "C" int qj_apply_tile_pass(void* state, int dtype, int nqubits, int low_b
   if (dtype == QJ_COMPLEX128)
     return tile_pass_impl<double>(state, nqubits, low_bits, high_bits, nhigh, ops, nops, data, data_bytes, s);
   return set_error(QJ_EVALUE, "apply_tile_pass: dtype must be complex64/complex128");
+}
+
+extern "C" int qj_apply_bit_swaps(void* state, int dtype, int nqubits, const int32_t* pairs, int npairs,
+                                  void* stream) {
+  if (!state) return set_error(QJ_EVALUE, "apply_bit_swaps: null state");
+  if (npairs > 0 && !pairs) return set_error(QJ_EVALUE, "apply_bit_swaps: null pairs");
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == QJ_COMPLEX64) return bit_swaps_impl<float>(state, nqubits, pairs, npairs, s);
+  if (dtype == QJ_COMPLEX128) return bit_swaps_impl<double>(state, nqubits, pairs, npairs, s);
+  return set_error(QJ_EVALUE, "apply_bit_swaps: dtype must be complex64/complex128");
 }

@@ -13,7 +13,12 @@ positions.  Gates become ops of two kinds:
   any pass wherever their bits are, and consecutive ones merge: same
   control set -> angles add; a controlled phase with one control and no
   constant is symmetric in its two bits, so a ladder CU1(j, i), j > i, of
-  a QFT merges into one op controlled by i.
+  a QFT merges into one op controlled by i;
+* swap: an uncontrolled SWAP, a relabelling of two tile bits inside a pass.
+
+A run of consecutive SWAPs on distinct qubits that does not fit the current
+pass becomes a bit-swap pass of its own (``qj_apply_bit_swaps``: one read
+and one write for the whole run, e.g. the final swaps of a QFT).
 
 Passes keep the gates in circuit order, so the result equals gate-by-gate
 execution up to rounding.
@@ -25,7 +30,7 @@ import numpy as np
 
 from qibojit_amd.fusion import target_block
 
-TILE_DENSE, TILE_PHASE = 0, 1
+TILE_DENSE, TILE_PHASE, TILE_SWAP = 0, 1, 2
 MAX_K = 5
 TILE_OP_DTYPE = np.dtype(
     [
@@ -51,6 +56,17 @@ class DenseOp:
 
     def bits(self):
         return set(self.targets) | set(self.controls)
+
+
+class SwapOp:
+    """Exchange of two global bits (an uncontrolled SWAP)."""
+
+    def __init__(self, a, b):
+        self.targets = [a, b]
+        self.controls = frozenset()
+
+    def bits(self):
+        return set(self.targets)
 
 
 class PhaseOp:
@@ -94,6 +110,8 @@ def gate_to_op(custom_matrix_of, gate, nqubits):
     u = target_block(custom_matrix_of, gate)
     tbits = [nqubits - 1 - q for q in targets]
     cbits = [nqubits - 1 - q for q in gate.control_qubits]
+    if type(gate).__name__ == "SWAP" and not cbits:
+        return SwapOp(*tbits)
     if len(targets) == 1 and u[0, 1] == 0 and u[1, 0] == 0:
         d0, d1 = complex(u[0, 0]), complex(u[1, 1])
         if abs(abs(d0) - 1) < 1e-14 and abs(abs(d1) - 1) < 1e-14:
@@ -111,6 +129,8 @@ def column_op(op, nqubits):
     """
     if isinstance(op, DenseOp):
         return DenseOp([b - nqubits for b in op.targets], [c - nqubits for c in op.controls], np.conj(op.matrix))
+    if isinstance(op, SwapOp):
+        return SwapOp(*(b - nqubits for b in op.targets))
     return PhaseOp([c - nqubits for c in op.controls], -op.c0, {b - nqubits: -t for b, t in op.thetas.items()})
 
 
@@ -135,7 +155,7 @@ class Pass:
         return len(self.high) + len(need) <= max_high
 
     def add(self, op):
-        if isinstance(op, DenseOp):
+        if isinstance(op, (DenseOp, SwapOp)):
             for b in op.targets:
                 if b >= self.low and b not in self.high:
                     self.high.append(b)
@@ -147,38 +167,75 @@ class Pass:
         self.ops.append(op)
 
 
+class SwapPass:
+    """Disjoint bit swaps applied in one pass (qj_apply_bit_swaps)."""
+
+    def __init__(self, pairs):
+        self.pairs = [tuple(p) for p in pairs]
+
+
 def fits_alone(op, low, max_high):
     """Whether ``op`` fits an empty pass of this geometry."""
     return Pass(low).fits(op, max_high)
 
 
+def _swap_run(ops, i):
+    """End of the run of SwapOps on distinct bits starting at ops[i]."""
+    used, j = set(), i
+    while j < len(ops) and isinstance(ops[j], SwapOp) and not used & set(ops[j].targets):
+        used |= set(ops[j].targets)
+        j += 1
+    return j
+
+
 def plan(ops, nqubits, low, max_high):
     """Greedy cut of an op list into passes (circuit order preserved); every
-    op must satisfy ``fits_alone``."""
+    op must satisfy ``fits_alone``.  A run of two or more swaps on distinct
+    bits that does not fit the current pass becomes a SwapPass."""
     passes = []
     cur = Pass(low)
-    for op in ops:
+    i = 0
+    while i < len(ops):
+        op = ops[i]
+        if isinstance(op, SwapOp):
+            j = _swap_run(ops, i)
+            if j - i >= 2:
+                trial = Pass(low)
+                trial.high = list(cur.high)
+                fits = True
+                for o in ops[i:j]:
+                    fits = fits and trial.fits(o, max_high)
+                    trial.add(o)
+                if fits:
+                    for o in ops[i:j]:
+                        cur.add(o)
+                else:
+                    if cur.ops:
+                        passes.append(cur)
+                    passes.append(SwapPass([o.targets for o in ops[i:j]]))
+                    cur = Pass(low)
+                i = j
+                continue
         if not cur.fits(op, max_high):
             passes.append(cur)
             cur = Pass(low)
         cur.add(op)
+        i += 1
     if cur.ops:
         passes.append(cur)
     for p in passes:
+        if isinstance(p, SwapPass):
+            continue
         # widen the tile to the full budget with the lowest unused bits
         free = [b for b in range(low, nqubits) if b not in p.high]
         p.high = sorted(p.high + free[: max(0, min(max_high, nqubits - low) - len(p.high))])
     return passes
 
 
-PHASE_LA = 8  # tile-local bits indexing phase table A (kernel constant)
-
-
 def encode(p, nqubits, npdtype):
     """(ops array, data bytes, high bits) of a pass for qj_apply_tile_pass."""
     low, high = p.low, p.high
     tb = low + len(high)
-    la = min(PHASE_LA, tb)
     local = {b: b for b in range(low)}
     local.update({b: low + j for j, b in enumerate(high)})
     # out-of-tile bits, ascending: bit r of the tile index t is outside[r]
@@ -198,6 +255,11 @@ def encode(p, nqubits, npdtype):
         return start
 
     for i, op in enumerate(p.ops):
+        if isinstance(op, SwapOp):
+            ops[i]["kind"] = TILE_SWAP
+            ops[i]["k"] = 2
+            ops[i]["target"][:2] = [local[b] for b in op.targets]
+            continue
         lmask = sum(1 << local[c] for c in op.controls if c in local)
         gmask = sum(1 << c for c in op.controls if c not in local)
         ops[i]["local_ctrl"] = lmask
@@ -209,29 +271,34 @@ def encode(p, nqubits, npdtype):
             ops[i]["offset"] = put(np.ascontiguousarray(op.matrix, dtype=npdtype).tobytes())
         else:
             ops[i]["kind"] = TILE_PHASE
-            head = np.array([ga, nout - ga, la, 0], dtype="<i4")
-            # angle tables: OA/OB over the tile index t (c0 folded into OA),
-            # A/B over the tile-local bits below / from la
-            tabs = [np.zeros(2**ga), np.zeros(2 ** (nout - ga)), np.zeros(2**la), np.zeros(2 ** (tb - la))]
-            tabs[0] += op.c0
+            head = np.array([ga, nout - ga, tb, 0], dtype="<i4")
+            # OA/OB: angle tables over the tile index t (c0 folded into OA);
+            # W: one angle per tile-local bit
+            oa, ob, w = np.zeros(2**ga), np.zeros(2 ** (nout - ga)), np.zeros(tb)
+            oa += op.c0
             for b, th in op.thetas.items():
                 if b in local:
-                    lb = local[b]
-                    k, sh = (2, lb) if lb < la else (3, lb - la)
+                    w[local[b]] += th
                 else:
                     r = rank[b]
-                    k, sh = (0, r) if r < ga else (1, r - ga)
-                idx = np.arange(tabs[k].size)
-                tabs[k] += th * ((idx >> sh) & 1)
-            blob = head.tobytes() + b"".join(np.exp(1j * a).astype(npdtype).tobytes() for a in tabs)
+                    tab, sh = (oa, r) if r < ga else (ob, r - ga)
+                    tab += th * ((np.arange(tab.size) >> sh) & 1)
+            blob = head.tobytes() + b"".join(np.exp(1j * a).astype(npdtype).tobytes() for a in (oa, ob, w))
             ops[i]["offset"] = put(blob)
     data = b"".join(chunks)
     return ops, data, high
 
 
 def run_passes(lib_call, state_ptr, dtype_code, nqubits, passes, npdtype, stream):
-    """Issue one qj_apply_tile_pass per pass (stream-ordered, no host sync)."""
+    """Issue one qj_apply_tile_pass (or qj_apply_bit_swaps) per pass
+    (stream-ordered, no host sync)."""
     for p in passes:
+        if isinstance(p, SwapPass):
+            flat = [b for pair in p.pairs for b in pair]
+            arr = (ctypes.c_int32 * len(flat))(*flat)
+            lib_call("qj_apply_bit_swaps", state_ptr, dtype_code, nqubits,
+                     ctypes.cast(arr, ctypes.POINTER(ctypes.c_int32)), len(p.pairs), stream)
+            continue
         ops, data, high = encode(p, nqubits, npdtype)
         harr = (ctypes.c_int32 * max(1, len(high)))(*high)
         dbuf = ctypes.create_string_buffer(data, max(1, len(data)))

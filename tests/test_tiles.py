@@ -10,7 +10,7 @@ from oracle import gate_matrices as gm
 from oracle import port
 from qibojit_amd import tiles
 from qibojit_amd.qibo_compat import gates
-from tests.tile_emulator import apply_tile_pass
+from tests.tile_emulator import apply_bit_swaps, apply_tile_pass
 
 
 def _run_planned(gate_list, psi, nbits, low, max_high, density_nq=None):
@@ -22,6 +22,9 @@ def _run_planned(gate_list, psi, nbits, low, max_high, density_nq=None):
         if pending:
             passes = tiles.plan(pending, nbits, low, max_high)
             for p in passes:
+                if isinstance(p, tiles.SwapPass):
+                    state = apply_bit_swaps(state, nbits, p.pairs)
+                    continue
                 assert len(p.high) <= max_high and all(b >= low for b in p.high)
                 enc_ops, data, high = tiles.encode(p, nbits, psi.dtype)
                 state = apply_tile_pass(state, nbits, p.low, high, enc_ops, data, psi.dtype)
@@ -96,8 +99,11 @@ def test_qft_phase_ladders_merge():
         want = port.PortBackend(2).apply_gate(g, want, n)
     got, passes = _run_planned(c, psi, n, 4, 5)
     np.testing.assert_allclose(got, want, atol=1e-12)
-    nphase = sum(isinstance(op, tiles.PhaseOp) for p in passes for op in p.ops)
+    nphase = sum(isinstance(op, tiles.PhaseOp) for p in passes if isinstance(p, tiles.Pass) for op in p.ops)
     assert nphase <= n  # one merged ladder per H
+    # the final swaps (bit b <-> bit n-1-b) are one bit-swap pass
+    swaps = [p for p in passes if isinstance(p, tiles.SwapPass)]
+    assert len(swaps) == 1 and sorted(map(sorted, swaps[0].pairs)) == [[b, n - 1 - b] for b in range(n // 2)]
     # closed form: QFT matrix on the bit-reversed... compare with the DFT
     dft = np.fft.ifft(psi) * np.sqrt(2**n)
     np.testing.assert_allclose(got, dft, atol=1e-12)
@@ -140,8 +146,11 @@ def test_phase_merge_rules():
 def test_diagonal_gates_become_phase_ops():
     for g in (gates.Z(0), gates.RZ(1, 0.3), gates.CU1(0, 2, 0.5), gates.CZ(1, 2), gates.T(0), gates.CCZ(0, 1, 2)):
         assert isinstance(tiles.gate_to_op(port.custom_matrix, g, 3), tiles.PhaseOp), g.name
-    for g in (gates.H(0), gates.X(1), gates.CNOT(0, 1), gates.SWAP(0, 2), gates.fSim(0, 1, 0.1, 0.2)):
+    for g in (gates.H(0), gates.X(1), gates.CNOT(0, 1), gates.SWAP(0, 2).controlled_by(1),
+              gates.fSim(0, 1, 0.1, 0.2)):
         assert isinstance(tiles.gate_to_op(port.custom_matrix, g, 3), tiles.DenseOp), g.name
+    op = tiles.gate_to_op(port.custom_matrix, gates.SWAP(0, 2), 3)
+    assert isinstance(op, tiles.SwapOp) and op.targets == [2, 0]
     psi = gm.random_state(3, np.random.default_rng(1))
     got, _ = _run_planned([gates.RZ(1, 0.3)], psi, 3, 1, 2)
     np.testing.assert_allclose(got, er.apply_gate(psi, gm.RZ(0.3), [1]), atol=1e-15)
@@ -174,3 +183,39 @@ def test_plan_blocks_fuses_a_brickwork():
     blocks = plan_blocks(seq, 5)
     assert len(blocks) <= len(seq) / 4
     assert max(len(q) for q, _ in blocks) == 5
+
+
+def test_swap_runs_become_bit_swap_passes():
+    """In-tile swaps relabel bits; a run of swaps on distinct qubits that
+    does not fit the pass is one bit-swap pass (emulated on the CPU)."""
+    n = 11
+    rng = np.random.default_rng(21)
+    c = []
+    for q in range(n):
+        c.append(gates.U3(q, *rng.uniform(0, 2 * np.pi, 3)))
+    c += [gates.SWAP(0, 10), gates.SWAP(1, 9), gates.SWAP(2, 3)]          # run of 3, spans the state
+    c += [gates.CU1(4, 0, 0.3), gates.SWAP(5, 6), gates.H(6)]            # lone swap: in a tile pass
+    c += [gates.SWAP(7, 8), gates.SWAP(8, 9), gates.RX(8, 0.4)]           # overlapping: two runs of 1
+    c += [gates.SWAP(0, 1).controlled_by(4), gates.CZ(2, 9)]              # controlled swap stays dense
+    psi = gm.random_state(n, rng)
+    want = psi.copy()
+    for g in c:
+        want = port.PortBackend(2).apply_gate(g, want, n)
+    got, passes = _run_planned(c, psi, n, 3, 4)
+    np.testing.assert_allclose(got, want, atol=1e-12)
+    assert any(isinstance(p, tiles.SwapPass) and len(p.pairs) == 3 for p in passes)
+    in_tile = [op for p in passes if isinstance(p, tiles.Pass) for op in p.ops if isinstance(op, tiles.SwapOp)]
+    assert len(in_tile) >= 2
+
+
+def test_bit_swaps_emulator_closed_form():
+    """apply_bit_swaps == numpy axis permutation (qubit q = axis q, bit n-1-q)."""
+    n = 7
+    psi = gm.random_state(n, np.random.default_rng(8))
+    pairs = [(0, 6), (2, 4)]
+    perm = list(range(n))
+    for a, b in pairs:
+        qa, qb = n - 1 - a, n - 1 - b
+        perm[qa], perm[qb] = perm[qb], perm[qa]
+    want = np.transpose(psi.reshape((2,) * n), perm).reshape(-1)
+    np.testing.assert_allclose(apply_bit_swaps(psi, n, pairs), want, atol=0)

@@ -4,7 +4,7 @@ the CPU (no GPU)."""
 
 import numpy as np
 
-from qibojit_amd.tiles import TILE_DENSE, TILE_PHASE
+from qibojit_amd.tiles import TILE_DENSE, TILE_PHASE, TILE_SWAP
 
 
 def apply_tile_pass(state, nqubits, low, high, ops, data, npdtype):
@@ -38,15 +38,31 @@ def apply_tile_pass(state, nqubits, low, high, ops, data, npdtype):
                 rows = [i | sum(((r >> (k - 1 - u)) & 1) << tg[u] for u in range(k)) for r in range(n)]
                 new[rows] = m @ out[rows]
             out = new
+        elif op["kind"] == TILE_SWAP:
+            a, b = (pos[int(t)] for t in op["target"][:2])
+            flip = ((idx >> a) & 1) != ((idx >> b) & 1)
+            partner = idx ^ ((1 << a) | (1 << b))
+            out = np.where(flip, out[partner], out)
         else:
             assert op["kind"] == TILE_PHASE
-            ga, gb, la, _ = (int(v) for v in np.frombuffer(data, dtype="<i4", count=4, offset=off))
-            assert ga + gb == len(outside) and la == min(8, tb)
+            ga, gb, tbh, _ = (int(v) for v in np.frombuffer(data, dtype="<i4", count=4, offset=off))
+            assert ga + gb == len(outside) and ga == (len(outside) + 1) // 2 and tbh == tb
             tabs, o = [], off + 16
-            for size in (2**ga, 2**gb, 2**la, 2 ** (tb - la)):
+            for size in (2**ga, 2**gb, tb):
                 tabs.append(np.frombuffer(data, dtype=npdtype, count=size, offset=o).astype(np.complex128))
                 o += size * itemsize
-            f = (tabs[0][tindex & (2**ga - 1)] * tabs[1][tindex >> ga]
-                 * tabs[2][local & (2**la - 1)] * tabs[3][local >> la])
+            f = tabs[0][tindex & (2**ga - 1)] * tabs[1][tindex >> ga]
+            for b in range(tb):
+                f = np.where((local >> b) & 1, f * tabs[2][b], f)
             out = np.where(ok, out * f, out)
     return out.astype(state.dtype)
+
+
+def apply_bit_swaps(state, nqubits, pairs):
+    """qj_apply_bit_swaps: psi'(i) = psi(i with the bits of each pair exchanged)."""
+    idx = np.arange(state.size, dtype=np.int64)
+    src = idx.copy()
+    for a, b in pairs:
+        flip = ((idx >> a) & 1) != ((idx >> b) & 1)
+        src = np.where(flip, src ^ ((1 << a) | (1 << b)), src)
+    return np.asarray(state)[src]
