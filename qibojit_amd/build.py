@@ -34,6 +34,12 @@ CFLAGS = [
 ]
 
 
+# Per-source flags.  qj_tile.hip: branches on wave-uniform values (the op
+# dispatch of the register tile kernel) stay unstructured, which keeps the
+# register-resident tile from being copied around every dispatch case.
+EXTRA = {"qj_tile.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]}
+
+
 def _run(cmd):
     proc = subprocess.run(cmd, capture_output=True, text=True, check=False)
     if proc.returncode != 0:
@@ -48,7 +54,7 @@ def build(verbose=False):
 
     def compile_one(src):
         obj = os.path.join(BUILD_DIR, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, *CFLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         _run(cmd)

@@ -356,33 +356,38 @@ __global__ void __launch_bounds__(BSMAX, WPE)
     // the factors are written; every thread is done with the previous tile
     __syncthreads();
     RegLay<NRB> cur = reg_layout<NRB>(segs[0], NT, tid);
+#pragma unroll
+    for (int j = 0; j < NRB; ++j) cur.rb[j] = uni(cur.rb[j]);
     for (int s = 0; s < nseg; ++s) {
       const RegSeg& sg = segs[s];
       if (s > 0) {
         // re-layout through LDS; the first barrier keeps the previous
         // re-layout's reads ahead of these writes
         RegLay<NRB> nxt = reg_layout<NRB>(sg, NT, tid);
+#pragma unroll
+        for (int j = 0; j < NRB; ++j) nxt.rb[j] = uni(nxt.rb[j]);
+        const int sw_cur = lds_swz<R>(cur.ethr), sw_nxt = lds_swz<R>(nxt.ethr);
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < NR; ++r) lds[lds_swz<R>(reg_elem<NRB>(cur, r))] = x[r];
+        for (int r = 0; r < NR; ++r) lds[reg_lds<R, NRB>(sw_cur, cur, r)] = x[r];
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < NR; ++r) x[r] = lds[lds_swz<R>(reg_elem<NRB>(nxt, r))];
+        for (int r = 0; r < NR; ++r) x[r] = lds[reg_lds<R, NRB>(sw_nxt, nxt, r)];
         cur = nxt;
       }
-#pragma unroll
-      for (int j = 0; j < NRB; ++j) cur.rb[j] = uni(cur.rb[j]);
       const int o1 = uni(sg.o1);
       for (int o = uni(sg.o0); o < o1; ++o) {
         const RegOp& op = ops[o];
         const u64 gc = uni64(op.gctrl);
         if ((base & gc) != gc) continue;
-        const uint32_t lc = (uint32_t)uni((int)op.lctrl);
+        const uint32_t lct = (uint32_t)uni((int)op.lct);
+        if (((uint32_t)cur.ethr & lct) != lct) continue;  // threads failing thread-bit controls
+        const int kind = uni(op.kind), rcm = uni(op.rcm);
         const cplx<R>* d = reinterpret_cast<const cplx<R>*>(ldata + uni((int)op.loff));
-        if (uni(op.kind) == QJ_TILE_PHASE)
-          reg_phase<R, NRB>(x, f[uni(op.fidx)], d, NT, tid, cur.ethr, lc, uni(op.mask));
+        if (kind == kRegPhase)
+          reg_phase<R, NRB>(x, f[uni(op.fidx)], d, NT, tid, rcm);
         else
-          reg_dense_any<R, NRB>(x, uni(op.mask), d, cur, lc);
+          reg_dense_any<R, NRB>(x, kind, uni(op.mask), d, rcm);
       }
     }
 #pragma unroll
@@ -394,20 +399,21 @@ __global__ void __launch_bounds__(BSMAX, WPE)
 struct SwapSpec {
   TileSpec ts;             // tile geometry (low bits + high positions)
   int8_t pl[16];           // tile bit -> tile bit of its partner
-  int8_t po[64];           // out-of-tile rank -> rank of its partner
-  int8_t tord[64];         // bit k of the work index -> out-of-tile rank
+  int nbytes;              // bytes of the tile index looked up in `tabs`
 };
 
 // Tile element e = (tid + i * kSwapBlock) * V + k splits into a thread part, an
 // iteration part and k, so its state offset and its permuted index are ORs
-// of per-thread and per-iteration values.  Every load of a tile is issued
-// before any LDS write.
+// of per-thread and per-iteration values.  The state offsets of tile t and
+// of its partner tile come from byte tables (tabs[2*byte] for the tile,
+// tabs[2*byte+1] for the partner, 256 entries each).  Every load of a tile
+// is issued before any LDS write.
 constexpr int kSwapBlock = 512;
 constexpr int kSwapIters = 4;  // (2^12 amplitudes) / (512 * 2), (2^11) / (512 * 1)
 
 template <typename R, int V>
 __global__ void __launch_bounds__(kSwapBlock)
-    k_bit_swaps(cplx<R>* __restrict__ st, u64 ntiles, Ins tins, SwapSpec sp) {
+    k_bit_swaps(cplx<R>* __restrict__ st, u64 ntiles, SwapSpec sp, const u64* __restrict__ tabs) {
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   const int TB = sp.ts.TB, L = sp.ts.L, n = 1 << TB;
   cplx<R>* a = reinterpret_cast<cplx<R>*>(smem_raw);
@@ -426,26 +432,29 @@ __global__ void __launch_bounds__(kSwapBlock)
   const u64 off_thr = state_off(e_thr);
   const int p_thr = perm(e_thr);
   const int p_one = perm(1);
-  for (u64 w = blockIdx.x; w < ntiles; w += gridDim.x) {
-    // work index -> tile (natural order, or each bit next to its partner's)
-    u64 t = 0, t2 = 0;
+  u64 off_it[kSwapIters];
+  int p_it[kSwapIters];
 #pragma unroll
-    for (int k = 0; k < 64; ++k)
-      if (k < sp.ts.nout && ((w >> k) & 1)) t |= u64(1) << sp.tord[k];
-#pragma unroll
-    for (int r = 0; r < 64; ++r)
-      if (r < sp.ts.nout && ((t >> r) & 1)) t2 |= u64(1) << sp.po[r];
-    if (t2 < t) continue;  // the partner tile's workgroup moves both (uniform)
-    const bool pair = t2 != t;
-    const u64 base1 = insert_bits(t, tins), base2 = insert_bits(t2, tins);
+  for (int i = 0; i < kSwapIters; ++i) {
+    off_it[i] = off_thr | state_off(i * kSwapBlock * V);
+    p_it[i] = p_thr | perm(i * kSwapBlock * V);
+  }
+  const int nbytes = sp.nbytes;
+  for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    u64 base1 = 0, base2 = 0;
+    for (int k = 0; k < nbytes; ++k) {
+      const int byte = (int)((t >> (8 * k)) & 255);
+      base1 |= tabs[(2 * k) * 256 + byte];
+      base2 |= tabs[(2 * k + 1) * 256 + byte];
+    }
+    if (base2 < base1) continue;  // the partner tile's workgroup moves both (uniform)
+    const bool pair = base2 != base1;
     cvec<R, V> v1[kSwapIters], v2[kSwapIters];
 #pragma unroll
     for (int i = 0; i < kSwapIters; ++i) {
-      const int ei = i * kSwapBlock * V;
-      if (ei >= n || !active) continue;
-      const u64 off = off_thr | state_off(ei);
-      v1[i] = ldv<R, V>(st + base1 + off);
-      if (pair) v2[i] = ldv<R, V>(st + base2 + off);
+      if (i * kSwapBlock * V >= n || !active) continue;
+      v1[i] = ldv<R, V>(st + (base1 | off_it[i]));
+      if (pair) v2[i] = ldv<R, V>(st + (base2 | off_it[i]));
     }
 #pragma unroll
     for (int i = 0; i < kSwapIters; ++i) {
@@ -459,22 +468,19 @@ __global__ void __launch_bounds__(kSwapBlock)
       }
     }
     __syncthreads();
-    // element e of tile t2 <- element pi(e) of tile t, and the reverse
+    // element e of the partner tile <- element pi(e) of this tile, and the reverse
 #pragma unroll
     for (int i = 0; i < kSwapIters; ++i) {
-      const int ei = i * kSwapBlock * V;
-      if (ei >= n || !active) continue;
-      const u64 off = off_thr | state_off(ei);
-      const int p0 = p_thr | perm(ei);
+      if (i * kSwapBlock * V >= n || !active) continue;
       cvec<R, V> o1, o2;
 #pragma unroll
       for (int k = 0; k < V; ++k) {
-        const int p = k ? (p0 | p_one) : p0;
+        const int p = k ? (p_it[i] | p_one) : p_it[i];
         o1.v[k] = a[lds_swz<R>(p)];
         o2.v[k] = b[lds_swz<R>(p)];
       }
-      stv<R, V>(st + base2 + off, o1);
-      if (pair) stv<R, V>(st + base1 + off, o2);
+      stv<R, V>(st + (base2 | off_it[i]), o1);
+      if (pair) stv<R, V>(st + (base1 | off_it[i]), o2);
     }
     __syncthreads();
   }
@@ -779,45 +785,41 @@ static int bit_swaps_impl(void* state, int nqubits, const int32_t* pairs, int np
   ts.TB = tb;
   ts.H = tb - L;
   ts.nout = nqubits - tb;
-  Ins tins;
-  std::memset(&tins, 0, sizeof(tins));
   int local_of[64], gpos[16];
   int nl = 0;
   for (int b = 0; b < nqubits; ++b)
     if ((in_tile >> b) & 1) {
       if (b >= L) ts.hb[nl - L] = b;
-      tins.pos[nl] = b;
       gpos[nl] = b;
       local_of[b] = nl++;
     }
-  tins.npos = tb;
   for (int i = 0; i < tb; ++i) sp.pl[i] = (int8_t)local_of[partner[gpos[i]]];
-  int rank_of[64], outside[64];
+  int outside[64];
   int no = 0;
   for (int b = 0; b < nqubits; ++b)
-    if (!((in_tile >> b) & 1)) {
-      outside[no] = b;
-      rank_of[b] = no++;
-    }
-  for (int r = 0; r < no; ++r) sp.po[r] = (int8_t)rank_of[partner[outside[r]]];
-  // work order: natural (tile index = work index), or paired (each bit
-  // next to its partner's bit; QJ_SWAP_ORDER=paired)
-  const char* order = std::getenv("QJ_SWAP_ORDER");
-  const bool paired = order && std::strcmp(order, "paired") == 0;
-  {
-    int k = 0;
-    u64 placed = 0;
-    for (int r = 0; r < no; ++r) {
-      if ((placed >> r) & 1) continue;
-      sp.tord[k++] = (int8_t)r;
-      placed |= u64(1) << r;
-      const int q = sp.po[r];
-      if (paired && !((placed >> q) & 1)) {
-        sp.tord[k++] = (int8_t)q;
-        placed |= u64(1) << q;
+    if (!((in_tile >> b) & 1)) outside[no++] = b;
+  // byte tables: state offset of tile t (and of its partner) = OR over the
+  // bytes of t of one entry each
+  const int nbytes = (no + 7) / 8;
+  std::vector<u64> tabs((size_t)std::max(1, 2 * nbytes) * 256, 0);
+  for (int k = 0; k < nbytes; ++k)
+    for (int v = 0; v < 256; ++v) {
+      u64 o1 = 0, o2 = 0;
+      for (int j = 0; j < 8; ++j) {
+        const int r = 8 * k + j;
+        if (r < no && ((v >> j) & 1)) {
+          o1 |= u64(1) << outside[r];
+          o2 |= u64(1) << partner[outside[r]];
+        }
       }
+      tabs[(size_t)(2 * k) * 256 + v] = o1;
+      tabs[(size_t)(2 * k + 1) * 256 + v] = o2;
     }
-  }
+  sp.nbytes = nbytes;
+  void* dtabs = nullptr;
+  int ticket = -1;
+  int rc = ring_upload(tabs.data(), tabs.size() * sizeof(u64), s, &dtabs, &ticket);
+  if (rc) return rc;
   const u64 ntiles = u64(1) << ts.nout;
   const size_t lds = 2 * ((size_t)1 << tb) * sizeof(cplx<R>);
   int cus = 0;
@@ -825,8 +827,11 @@ static int bit_swaps_impl(void* state, int nqubits, const int32_t* pairs, int np
   device_limits(&cus, &max_lds);
   const u64 per_cu = std::max<u64>(1, std::min<u64>(kLdsPerCu / lds, 2048 / kSwapBlock));
   const u64 grid = std::min<u64>(ntiles, (u64)cus * per_cu);
-  k_bit_swaps<R, V><<<dim3((unsigned)grid), dim3(kSwapBlock), lds, s>>>(static_cast<cplx<R>*>(state), ntiles, tins, sp);
-  return check_launch("apply_bit_swaps launch");
+  k_bit_swaps<R, V><<<dim3((unsigned)grid), dim3(kSwapBlock), lds, s>>>(static_cast<cplx<R>*>(state), ntiles, sp,
+                                                                       static_cast<const u64*>(dtabs));
+  rc = check_launch("apply_bit_swaps launch");
+  ring_release(ticket, s);
+  return rc;
 }
 
 }  // namespace qj

@@ -25,6 +25,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -49,13 +50,17 @@ struct RegSeg {
   int8_t slot[kRegMaxBits];      // slot -> physical tile bit
 };
 
+constexpr int kRegDense = 0;      // complex matrix
+constexpr int kRegPhase = 1;
+constexpr int kRegDenseReal = 2;  // 2x2 matrix with zero imaginary parts
+
 struct RegOp {
-  int32_t kind;     // QJ_TILE_DENSE or QJ_TILE_PHASE
-  int32_t mask;     // dense: register-index bits of the targets;
-                    // phase: register-index bits of the local controls
-  uint32_t lctrl;   // dense: tile-local control mask (physical bits);
-                    // phase: its thread-bit part
+  int32_t kind;     // kRegDense, kRegPhase or kRegDenseReal
+  int32_t mask;     // dense: register-index bits of the targets
+  uint32_t lct;     // local controls on thread bits (mask of tile bits)
+  int32_t rcm;      // local controls on register bits (register-index bits)
   int32_t fidx;     // phase: index of its out-of-tile factor
+  int32_t pad;
   u64 gctrl;        // out-of-tile control mask (state bit positions)
   u64 loff;         // LDS-part offset: dense matrix, or phase tables
 };
@@ -89,6 +94,17 @@ struct RegLay {
   int ethr;       // tile bits set by the thread index
   int rb[NRB];    // tile bit of register-index bit j
 };
+
+// LDS slot of the element in register r: the swizzle is linear over GF(2),
+// so it is the thread part's swizzle XOR the (wave-uniform) register part's.
+template <typename R, int NRB>
+__host__ __device__ __forceinline__ int reg_lds(int sw_thr, const RegLay<NRB>& L, int r) {
+  int e = 0;
+#pragma unroll
+  for (int j = 0; j < NRB; ++j)
+    if ((r >> j) & 1) e |= L.rb[j];
+  return sw_thr ^ lds_swz<R>(e);
+}
 
 template <int NRB>
 __host__ __device__ __forceinline__ RegLay<NRB> reg_layout(const RegSeg& sg, int NT, int tid) {
@@ -134,38 +150,98 @@ __host__ __device__ __forceinline__ u64 reg_state_offset(const u64 (&rb)[NRB], i
 }
 
 // ------------------------------------------------------------------- ops --
+__host__ __device__ __forceinline__ float qfma(float a, float b, float c) { return fmaf(a, b, c); }
+__host__ __device__ __forceinline__ double qfma(double a, double b, double c) { return fma(a, b, c); }
+
+// acc + m * x as four fused multiply-adds.
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> cfma(cplx<R> m, cplx<R> x, cplx<R> acc) {
+  acc.re = qfma(m.re, x.re, acc.re);
+  acc.re = qfma(-m.im, x.im, acc.re);
+  acc.im = qfma(m.re, x.im, acc.im);
+  acc.im = qfma(m.im, x.re, acc.im);
+  return acc;
+}
+
 // Dense op on the register bits MASK: row index of register g | pdep(c, MASK)
-// is c (the lowest mask bit is the matrix index LSB).
-template <typename R, int NRB, int MASK>
+// is c (the lowest mask bit is the matrix index LSB).  Groups whose register
+// bits fail the controls `rcm` are skipped; the caller masks the threads
+// whose thread bits fail theirs.  REAL: a 2x2 matrix with real entries.
+template <typename R, int NRB, int MASK, bool REAL>
 __host__ __device__ __forceinline__ void reg_dense(cplx<R> (&x)[1 << NRB], const cplx<R>* __restrict__ m,
-                                                   const RegLay<NRB>& L, uint32_t lc) {
+                                                   int rcm) {
   constexpr int NR = 1 << NRB;
   constexpr int N = 1 << popc_c(MASK);
+  if constexpr (N == 2) {
+    // 2x2: partial sums first, then each output as an FMA that is the last
+    // use of its own input, so the result can stay in that input's register
+    const cplx<R> m00 = m[0], m01 = m[1], m10 = m[2], m11 = m[3];
 #pragma unroll
-  for (int g = 0; g < NR; ++g) {
-    if (g & MASK) continue;
-    if (((uint32_t)reg_elem<NRB>(L, g) & lc) != lc) continue;
-    cplx<R> in[N];
+    for (int g = 0; g < NR; ++g) {
+      if (g & MASK) continue;
+      if ((g & rcm) != rcm) continue;
+      cplx<R>& a = x[g];
+      cplx<R>& b = x[g | MASK];
+      if (REAL) {
+        const R pa_re = m01.re * b.re, pa_im = m01.re * b.im;
+        const R pb_re = m10.re * a.re, pb_im = m10.re * a.im;
+        a.re = qfma(m00.re, a.re, pa_re);
+        a.im = qfma(m00.re, a.im, pa_im);
+        b.re = qfma(m11.re, b.re, pb_re);
+        b.im = qfma(m11.re, b.im, pb_im);
+      } else {
+        R pa_re = qfma(m01.re, b.re, -m00.im * a.im);
+        pa_re = qfma(-m01.im, b.im, pa_re);
+        R pa_im = qfma(m01.re, b.im, m00.im * a.re);
+        pa_im = qfma(m01.im, b.re, pa_im);
+        R pb_re = qfma(m10.re, a.re, -m11.im * b.im);
+        pb_re = qfma(-m10.im, a.im, pb_re);
+        R pb_im = qfma(m10.re, a.im, m11.im * b.re);
+        pb_im = qfma(m10.im, a.re, pb_im);
+        a.re = qfma(m00.re, a.re, pa_re);
+        a.im = qfma(m00.re, a.im, pa_im);
+        b.re = qfma(m11.re, b.re, pb_re);
+        b.im = qfma(m11.re, b.im, pb_im);
+      }
+    }
+  } else {
 #pragma unroll
-    for (int c = 0; c < N; ++c) in[c] = x[g | pdep_c(c, MASK)];
+    for (int g = 0; g < NR; ++g) {
+      if (g & MASK) continue;
+      if ((g & rcm) != rcm) continue;
+      cplx<R> in[N];
 #pragma unroll
-    for (int r = 0; r < N; ++r) {
-      cplx<R> acc = cmul(m[r * N], in[0]);
+      for (int c = 0; c < N; ++c) in[c] = x[g | pdep_c(c, MASK)];
 #pragma unroll
-      for (int c = 1; c < N; ++c) acc = cadd(acc, cmul(m[r * N + c], in[c]));
-      x[g | pdep_c(r, MASK)] = acc;
+      for (int r = 0; r < N; ++r) {
+        cplx<R> acc = cmul(m[r * N], in[0]);
+#pragma unroll
+        for (int c = 1; c < N; ++c) acc = cfma(m[r * N + c], in[c], acc);
+        x[g | pdep_c(r, MASK)] = acc;
+      }
     }
   }
 }
 
 template <typename R, int NRB>
-__host__ __device__ __forceinline__ void reg_dense_any(cplx<R> (&x)[1 << NRB], int mask,
-                                                       const cplx<R>* __restrict__ m, const RegLay<NRB>& L,
-                                                       uint32_t lc) {
+__host__ __device__ __forceinline__ void reg_dense_any(cplx<R> (&x)[1 << NRB], int kind, int mask,
+                                                       const cplx<R>* __restrict__ m, int rcm) {
+  if (kind == kRegDenseReal) {
+    switch (mask) {
+#define QJ_REG_REAL(M)                                                         \
+  case M:                                                                      \
+    if constexpr (M < (1 << NRB)) reg_dense<R, NRB, M, true>(x, m, rcm);       \
+    break;
+      QJ_REG_REAL(1) QJ_REG_REAL(2) QJ_REG_REAL(4) QJ_REG_REAL(8) QJ_REG_REAL(16)
+#undef QJ_REG_REAL
+      default: break;
+    }
+    return;
+  }
   switch (mask) {
-#define QJ_REG_CASE(M)                                          \
-  case M:                                                       \
-    if constexpr (M < (1 << NRB)) reg_dense<R, NRB, M>(x, m, L, lc); \
+#define QJ_REG_CASE(M)                                                         \
+  case M:                                                                      \
+    if constexpr (M < (1 << NRB)) reg_dense<R, NRB, M, false>(x, m, rcm);      \
     break;
     QJ_REG_CASE(1) QJ_REG_CASE(2) QJ_REG_CASE(3) QJ_REG_CASE(4) QJ_REG_CASE(5) QJ_REG_CASE(6)
     QJ_REG_CASE(7) QJ_REG_CASE(8) QJ_REG_CASE(9) QJ_REG_CASE(10) QJ_REG_CASE(11) QJ_REG_CASE(12)
@@ -181,25 +257,24 @@ __host__ __device__ __forceinline__ void reg_dense_any(cplx<R> (&x)[1 << NRB], i
 // Phase op in a layout: the factor of element (thread tid, register r) is
 // f * TL[tid mod 64] * TW[tid / 64] * TR[r] -- the products of the per-bit
 // factors over the set lane, wave and register slots (tables built by the
-// planner) -- where the local controls hold: the thread-bit controls `lct`
-// (a mask of tile bits, tested on the thread's part `ethr`) and the
-// register-bit controls `rcm` (register-index bits; other registers are
-// skipped).
+// planner).  Registers whose bits fail the controls `rcm` are skipped; the
+// caller masks the threads whose thread bits fail theirs.
 template <typename R, int NRB>
 __host__ __device__ __forceinline__ void reg_phase(cplx<R> (&x)[1 << NRB], cplx<R> f,
-                                                   const cplx<R>* __restrict__ tab, int NT, int tid, int ethr,
-                                                   uint32_t lct, int rcm) {
+                                                   const cplx<R>* __restrict__ tab, int NT, int tid, int rcm) {
   constexpr int NR = 1 << NRB;
   const cplx<R>* tl = tab;
   const cplx<R>* tw = tl + 64;
   const cplx<R>* tr = tw + (1 << (NT - 6));
   const cplx<R> g = cmul(cmul(f, tl[tid & 63]), tw[tid >> 6]);
-  const bool on = ((uint32_t)ethr & lct) == lct;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     if ((r & rcm) != rcm) continue;
-    const cplx<R> y = cmul(x[r], cmul(g, tr[r]));
-    if (on) x[r] = y;
+    const cplx<R> h = cmul(g, tr[r]);
+    // x *= h with the final FMAs as the last uses of x.re and x.im
+    const R t_re = x[r].im * h.im, t_im = x[r].re * h.im;
+    x[r].re = qfma(x[r].re, h.re, -t_re);
+    x[r].im = qfma(x[r].im, h.re, t_im);
   }
 }
 
@@ -366,10 +441,15 @@ inline bool reg_plan(const RegPlanIn& in, std::vector<char>& blob) {
       const qj_tile_op& op = in.ops[p.op];
       RegOp& ro = rops[i];
       std::memset(&ro, 0, sizeof(ro));
-      ro.kind = op.kind;
-      ro.lctrl = p.lc;
+      ro.kind = op.kind == QJ_TILE_PHASE ? kRegPhase : kRegDense;
       ro.gctrl = op.global_ctrl;
       ro.fidx = -1;
+      // local controls: thread-bit part (tile bits) and register-index part
+      uint32_t thr_bits = 0;
+      for (int q = 0; q < NT; ++q) thr_bits |= uint32_t(1) << sg.slot[q];
+      ro.lct = p.lc & thr_bits;
+      for (int j = 0; j < NRB; ++j)
+        if ((p.lc >> sg.slot[NT + j]) & 1) ro.rcm |= 1 << j;
       if (op.kind == QJ_TILE_DENSE) {
         const int k = p.k, N = 1 << k;
         int j[QJ_TILE_MAX_K];
@@ -392,10 +472,16 @@ inline bool reg_plan(const RegPlanIn& in, std::vector<char>& blob) {
         }
         std::vector<char> mat((size_t)N * N * in.esize);
         const char* src = in.data + op.offset;
+        bool real = k == 1;
         for (int r = 0; r < N; ++r)
-          for (int c = 0; c < N; ++c)
-            std::memcpy(mat.data() + ((size_t)r * N + c) * in.esize,
-                        src + ((size_t)perm[r] * N + perm[c]) * in.esize, in.esize);
+          for (int c = 0; c < N; ++c) {
+            char* dst = mat.data() + ((size_t)r * N + c) * in.esize;
+            std::memcpy(dst, src + ((size_t)perm[r] * N + perm[c]) * in.esize, in.esize);
+            const bool zero_im = in.esize == 8 ? *reinterpret_cast<const float*>(dst + 4) == 0.0f
+                                               : *reinterpret_cast<const double*>(dst + 8) == 0.0;
+            real = real && zero_im;
+          }
+        if (real) ro.kind = kRegDenseReal;
         ro.loff = put(ldata, mat.data(), mat.size());
       } else {
         // ABI block: int32 {ga, gb, tb, pad}, OA[2^ga], OB[2^gb], W[tb] (logical bits)
@@ -447,12 +533,6 @@ inline bool reg_plan(const RegPlanIn& in, std::vector<char>& blob) {
         table(0, kRegLanes);
         table(kRegLanes, nw);
         table(NT, NRB);
-        // controls: thread-bit part as a tile-bit mask, register part as register-index bits
-        uint32_t thr_bits = 0;
-        for (int q = 0; q < NT; ++q) thr_bits |= uint32_t(1) << sg.slot[q];
-        ro.lctrl = p.lc & thr_bits;
-        for (int j = 0; j < NRB; ++j)
-          if ((p.lc >> sg.slot[NT + j]) & 1) ro.mask |= 1 << j;
         ro.loff = put(ldata, tab.data(), tab.size());
         ro.fidx = (int)ftab.size();
         ftab.push_back(put(gtab, src, tabs));

@@ -84,7 +84,8 @@ static Case<R> make_case(std::mt19937_64& rng, int nq, int L, int nhigh, int nrb
       if (rng() % 3 == 0) op.local_ctrl = u64(1) << bits[op.k];
       if (nout && rng() % 3 == 0) op.global_ctrl = u64(1) << outside[rng() % nout];
       std::vector<cplx<R>> m((size_t)1 << (2 * op.k));
-      for (auto& v : m) v = rc();
+      const bool real = op.k == 1 && rng() % 2;  // real 2x2 matrices take their own kernel path
+      for (auto& v : m) v = real ? mk((R)U(rng), (R)0) : rc();
       op.offset = put<R>(c.data, m.data(), m.size() * sizeof(cplx<R>));
     } else {
       op.kind = QJ_TILE_PHASE;
@@ -205,7 +206,8 @@ static bool run_kernel(const Case<R>& c, const std::vector<char>& blob, std::vec
         std::fill(hits.begin(), hits.end(), 0);
         for (int tid = 0; tid < nthr; ++tid)
           for (int r = 0; r < NR; ++r) {
-            const int a = lds_swz<R>(reg_elem<NRB>(cur[tid], r));
+            const int a = reg_lds<R, NRB>(lds_swz<R>(cur[tid].ethr), cur[tid], r);
+            if (a != lds_swz<R>(reg_elem<NRB>(cur[tid], r))) return false;  // linearity of the swizzle
             lds[a] = x[tid][r];
             ++hits[a];
           }
@@ -213,7 +215,7 @@ static bool run_kernel(const Case<R>& c, const std::vector<char>& blob, std::vec
           if (v != 1) return false;  // a re-layout must write every LDS slot once
         for (int tid = 0; tid < nthr; ++tid) {
           cur[tid] = reg_layout<NRB>(sg, NT, tid);
-          for (int r = 0; r < NR; ++r) x[tid][r] = lds[lds_swz<R>(reg_elem<NRB>(cur[tid], r))];
+          for (int r = 0; r < NR; ++r) x[tid][r] = lds[reg_lds<R, NRB>(lds_swz<R>(cur[tid].ethr), cur[tid], r)];
         }
       }
       for (int o = sg.o0; o < sg.o1; ++o) {
@@ -221,11 +223,12 @@ static bool run_kernel(const Case<R>& c, const std::vector<char>& blob, std::vec
         if ((base & op.gctrl) != op.gctrl) continue;
         const cplx<R>* d = reinterpret_cast<const cplx<R>*>(ldata + op.loff);
         for (int tid = 0; tid < nthr; ++tid) {
+          if (((uint32_t)cur[tid].ethr & op.lct) != op.lct) continue;
           cplx<R>(&xr)[NR] = *reinterpret_cast<cplx<R>(*)[NR]>(x[tid].data());
-          if (op.kind == QJ_TILE_PHASE)
-            reg_phase<R, NRB>(xr, f[op.fidx], d, NT, tid, cur[tid].ethr, op.lctrl, op.mask);
+          if (op.kind == kRegPhase)
+            reg_phase<R, NRB>(xr, f[op.fidx], d, NT, tid, op.rcm);
           else
-            reg_dense_any<R, NRB>(xr, op.mask, d, cur[tid], op.lctrl);
+            reg_dense_any<R, NRB>(xr, op.kind, op.mask, d, op.rcm);
         }
       }
     }
