@@ -326,6 +326,27 @@ int qj_measure_frequencies(int64_t* frequencies, const void* probs,
                            const uint32_t* chain_seeds, int nchains,
                            void* stream);
 
+/* Shot sampler below the Metropolis threshold: NumPy's
+ * RandomState.choice(nstates, nshots, p=probs), which qibo's sample_shots
+ * runs on the host and CupyBackend.sample_shots (gpu.py:770-773) copies the
+ * probabilities back for.  cdf = cumsum(probs) in FP64, cdf /= cdf[-1],
+ * samples[j] = first index i with cdf[i] > uniforms[j] (searchsorted,
+ * side='right').  `probs` (QJ_FLOAT32/QJ_FLOAT64), `uniforms` (double,
+ * nshots values in [0, 1), drawn by the caller with the generator choice()
+ * would use) and `samples` (int64, nshots) are device arrays.  Up to 2^16
+ * states the cumsum is sequential in index order, as NumPy's; above, a
+ * blocked FP64 scan. */
+int qj_sample_shots(const void* probs, int prob_dtype, int64_t nstates,
+                    const double* uniforms, int64_t nshots, int64_t* samples,
+                    void* stream);
+/* Counts of samples: frequencies[v] += 1 for each samples[j] = v in
+ * [0, nstates) (device int64 arrays); samples outside that range add to
+ * *out_of_range (device int64) instead.  The device side of
+ * calculate_frequencies (gpu.py:744-748 counts on the host with np.unique). */
+int qj_count_samples(const int64_t* samples, int64_t nshots,
+                     int64_t* frequencies, int64_t nstates,
+                     int64_t* out_of_range, void* stream);
+
 /* ---- reductions (FP64 accumulation; synchronise `stream`) ---- */
 
 /* Compare two states of n amplitudes, each complex64 or complex128 (FP64

@@ -566,10 +566,67 @@ class HipBackend(Backend):
         del keep
         return probs
 
+    def sample_shots(self, probabilities, nshots):
+        """Shots drawn on the device (the reference copies the probabilities
+        to the host and samples there, gpu.py:770-773): the result of
+        ``RandomState.choice(n, nshots, p=probabilities)`` for the backend's
+        generator, whose uniforms are drawn here on the host so the stream
+        stays the one ``set_seed`` started.  Returns an int64 device tensor."""
+        nshots = int(nshots)
+        probs = self._as_real_probs(probabilities)
+        nstates = probs.numel()
+        samples = torch.empty(nshots, dtype=torch.int64, device=self._torch_device)
+        if nshots == 0:
+            return samples
+        rng = getattr(self, "_rng", np.random)  # qibo's backends draw from np.random
+        uniforms = torch.from_numpy(rng.random_sample(nshots)).to(self._torch_device)
+        call("qj_sample_shots", _ptr(probs), _NATIVE_CODES[probs.dtype], nstates, _ptr(uniforms), nshots,
+             _ptr(samples), self._stream())
+        return samples
+
+    def calculate_frequencies(self, samples):
+        """Counter of sample values (gpu.py:744-748 counts on the host with
+        np.unique).  Device int64 samples are counted on the device when the
+        value range is small enough for a dense count array."""
+        if isinstance(samples, torch.Tensor) and samples.is_cuda and samples.dtype == torch.int64 \
+                and samples.numel() > 0:
+            nstates = int(samples.max().item()) + 1
+            if int(samples.min().item()) >= 0 and nstates <= (1 << 24):
+                counts = self._count_samples(samples, nstates)
+                return Counter({i: int(c) for i, c in enumerate(counts) if c > 0})
+        samples = np.asarray(self.to_numpy(samples))
+        res, counts = np.unique(samples, return_counts=True)
+        return Counter({int(k): int(v) for k, v in zip(res, counts)})
+
+    def _count_samples(self, samples, nstates):
+        freqs = torch.zeros(nstates + 1, dtype=torch.int64, device=self._torch_device)
+        bad = freqs[nstates:]
+        call("qj_count_samples", _ptr(samples), samples.numel(), _ptr(freqs), nstates, _ptr(bad), self._stream())
+        counts = self.to_numpy(freqs)
+        if counts[nstates]:
+            raise ValueError(f"{int(counts[nstates])} samples outside [0, {nstates}).")
+        return counts[:nstates]
+
+    def _as_real_probs(self, probabilities):
+        """Probabilities as a contiguous float32/float64 device tensor."""
+        probs = probabilities
+        if not isinstance(probs, torch.Tensor):
+            probs = np.asarray(probs)
+            if probs.dtype not in (np.float32, np.float64):
+                probs = probs.astype(np.float64)
+        probs = self.cast(probs, dtype=_dtype_name(probs.dtype)).reshape(-1).contiguous()
+        if probs.dtype not in (torch.float32, torch.float64):
+            probs = probs.to(torch.float64)
+        return probs
+
     def sample_frequencies(self, probabilities, nshots):
-        """cpu.py:383-394: Metropolis chains from SHOT_METROPOLIS_THRESHOLD shots on."""
+        """cpu.py:383-394: Metropolis chains from SHOT_METROPOLIS_THRESHOLD shots on;
+        below, shots drawn on the device (sample_shots) and counted there."""
         if nshots < SHOT_METROPOLIS_THRESHOLD:
-            return super().sample_frequencies(probabilities, nshots)
+            probs = self._as_real_probs(probabilities)
+            samples = self.sample_shots(probs, nshots)
+            counts = self._count_samples(samples, probs.numel())
+            return Counter({i: int(c) for i, c in enumerate(counts) if c > 0})
         seed = self.random_integers(0, int(1e8), dtype=np.int64)[0]
         nqubits = int(math.log2(tuple(probabilities.shape)[0]))
         frequencies = torch.zeros(2**nqubits, dtype=torch.int64, device=self._torch_device)

@@ -680,6 +680,151 @@ static int metropolis_impl(int64_t* freqs, const T* probs, int64_t nstates, int6
   return rc;
 }
 
+// ------------------------------------------------- shot sampling (CDF) --
+// NumPy's RandomState.choice(n, nshots, p=probs): cdf = cumsum(p) in FP64,
+// cdf /= cdf[-1], sample = searchsorted(cdf, u, side='right') for uniforms u
+// drawn by the caller.  Up to kSeqCdfStates the cumsum runs in one thread in
+// index order (the sum NumPy computes); above, a blocked scan (per-thread runs
+// of kCdfRun, block offsets from a sequential pass over the block sums).
+constexpr int64_t kSeqCdfStates = int64_t(1) << 16;
+constexpr int kCdfRun = 64;
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_cdf_seq(const T* __restrict__ p, int64_t n, double* __restrict__ cdf) {
+  if (threadIdx.x != 0) return;
+  double c = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    c += (double)p[i];
+    cdf[i] = c;
+  }
+}
+
+// Pass 1: sum of each block's chunk (kBlock threads x kCdfRun elements).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cdf_block_sums(const T* __restrict__ p, int64_t n,
+                                                           double* __restrict__ bsum) {
+  __shared__ double part[kBlock];
+  const int64_t base = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * kCdfRun;
+  double c = 0.0;
+  for (int k = 0; k < kCdfRun; ++k)
+    if (base + k < n) c += (double)p[base + k];
+  part[threadIdx.x] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kBlock; ++i) t += part[i];
+    bsum[blockIdx.x] = t;
+  }
+}
+
+// Pass 2 (one thread): exclusive prefix of the block sums, in place.
+__global__ void __launch_bounds__(64) k_cdf_block_offsets(double* __restrict__ bsum, int64_t nblocks) {
+  if (threadIdx.x != 0) return;
+  double c = 0.0;
+  for (int64_t b = 0; b < nblocks; ++b) {
+    const double v = bsum[b];
+    bsum[b] = c;
+    c += v;
+  }
+}
+
+// Pass 3: each thread writes the running sums of its run, offset by the
+// block's prefix and the sums of the block's earlier threads.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cdf_block_scan(const T* __restrict__ p, int64_t n,
+                                                           const double* __restrict__ boff,
+                                                           double* __restrict__ cdf) {
+  __shared__ double part[kBlock];
+  const int64_t base = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * kCdfRun;
+  double c = 0.0;
+  for (int k = 0; k < kCdfRun; ++k)
+    if (base + k < n) c += (double)p[base + k];
+  part[threadIdx.x] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = boff[blockIdx.x];
+    for (int i = 0; i < kBlock; ++i) {
+      const double v = part[i];
+      part[i] = t;
+      t += v;
+    }
+  }
+  __syncthreads();
+  c = part[threadIdx.x];
+  for (int k = 0; k < kCdfRun; ++k)
+    if (base + k < n) {
+      c += (double)p[base + k];
+      cdf[base + k] = c;
+    }
+}
+
+// cdf /= cdf[n-1], then each uniform's first index with cdf > u.
+__global__ void __launch_bounds__(kBlock) k_cdf_normalize(double* __restrict__ cdf, int64_t n) {
+  const double total = cdf[n - 1];
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    cdf[i] = cdf[i] / total;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cdf_search(const double* __restrict__ cdf, int64_t n,
+                                                       const double* __restrict__ u, int64_t nshots,
+                                                       int64_t* __restrict__ samples) {
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < nshots; j += (int64_t)gridDim.x * kBlock) {
+    const double x = u[j];
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int64_t mid = lo + (hi - lo) / 2;
+      if (cdf[mid] > x)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    // cdf[n-1] == 1 > u for u in [0, 1): lo < n; kept in range regardless
+    samples[j] = lo < n ? lo : n - 1;
+  }
+}
+
+// frequencies[s] += 1 for every sample s in [0, nstates); others are counted
+// in *bad (device int64) and left out.
+__global__ void __launch_bounds__(kBlock) k_count_samples(const int64_t* __restrict__ samples, int64_t nshots,
+                                                          unsigned long long* __restrict__ freqs, int64_t nstates,
+                                                          unsigned long long* __restrict__ bad) {
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < nshots; j += (int64_t)gridDim.x * kBlock) {
+    const int64_t v = samples[j];
+    if (v >= 0 && v < nstates)
+      atomicAdd(freqs + v, 1ull);
+    else
+      atomicAdd(bad, 1ull);
+  }
+}
+
+template <typename T>
+static int sample_shots_impl(const T* probs, int64_t nstates, const double* u, int64_t nshots, int64_t* samples,
+                             hipStream_t s) {
+  if (nstates < 1) return set_error(QJ_EVALUE, "sample_shots: nstates must be >= 1");
+  if (nshots < 0) return set_error(QJ_EVALUE, "sample_shots: nshots must be >= 0");
+  if (nshots == 0) return QJ_OK;
+  DeviceScratch* sc = nullptr;
+  int rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  const int64_t per_block = (int64_t)kBlock * kCdfRun;
+  const int64_t nblocks = (nstates + per_block - 1) / per_block;
+  const size_t need = (size_t)(nstates + nblocks) * sizeof(double);
+  rc = scratch_reserve_big(sc, need, s);
+  if (rc) return rc;
+  double* cdf = static_cast<double*>(sc->big);
+  double* bsum = cdf + nstates;
+  if (nstates <= kSeqCdfStates) {
+    k_cdf_seq<T><<<dim3(1), dim3(64), 0, s>>>(probs, nstates, cdf);
+  } else {
+    k_cdf_block_sums<T><<<dim3((unsigned)nblocks), dim3(kBlock), 0, s>>>(probs, nstates, bsum);
+    k_cdf_block_offsets<<<dim3(1), dim3(64), 0, s>>>(bsum, nblocks);
+    k_cdf_block_scan<T><<<dim3((unsigned)nblocks), dim3(kBlock), 0, s>>>(probs, nstates, bsum, cdf);
+  }
+  k_cdf_normalize<<<dim3((unsigned)grid_for(nstates, kBlock)), dim3(kBlock), 0, s>>>(cdf, nstates);
+  k_cdf_search<<<dim3((unsigned)grid_for(nshots, kBlock)), dim3(kBlock), 0, s>>>(cdf, nstates, u, nshots, samples);
+  return check_launch("sample_shots launch");
+}
+
 }  // namespace qj
 
 using namespace qj;
@@ -751,6 +896,30 @@ int qj_measure_frequencies(int64_t* frequencies, const void* probs, int prob_dty
   if (prob_dtype == QJ_FLOAT64)
     return metropolis_impl<double>(frequencies, (const double*)probs, nstates, nshots, chain_seeds, nchains, s);
   return set_error(QJ_EVALUE, "measure_frequencies: probs must be float32/float64");
+}
+
+int qj_sample_shots(const void* probs, int prob_dtype, int64_t nstates, const double* uniforms, int64_t nshots,
+                    int64_t* samples, void* stream) {
+  if (!probs || (nshots > 0 && (!uniforms || !samples))) return set_error(QJ_EVALUE, "sample_shots: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (prob_dtype == QJ_FLOAT32)
+    return sample_shots_impl<float>((const float*)probs, nstates, uniforms, nshots, samples, s);
+  if (prob_dtype == QJ_FLOAT64)
+    return sample_shots_impl<double>((const double*)probs, nstates, uniforms, nshots, samples, s);
+  return set_error(QJ_EVALUE, "sample_shots: probs must be float32/float64");
+}
+
+int qj_count_samples(const int64_t* samples, int64_t nshots, int64_t* frequencies, int64_t nstates,
+                     int64_t* out_of_range, void* stream) {
+  if (nshots < 0 || nstates < 1) return set_error(QJ_EVALUE, "count_samples: bad sizes");
+  if (!frequencies || !out_of_range || (nshots > 0 && !samples))
+    return set_error(QJ_EVALUE, "count_samples: null pointer");
+  if (nshots == 0) return QJ_OK;
+  hipStream_t s = (hipStream_t)stream;
+  k_count_samples<<<dim3((unsigned)grid_for(nshots, kBlock)), dim3(kBlock), 0, s>>>(
+      samples, nshots, reinterpret_cast<unsigned long long*>(frequencies), nstates,
+      reinterpret_cast<unsigned long long*>(out_of_range));
+  return check_launch("count_samples launch");
 }
 
 int qj_norm_squared(const void* state, int dtype, uint64_t n, double* out, void* stream) {

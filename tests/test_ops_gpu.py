@@ -204,3 +204,77 @@ def test_density_matrix_probabilities_and_collapse(backend, dtype, qubits, shot)
                                                       density_matrix=True))
         want = er.collapse_density_matrix(rho, qubits, results, normalize=normalize)
         np.testing.assert_allclose(got, want, atol=tol)
+
+
+@pytest.mark.parametrize("nstates", [1, 2, 8, 1000, 2**16, 2**17 + 3])
+@pytest.mark.parametrize("realtype", [np.float64, np.float32])
+def test_sample_shots_match_numpy_choice(backend, nstates, realtype):
+    """sample_shots == RandomState(seed).choice(n, nshots, p) for the same
+    seed (the generator the reference's host sampling uses): identical up
+    to 2^16 states (sequential FP64 cumsum, as NumPy's); above, the blocked
+    scan may round a CDF entry differently, so a sample differs only if a
+    uniform falls within that rounding of a boundary (none expected)."""
+    rng = np.random.default_rng(nstates)
+    p = rng.random(nstates) ** 3
+    p[rng.integers(nstates, size=max(1, nstates // 5))] = 0
+    if p.sum() == 0:
+        p[0] = 1
+    p = (p / p.sum()).astype(realtype)
+    nshots = 5000
+    backend.set_seed(1234)
+    got = backend.to_numpy(backend.sample_shots(p, nshots))
+    want = np.random.RandomState(1234).choice(nstates, size=nshots, p=p.astype(np.float64))
+    assert got.dtype == np.int64 and got.shape == (nshots,)
+    np.testing.assert_array_equal(got, want)
+    assert np.all(p[got] > 0)
+    # the generator advanced exactly as choice() advances it
+    rs = np.random.RandomState(1234)
+    rs.choice(nstates, size=nshots, p=p.astype(np.float64))
+    assert backend._rng.randint(0, 2**31) == rs.randint(0, 2**31)
+
+
+def test_sample_shots_stream_continues_like_numpy(backend):
+    p = np.array([0.1, 0.2, 0.3, 0.4])
+    backend.set_seed(7)
+    a = backend.to_numpy(backend.sample_shots(p, 100))
+    b = backend.to_numpy(backend.sample_shots(p, 50))
+    rs = np.random.RandomState(7)
+    np.testing.assert_array_equal(a, rs.choice(4, size=100, p=p))
+    np.testing.assert_array_equal(b, rs.choice(4, size=50, p=p))
+    assert backend.to_numpy(backend.sample_shots(p, 0)).shape == (0,)
+
+
+def test_sample_frequencies_below_threshold_match_numpy(backend):
+    p = np.zeros(32)
+    p[[1, 5, 6, 17, 31]] = [0.1, 0.3, 0.2, 0.25, 0.15]
+    backend.set_seed(99)
+    got = backend.sample_frequencies(p, 20000)
+    samples = np.random.RandomState(99).choice(32, size=20000, p=p)
+    res, counts = np.unique(samples, return_counts=True)
+    assert dict(got) == {int(k): int(v) for k, v in zip(res, counts)}
+
+
+def test_calculate_frequencies_device_and_host_paths(backend):
+    rng = np.random.default_rng(3)
+    samples = rng.integers(0, 300, size=10000)
+    want = dict(zip(*np.unique(samples, return_counts=True)))
+    want = {int(k): int(v) for k, v in want.items()}
+    dev = backend.calculate_frequencies(backend.cast(samples, dtype="int64"))
+    host = backend.calculate_frequencies(samples)
+    assert dict(dev) == want and dict(host) == want
+
+
+def test_count_samples_rejects_out_of_range(backend):
+    import ctypes
+
+    from qibojit_amd import _native
+
+    samples = backend.cast(np.array([0, 3, 7, -1, 2, 9]), dtype="int64")
+    freqs = backend.cast(np.zeros(9, dtype=np.int64), dtype="int64")
+    _native.call("qj_count_samples", ctypes.c_void_p(samples.data_ptr()), 6, ctypes.c_void_p(freqs.data_ptr()), 8,
+                 ctypes.c_void_p(freqs.data_ptr() + 8 * 8), backend._stream())
+    out = backend.to_numpy(freqs)
+    np.testing.assert_array_equal(out[:8], [1, 0, 1, 1, 0, 0, 0, 1])
+    assert out[8] == 2  # -1 and 9
+    with pytest.raises(ValueError):
+        backend._count_samples(samples, 8)
