@@ -222,7 +222,11 @@ int qj_apply_bit_swaps(void* state, int dtype, int nqubits, const int32_t* pairs
  * 0 = no gradient) into partials[((bit * nrot + j) * nrows + b) * ncols + c]
  * (c < ncols, the workgroups per state) in FP64, then applies the step's
  * U^+ to both.  For exp(-i theta G / 2) and lam = U_{>k}^+ H psi_L this is
- * d<psi_L|H|psi_L>/d theta. */
+ * d<psi_L|H|psi_L>/d theta.  A nonzero gens entry also declares the step's
+ * matrix to be exp(-i theta G / 2) (any theta, per state), and the pass
+ * applies it in that form: Y real, Z diagonal, X real on the diagonal and
+ * imaginary off it; the matrix entries outside that form are not read.
+ * gens may be given without lam (forward passes use the forms too). */
 #define QJ_ROT_MAX_BITS 5
 #define QJ_ROT_MAX_ADJ_BITS 5 /* with lam (psi and lam in registers): 5 for complex64, 4 for complex128 */
 #define QJ_ROT_MAX_STEPS 3

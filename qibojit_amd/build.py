@@ -34,10 +34,12 @@ CFLAGS = [
 ]
 
 
-# Per-source flags.  qj_tile.hip: branches on wave-uniform values (the op
-# dispatch of the register tile kernel) stay unstructured, which keeps the
-# register-resident tile from being copied around every dispatch case.
-EXTRA = {"qj_tile.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]}
+# Per-source flags.  Branches on wave-uniform values (the op dispatch of the
+# register tile kernel, the per-step rotation forms of the rotation passes)
+# stay unstructured, which keeps register-resident amplitudes from being
+# copied around every branch.
+_UNIFORM_CFG = ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
+EXTRA = {"qj_tile.hip": _UNIFORM_CFG, "qj_group.hip": _UNIFORM_CFG}
 
 
 def _run(cmd):

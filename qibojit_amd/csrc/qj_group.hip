@@ -66,6 +66,9 @@ __device__ __forceinline__ cplx<R> cconj(cplx<R> a) {
   return mk(a.re, -a.im);
 }
 
+__device__ __forceinline__ float rfma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double rfma(double a, double b, double c) { return fma(a, b, c); }
+
 // Applies the 2x2 matrix m (row-major; its conjugate transpose if DAG) to
 // local bit A of the 2^K registers x.
 template <typename R, int K, int A, bool DAG>
@@ -83,7 +86,48 @@ __device__ __forceinline__ void rot_apply(cplx<R> (&x)[1 << K], const cplx<R>* m
   }
 }
 
-// Im <lam|G|psi> over the group for G = X (1), Y (2) or Z (3) on local bit A.
+// The same for a step whose matrix is exp(-i theta G / 2) of generator g
+// (1 X, 2 Y, 3 Z; 0: any 2x2), using the form g fixes: Y real, Z diagonal,
+// X real on the diagonal and imaginary off it (4 multiply-adds per output
+// amplitude instead of 8 for a general complex matrix).
+template <typename R, int K, int A, bool DAG>
+__device__ __forceinline__ void rot_apply_g(cplx<R> (&x)[1 << K], const cplx<R>* m, int g) {
+  if (g == 2) {
+    const R m00 = m[0].re, m01 = DAG ? m[2].re : m[1].re, m10 = DAG ? m[1].re : m[2].re, m11 = m[3].re;
+#pragma unroll
+    for (int r = 0; r < (1 << K); ++r) {
+      if (r & (1 << A)) continue;
+      const cplx<R> a = x[r], b = x[r | (1 << A)];
+      x[r] = mk(rfma(m01, b.re, m00 * a.re), rfma(m01, b.im, m00 * a.im));
+      x[r | (1 << A)] = mk(rfma(m11, b.re, m10 * a.re), rfma(m11, b.im, m10 * a.im));
+    }
+  } else if (g == 3) {
+    const cplx<R> d0 = DAG ? cconj(m[0]) : m[0], d1 = DAG ? cconj(m[3]) : m[3];
+#pragma unroll
+    for (int r = 0; r < (1 << K); ++r) {
+      if (r & (1 << A)) continue;
+      x[r] = cmul(d0, x[r]);
+      x[r | (1 << A)] = cmul(d1, x[r | (1 << A)]);
+    }
+  } else if (g == 1) {
+    // m = [[c0, i s01], [i s10, c1]] (conjugate transpose: s01 <- -s10, s10 <- -s01)
+    const R c0 = m[0].re, c1 = m[3].re;
+    const R s01 = DAG ? -m[2].im : m[1].im, s10 = DAG ? -m[1].im : m[2].im;
+#pragma unroll
+    for (int r = 0; r < (1 << K); ++r) {
+      if (r & (1 << A)) continue;
+      const cplx<R> a = x[r], b = x[r | (1 << A)];
+      x[r] = mk(rfma(-s01, b.im, c0 * a.re), rfma(s01, b.re, c0 * a.im));
+      x[r | (1 << A)] = mk(rfma(c1, b.re, -s10 * a.im), rfma(c1, b.im, s10 * a.re));
+    }
+  } else {
+    rot_apply<R, K, A, DAG>(x, m);
+  }
+}
+
+// Im <lam|G|psi> over the group for G = X (1), Y (2) or Z (3) on local bit A:
+// each pair's four-term sum in the state's precision (the products of two
+// amplitudes), accumulated in FP64.
 template <typename R, int K, int A>
 __device__ __forceinline__ double rot_gen(const cplx<R> (&p)[1 << K], const cplx<R> (&l)[1 << K], int gen) {
   double acc = 0.0;
@@ -91,15 +135,17 @@ __device__ __forceinline__ double rot_gen(const cplx<R> (&p)[1 << K], const cplx
   for (int r = 0; r < (1 << K); ++r) {
     if (r & (1 << A)) continue;
     const int s = r | (1 << A);
-    const double p0r = p[r].re, p0i = p[r].im, p1r = p[s].re, p1i = p[s].im;
-    const double l0r = l[r].re, l0i = l[r].im, l1r = l[s].re, l1i = l[s].im;
+    const R p0r = p[r].re, p0i = p[r].im, p1r = p[s].re, p1i = p[s].im;
+    const R l0r = l[r].re, l0i = l[r].im, l1r = l[s].re, l1i = l[s].im;
     // Im(conj(l0) g0 + conj(l1) g1), (g0, g1) = G (p0, p1)
+    R t;
     if (gen == 1)  // X: (p1, p0)
-      acc += l0r * p1i - l0i * p1r + l1r * p0i - l1i * p0r;
+      t = rfma(l0r, p1i, rfma(-l0i, p1r, rfma(l1r, p0i, -l1i * p0r)));
     else if (gen == 2)  // Y: (-i p1, i p0)
-      acc += -l0r * p1r - l0i * p1i + l1r * p0r + l1i * p0i;
+      t = rfma(-l0r, p1r, rfma(-l0i, p1i, rfma(l1r, p0r, l1i * p0i)));
     else  // Z: (p0, -p1)
-      acc += l0r * p0i - l0i * p0r - l1r * p1i + l1i * p1r;
+      t = rfma(l0r, p0i, rfma(-l0i, p0r, rfma(-l1r, p1i, l1i * p1r)));
+    acc += (double)t;
   }
   return acc;
 }
@@ -114,15 +160,15 @@ struct RotBit {
       if (!ADJ) {
 #pragma unroll
         for (int j = 0; j < QJ_ROT_MAX_STEPS; ++j)
-          if (j < a.nrot) rot_apply<R, K, A, false>(x, m + (A * a.nrot + j) * 4);
+          if (j < a.nrot) rot_apply_g<R, K, A, false>(x, m + (A * a.nrot + j) * 4, a.gen[A][j]);
       } else {
 #pragma unroll
         for (int jj = QJ_ROT_MAX_STEPS - 1; jj >= 0; --jj) {
           if (jj < a.nrot) {
             const int g = a.gen[A][jj];
             if (g) acc[A * QJ_ROT_MAX_STEPS + jj] += rot_gen<R, K, A>(x, y, g);
-            rot_apply<R, K, A, true>(x, m + (A * a.nrot + jj) * 4);
-            rot_apply<R, K, A, true>(y, m + (A * a.nrot + jj) * 4);
+            rot_apply_g<R, K, A, true>(x, m + (A * a.nrot + jj) * 4, g);
+            rot_apply_g<R, K, A, true>(y, m + (A * a.nrot + jj) * 4, g);
           }
         }
       }
@@ -333,14 +379,14 @@ __device__ __forceinline__ void tile_rot(cplx<float> (&x)[16], cplx<float> (&y)[
     if ((a.rotmask >> j) & 1) {
       if (!ADJ) {
 #pragma unroll
-        for (int s = 0; s < NROT; ++s) rot_apply<float, 4, A, false>(x, m + (j * NROT + s) * 4);
+        for (int s = 0; s < NROT; ++s) rot_apply_g<float, 4, A, false>(x, m + (j * NROT + s) * 4, a.gen[j][s]);
       } else {
 #pragma unroll
         for (int s = NROT - 1; s >= 0; --s) {
           const int g = a.gen[j][s];
           if (g) acc[j * NROT + s] += rot_gen<float, 4, A>(x, y, g);
-          rot_apply<float, 4, A, true>(x, m + (j * NROT + s) * 4);
-          rot_apply<float, 4, A, true>(y, m + (j * NROT + s) * 4);
+          rot_apply_g<float, 4, A, true>(x, m + (j * NROT + s) * 4, g);
+          rot_apply_g<float, 4, A, true>(y, m + (j * NROT + s) * 4, g);
         }
       }
     }

@@ -8,6 +8,9 @@ import numpy as np
 import pytest
 
 from oracle import ansatz_ref, einsum_ref
+from oracle.gate_matrices import RX as rx_matrix
+from oracle.gate_matrices import RY as ry_matrix
+from oracle.gate_matrices import RZ as rz_matrix
 from oracle.gate_matrices import haar_unitary, random_state
 
 pytestmark = pytest.mark.gpu
@@ -179,6 +182,62 @@ def test_rot_pass_forward_against_numpy(torch_mod, dtype, rng, load, bits):
         assert np.max(np.abs(got[s] - x)) < AMP[dtype]
 
 
+@pytest.mark.parametrize("bits,forms", [((0, 3, 5), (1, 2, 3)), ((1, 2, 4, 6), (2, 0, 3)), ((0, 1), (3, 1, 0))])
+def test_rot_pass_generator_forms(torch_mod, dtype, rng, bits, forms):
+    """Steps declared by their generator (X, Y, Z) run in the X/Y/Z matrix
+    forms, forward and adjoint (psi, lam and Im<lam|G|psi>), against numpy;
+    a step without a generator keeps the general 2x2 path."""
+    torch = torch_mod
+    from qibojit_amd import _native
+
+    n, nstates = 8, 2
+    code = _native.QJ_COMPLEX64 if dtype == "complex64" else _native.QJ_COMPLEX128
+    paulis = {1: np.array([[0, 1], [1, 0]]), 2: np.array([[0, -1j], [1j, 0]]), 3: np.diag([1.0, -1.0])}
+    k, nrot = len(bits), len(forms)
+    mats = np.zeros((nstates, k, nrot, 2, 2), dtype=np.complex128)
+    for s in range(nstates):
+        for a in range(k):
+            th = rng.uniform(-np.pi, np.pi, size=nrot)
+            by_form = {1: rx_matrix, 2: ry_matrix, 3: rz_matrix}
+            mats[s, a] = [by_form[f](th[j]) if f else haar_unitary(2, rng) for j, f in enumerate(forms)]
+    gens = [g for _ in bits for g in forms]
+    dm = torch.from_numpy(mats.astype(dtype).reshape(nstates, -1)).cuda()
+    host = np.stack([random_state(n, rng) for _ in range(nstates)]).astype(dtype)
+    lamh = np.stack([random_state(n, rng) for _ in range(nstates)]).astype(dtype)
+    # forward with the forms declared
+    src = torch.from_numpy(host).cuda()
+    dst = src.clone()
+    _rot_call(src, dst, None, None, code, n, 1 << n, None, nstates, list(bits), nrot, gens, 0, dm, k * nrot * 4,
+              None, 1)
+    for s in range(nstates):
+        x = host[s].astype(np.complex128)
+        for a, bit in enumerate(bits):
+            for j in range(nrot):
+                x = einsum_ref.apply_gate(x, mats[s, a, j], [n - 1 - bit], nqubits=n)
+        assert np.max(np.abs(_np(dst)[s] - x)) < AMP[dtype]
+    # adjoint
+    psi, lam = torch.from_numpy(host).cuda(), torch.from_numpy(lamh).cuda()
+    pout, lout = psi.clone(), lam.clone()
+    ncols = 2
+    part = torch.zeros((k * nrot, nstates, ncols), dtype=torch.float64).cuda()
+    _rot_call(psi, pout, lam, lout, code, n, 1 << n, None, nstates, list(bits), nrot, gens, 0, dm, k * nrot * 4,
+              part, ncols)
+    grad = _np(part).sum(axis=-1)
+    for s in range(nstates):
+        x, y = host[s].astype(np.complex128), lamh[s].astype(np.complex128)
+        for a, bit in enumerate(bits):
+            t = [n - 1 - bit]
+            for j in reversed(range(nrot)):
+                if forms[j]:
+                    gx = einsum_ref.apply_gate(x, paulis[forms[j]], t, nqubits=n)
+                    assert abs(grad[a * nrot + j, s] - np.vdot(y, gx).imag) < 10 * AMP[dtype]
+                u = mats[s, a, j].conj().T
+                x = einsum_ref.apply_gate(x, u, t, nqubits=n)
+                y = einsum_ref.apply_gate(y, u, t, nqubits=n)
+        assert np.max(np.abs(_np(pout)[s] - x)) < AMP[dtype]
+        assert np.max(np.abs(_np(lout)[s] - y)) < AMP[dtype]
+
+
 def test_rot_pass_zero_load_builds_product_state(torch_mod, rng):
     from qibojit_amd import _native
 
@@ -268,11 +327,17 @@ def test_rot_tile_pass_against_numpy(torch_mod, rng, load):
     rotmask = sum(1 << j for j in rot)
     host = np.stack([random_state(n, rng) for _ in range(nstates)]).astype(np.complex64)
     lamh = np.stack([random_state(n, rng) for _ in range(nstates)]).astype(np.complex64)
+    # steps with a generator are exp(-i theta G / 2) (the form the kernel
+    # relies on); the other steps of rotated bits are general unitaries
     mats = np.zeros((nstates, 12, nrot, 2, 2), dtype=np.complex128)
     for s in range(nstates):
         for j in range(12):
             for st in range(nrot):
-                mats[s, j, st] = haar_unitary(2, rng)
+                if j in rot:
+                    th = rng.uniform(-np.pi, np.pi)
+                    mats[s, j, st] = (ry_matrix(th), rz_matrix(th))[st]
+                else:
+                    mats[s, j, st] = haar_unitary(2, rng)
     gens = [g for j in range(12) for g in ((2, 3) if j in rot else (0, 0))]
     idx = np.arange(1 << n)
 
