@@ -169,14 +169,14 @@ class AnsatzEngine:
     that loads through the ladder permutation writes the other buffer);
     gradients add a second pair for lambda.  With ``tiles`` (default for
     complex64 and n >= 12) every forward pass is an LDS tile pass of 12 bits,
-    else a register pass of ``k_forward`` bits; ``tiles_adjoint`` does the
-    same for the adjoint passes (register passes of ``k_adjoint`` bits by
-    default: with psi and lambda the tile kernel holds 217 VGPRs and runs
-    at half the register kernel's bandwidth).
+    else a register pass of ``k_forward`` bits; ``tiles_adjoint`` (same
+    default) does the same for the adjoint passes, else register passes of
+    ``k_adjoint`` bits.  At 24 qubits, batch 16: 0.106 s per gradient batch
+    with tile adjoint passes, 0.112 s with 4-bit register passes.
     """
 
     def __init__(self, ansatz, batch, dtype="complex64", g=1.0, k_forward=5, k_adjoint=4, device=None,
-                 adjoint_cols=256, tfim_cols=256, stream=None, tiles=None, tiles_adjoint=False):
+                 adjoint_cols=256, tfim_cols=256, stream=None, tiles=None, tiles_adjoint=None):
         self.ansatz = ansatz
         self.batch = int(batch)
         if not 1 <= self.batch <= 65535:
@@ -189,6 +189,8 @@ class AnsatzEngine:
         self.k_adjoint = int(min(k_adjoint, ROT_MAX_ADJ_BITS[self.tdtype], n))
         if tiles is None:
             tiles = self.tdtype == torch.complex64 and n >= TILE_BITS
+        if tiles_adjoint is None:
+            tiles_adjoint = self.tdtype == torch.complex64 and n >= TILE_BITS
         if (tiles or tiles_adjoint) and (self.tdtype != torch.complex64 or n < TILE_BITS):
             raise ValueError("Tile passes need complex64 and at least 12 qubits.")
         self.tiles = bool(tiles)

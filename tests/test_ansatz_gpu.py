@@ -299,7 +299,7 @@ def test_tile_engine_energies_states_and_gradients(torch_mod, rng, n, layers, ba
     ansatz = HardwareEfficientAnsatz(n, layers)
     p = ansatz.random_parameters(rng, batch)
     tile = AnsatzEngine(ansatz, batch, dtype="complex64", tiles=True, tiles_adjoint=True)
-    reg = AnsatzEngine(ansatz, batch, dtype="complex64", tiles=False)
+    reg = AnsatzEngine(ansatz, batch, dtype="complex64", tiles=False, tiles_adjoint=False)
     e_t, g_t = tile.gradients(p)
     e_r, g_r = reg.gradients(p)
     np.testing.assert_allclose(_np(e_t), _np(e_r), atol=ENERGY["complex64"])
