@@ -211,3 +211,37 @@ def test_fusion_switch_gives_same_state(backend, rng):
         backend.fusion = True
     b = backend.to_numpy(backend.execute_circuit(c, psi.copy()).state())
     np.testing.assert_allclose(a, b, atol=1e-12, rtol=0)
+
+
+@pytest.mark.parametrize("kernel", ["register", "lds"])
+def test_tile_pass_of_swaps_only(backend, monkeypatch, kernel, dtype, rng):
+    """A pass whose only ops are in-tile swaps (pure relabelling: the store
+    writes through the permuted bits)."""
+    if kernel == "lds":
+        monkeypatch.setenv("QJ_TILE_KERNEL", "lds")
+    n = 14
+    npdt = np.complex64 if dtype == "complex64" else np.complex128
+    low, nhigh = (6, 7) if dtype == "complex64" else (5, 7)
+    p = tiles.Pass(low)
+    p.high = [6, 8, 9, 10, 11, 12, 13]
+    p.ops = [tiles.SwapOp(0, 13), tiles.SwapOp(2, 9), tiles.SwapOp(13, 6), tiles.SwapOp(1, 3)]
+    psi = gm.random_state(n, rng, npdt)
+    got, want = _run_and_emulate(backend, [p], n, dtype, psi)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_tile_pass_with_plan_over_the_lds_budget(backend, rng):
+    """Many phase ops (each carries per-segment factor tables) make the
+    register kernel's plan too large for its LDS share: the pass runs on the
+    LDS kernel and still matches the emulator."""
+    n, dtype = 14, "complex128"
+    p = tiles.Pass(5)
+    p.high = list(range(5, 12))
+    p.ops = []
+    for i in range(30):
+        c = int(rng.integers(0, 12))
+        p.ops.append(tiles.PhaseOp({c}, 0.0, {b: 0.01 * (i + b) for b in range(n) if b != c}))
+        p.ops.append(tiles.DenseOp([int(rng.integers(0, 12))], [], gm.haar_unitary(2, rng)))
+    psi = gm.random_state(n, rng)
+    got, want = _run_and_emulate(backend, [p], n, dtype, psi)
+    np.testing.assert_allclose(got, want, atol=1e-11, rtol=0)
