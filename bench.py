@@ -49,6 +49,7 @@ def parse():
     p.add_argument("--only-adjoint", action="store_true", help="adjoint-gradient benchmark only")
     p.add_argument("--only-batch", action="store_true", help="many-circuit ansatz benchmark only")
     p.add_argument("--only-precision", action="store_true", help="complex64 vs complex128 runs and the MFMA kernel")
+    p.add_argument("--only-clifford", action="store_true", help="stabilizer-tableau benchmark only")
     return p.parse_args()
 
 
@@ -654,6 +655,110 @@ def run_batch(backend, ncirc=1024, steps=5, g=1.0, seed=4096):
     return res
 
 
+def _clifford_gates(n, ngates, seed):
+    """Seeded gate list: 60% one-qubit gates (10 kinds), 40% two-qubit (5)."""
+    from qibojit_amd.backends import clifford_operations_hip as ch
+
+    rng = np.random.default_rng(seed)
+    two = rng.random(ngates) < 0.4
+    kinds = np.where(two, rng.integers(10, 15, ngates), rng.integers(0, 10, ngates)).astype(np.int32)
+    a = rng.integers(0, n, ngates)
+    b = (a + rng.integers(1, n, ngates)) % n if n > 1 else a
+    gates = np.stack([kinds, a, np.where(two, b, -1)], axis=1).astype(np.int32)
+    names = {v: k for k, v in ch.KINDS.items()}
+    return gates, names
+
+
+def run_clifford(sizes=((100, 100000), (1000, 100000), (10000, 20000)), reps=3, nmeas=64, cpu_seconds=2.0,
+                 seed=77):
+    """Stabilizer tableaux: a seeded random Clifford gate list from |0...0>
+    applied in one launch (k_cliff_circuit), the same list one launch per
+    gate (k_cliff_gate, as the reference launches), the oracle's per-gate
+    NumPy rules on the host (bounded sample), bit-exact check of the launch
+    against the oracle, and collapsing measurements of `nmeas` qubits."""
+    import torch
+
+    from oracle import clifford_ref as cr
+    from qibojit_amd.backends import clifford_operations_hip as ch
+
+    out = []
+    for n, ngates in sizes:
+        gates, names = _clifford_gates(n, ngates, seed + n)
+        rec = {"nqubits": n, "gates": ngates, "two_qubit_fraction": float(np.mean(gates[:, 0] >= 10))}
+        tab = ch.zero_state(n)
+        ch.apply_circuit(tab, gates, n)  # warm-up + parity run
+        torch.cuda.synchronize()
+        # oracle on the same list (bounded by time: checks the prefix it reaches)
+        sm = cr.zero_tableau(n)
+        t0 = time.perf_counter()
+        done = 0
+        for kind, a, b in gates:
+            cr.apply(sm, names[int(kind)], (int(a),) if kind < 10 else (int(a), int(b)), n)
+            done += 1
+            if time.perf_counter() - t0 > 4 * cpu_seconds and done < ngates:
+                break
+        cpu_t = time.perf_counter() - t0
+        rec["cpu_oracle_gates_per_s"] = done / cpu_t
+        rec["cpu_oracle_sample"] = f"first {done} gates, {cpu_t:.2f} s, NumPy rules on the dense uint8 tableau, 1 thread"
+        if done == ngates:
+            rec["bit_exact_vs_oracle"] = bool(np.array_equal(ch._clifford_post_execution_reshape(tab, n).cpu().numpy(), sm))
+        else:
+            check = ch.zero_state(n)
+            ch.apply_circuit(check, gates[:done], n)
+            rec["bit_exact_vs_oracle"] = bool(
+                np.array_equal(ch._clifford_post_execution_reshape(check, n).cpu().numpy(), sm))
+            rec["bit_exact_prefix_gates"] = done
+            del check
+        # one launch for the whole list
+        stream = torch.cuda.current_stream()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(stream)
+        for _ in range(reps):
+            ch.apply_circuit(tab, gates, n)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        t_launch = ev[0].elapsed_time(ev[1]) / 1e3 / reps
+        rec["launch_seconds"] = t_launch
+        rec["launch_gates_per_s"] = ngates / t_launch
+        rec["ns_per_gate"] = t_launch / ngates * 1e9
+        # host time incl. scheduling + descriptor upload
+        t0 = time.perf_counter()
+        ch.apply_circuit(tab, gates, n)
+        torch.cuda.synchronize()
+        rec["call_seconds_incl_host_schedule"] = time.perf_counter() - t0
+        # one launch per gate
+        sub = gates[: min(ngates, 20000)]
+        ev[0].record(stream)
+        for kind, a, b in sub:
+            ch._apply(int(kind), tab, int(a), int(b), n)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        t_pg = ev[0].elapsed_time(ev[1]) / 1e3
+        rec["per_gate_launch_gates_per_s"] = len(sub) / t_pg
+        rec["speedup_launch_vs_per_gate"] = rec["launch_gates_per_s"] / rec["per_gate_launch_gates_per_s"]
+        rec["speedup_launch_vs_cpu_oracle"] = rec["launch_gates_per_s"] / rec["cpu_oracle_gates_per_s"]
+        # the reference's traffic model: each gate reads and writes its 3 (5) columns
+        col_bytes = (2 * n + 1) / 8
+        rec["per_gate_model_GBps"] = float(np.mean(np.where(gates[:, 0] >= 10, 5, 3)) * 2 * col_bytes
+                                           * rec["launch_gates_per_s"] / 1e9)
+        rec["tableau_MB"] = (2 * n + 1) * ch.column_words(n) * 4 / 1e6
+        # collapsing measurements
+        rng = np.random.default_rng(seed)
+        qubits = [int(v) for v in rng.choice(n, size=min(nmeas, n), replace=False)]
+        np.random.seed(seed)
+        ch.M(tab, qubits[:2], n, collapse=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sample = ch.M(tab, qubits, n, collapse=True)
+        torch.cuda.synchronize()
+        rec["measure_seconds_per_qubit"] = (time.perf_counter() - t0) / len(qubits)
+        rec["measured_ones"] = int(sum(sample))
+        out.append(rec)
+        del tab
+        torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(n, min_seconds=10.0, max_seconds=30.0):
     """The reference's kernels restated in C/OpenMP, on this host's cores.
 
@@ -732,6 +837,9 @@ def main():
     if args.only_batch:
         print(json.dumps({"batch_1024": run_batch(backend)}), flush=True)
         return
+    if args.only_clifford:
+        print(json.dumps({"clifford": run_clifford()}), flush=True)
+        return
     if args.only_precision:
         print(json.dumps({"mfma_kernel_32q": run_mfma_kernel(backend), "precision_32q": run_precision(backend)},
                          default=str), flush=True)
@@ -760,6 +868,8 @@ def main():
         extras["adjoint_24q_8layers_batch16"] = run_adjoint(backend)
         print("# 1024 ansatz circuits", file=sys.stderr, flush=True)
         extras["batch_1024_ansatz_complex64"] = run_batch(backend)
+        print("# clifford tableaux", file=sys.stderr, flush=True)
+        extras["clifford_tableau"] = run_clifford()
 
     name, avg_ms, avg_bytes, achieved, launches = dominant_kernel(main_res["rows"])
     roofline = {

@@ -377,6 +377,85 @@ int qj_expectation_tfim(const void* state, int dtype, int nqubits, double g,
 int qj_axpby(const void* x, void* y, int dtype, uint64_t nelements,
              const double* alpha, const double* beta, void* stream);
 
+/* ---- stabilizer tableaux (Clifford circuits) ----
+ *
+ * Replaces the module functions of src/qibojit/backends/
+ * clifford_operations_gpu.py (CUDA RawKernels on a row-packed uint8 matrix)
+ * and clifford_operations_cpu.py (numba, unpacked uint8 matrix).
+ *
+ * The reference's tableau of n qubits is a (2n+1) x (2n+1) matrix of 0/1
+ * bytes: rows 0..n-1 destabilizers, n..2n-1 stabilizers, 2n scratch;
+ * columns 0..n-1 x bits, n..2n-1 z bits, 2n the sign r.  Here the device
+ * tableau (`tab`, uint32 words, device memory) stores it by column: column
+ * c occupies words [c * ws, (c + 1) * ws) with ws =
+ * qj_clifford_column_words(n), row j is bit j % 32 of word j / 32, and bits
+ * of rows >= 2n+1 are zero.  A tableau is (2n+1) * ws words. */
+#define QJ_CLIFF_H 0
+#define QJ_CLIFF_S 1
+#define QJ_CLIFF_Z 2
+#define QJ_CLIFF_X 3
+#define QJ_CLIFF_Y 4
+#define QJ_CLIFF_SX 5
+#define QJ_CLIFF_SDG 6
+#define QJ_CLIFF_SXDG 7
+#define QJ_CLIFF_RY_PI 8    /* RY(pi/2): the reference's RY_pi */
+#define QJ_CLIFF_RY_3PI_2 9 /* RY(3pi/2) */
+#define QJ_CLIFF_CNOT 10    /* first two-qubit kind: (control, target) */
+#define QJ_CLIFF_CZ 11
+#define QJ_CLIFF_SWAP 12
+#define QJ_CLIFF_ISWAP 13
+#define QJ_CLIFF_CY 14
+#define QJ_CLIFF_MAX_QUBITS (1 << 19)
+
+/* Words per column (ws) for n qubits: ceil((2n+1)/32) rounded up to a
+ * multiple of 4; -1 for n outside [1, QJ_CLIFF_MAX_QUBITS]. */
+int64_t qj_clifford_column_words(int nqubits);
+/* Dense (2n+1) x (2n+1) row-major byte matrix (device; nonzero = 1) ->
+ * device tableau, and back (bytes 0/1).  The layout change the reference
+ * makes with _clifford_pre/post_execution_reshape
+ * (clifford_operations_gpu.py:694-700). */
+int qj_clifford_pack(const uint8_t* dense, int nqubits, uint32_t* tab, void* stream);
+int qj_clifford_unpack(const uint32_t* tab, int nqubits, uint8_t* dense, void* stream);
+/* One gate (the kernels apply_H ... apply_CY, clifford_operations_gpu.py:
+ * 49-594; rules of clifford_operations_cpu.py:18-248) on every row.  kind is
+ * a QJ_CLIFF_* value; q1 is the target of two-qubit kinds (q0 the control)
+ * and ignored otherwise. */
+int qj_clifford_apply_gate(uint32_t* tab, int nqubits, int kind, int q0, int q1, void* stream);
+/* A gate list in one launch: `gates` is a host array of ngates triples
+ * (kind, q0, q1).  The result equals the gates applied one by one in list
+ * order.  No reference counterpart (the reference launches per gate). */
+int qj_clifford_apply_circuit(uint32_t* tab, int nqubits, const int32_t* gates, int64_t ngates, void* stream);
+/* *p_out (device int32) = first stabilizer p (0-based: row n+p) with x bit
+ * 1 on qubit q, or -1: the test of qibo's measurement loop that chooses
+ * between the random and the determined outcome. */
+int qj_clifford_first_stabilizer(const uint32_t* tab, int nqubits, int q, int32_t* p_out, void* stream);
+/* Random outcome on qubit q with first anticommuting stabilizer p
+ * (_random_outcome, clifford_operations_gpu.py:606-625): every row h < 2n,
+ * h != n+p, with x bit 1 on q becomes P_{n+p} P_h (rowsum); row p becomes
+ * the destabilizer row p; row n+p becomes (-1)^outcome Z_q.  The caller draws
+ * `outcome` (0 or 1). */
+int qj_clifford_random_outcome(uint32_t* tab, int nqubits, int p, int q, int outcome, void* stream);
+/* Determined outcome on qubit q (_determined_outcome,
+ * clifford_operations_gpu.py:628-640): the scratch row becomes the XOR of
+ * the stabilizer rows n+i with destabilizer x bit 1 on q, with the XOR of
+ * their signs (the reference's determined rowsum); *outcome_out (device
+ * int32) = that sign. */
+int qj_clifford_determined_outcome(uint32_t* tab, int nqubits, int q, int32_t* outcome_out, void* stream);
+/* rowsum of clifford_operations_cpu.py:337-385 on host arrays h, i of
+ * nrows row indices: row h[j] <- P_{i[j]} P_{h[j]} (x, z XORed; r = (2 r_h +
+ * 2 r_i + sum of the exponents of i) mod 4 != 0), every product from the
+ * state before the call.  The h rows must be distinct and no i row may be an
+ * h row.  determined != 0: every h[j] equals h[0], and row h[0] is XORed
+ * with every (row i[j] XOR row h[0]) and its sign with every new sign. */
+int qj_clifford_rowsum(uint32_t* tab, int nqubits, const int64_t* h, const int64_t* i, int64_t nrows,
+                       int determined, void* stream);
+/* numpy.packbits / numpy.unpackbits(count=...) along the last axis of a
+ * row-major byte matrix (device), big bit order: the reference's _packbits /
+ * _unpackbits (clifford_operations_cpu.py:256-327). */
+int qj_packbits_rows(const uint8_t* in, int64_t rows, int64_t cols, uint8_t* out, void* stream);
+int qj_unpackbits_rows(const uint8_t* in, int64_t rows, int64_t packed_cols, int64_t count, uint8_t* out,
+                       void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,4 +31,7 @@ How each part is pinned (so that a plausible slip fails a check):
     (tests/test_ops.py:236-256) and numpy.random.RandomState streams.
   * collapse / probabilities: slicing closed forms (tests/test_ops.py:65-83).
   * fused blocks: fused == unfused execution.
+  * clifford_ref: every rule against U P U^dagger of the gate's unitary,
+    rowsum against Pauli products, outcomes against the state vector of
+    the same Clifford circuit (tests/test_clifford.py).
 """

@@ -66,6 +66,21 @@ SIGNATURES = {
     "qj_compare_states": [_vp, _i32, _vp, _i32, _u64, _pdbl, _vp],
     "qj_apply_block_mfma": [_vp, _i32, _i32, _pi32, _i32, _vp, _vp],
     "qj_ansatz_lds": [_i32, _vp, _i32, _vp, _u64, _dbl, _vp, _i32, _i32, _vp],
+    "qj_clifford_pack": [_vp, _i32, _vp, _vp],
+    "qj_clifford_unpack": [_vp, _i32, _vp, _vp],
+    "qj_clifford_apply_gate": [_vp, _i32, _i32, _i32, _i32, _vp],
+    "qj_clifford_apply_circuit": [_vp, _i32, _pi32, _i64, _vp],
+    "qj_clifford_first_stabilizer": [_vp, _i32, _i32, _vp, _vp],
+    "qj_clifford_random_outcome": [_vp, _i32, _i32, _i32, _i32, _vp],
+    "qj_clifford_determined_outcome": [_vp, _i32, _i32, _vp, _vp],
+    "qj_clifford_rowsum": [_vp, _i32, _vp, _vp, _i64, _i32, _vp],
+    "qj_packbits_rows": [_vp, _i64, _i64, _vp, _vp],
+    "qj_unpackbits_rows": [_vp, _i64, _i64, _i64, _vp, _vp],
+}
+
+# entry points that return a value instead of a status
+VALUE_SIGNATURES = {
+    "qj_clifford_column_words": (ctypes.c_int64, [_i32]),
 }
 
 _LIB = None
@@ -90,6 +105,10 @@ def lib():
     for name, args in SIGNATURES.items():
         fn = getattr(handle, name)
         fn.restype = ctypes.c_int
+        fn.argtypes = args
+    for name, (res, args) in VALUE_SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
         fn.argtypes = args
     if handle.qj_abi_version() != ABI_VERSION:
         raise NativeLibraryMissing("libqjhip.so ABI version mismatch: rebuild the library.")

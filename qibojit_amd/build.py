@@ -16,7 +16,8 @@ LIB = os.path.join(HERE, "libqjhip.so")
 ARCH = os.environ.get("QJ_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["qj_runtime.hip", "qj_gates.hip", "qj_ops.hip", "qj_tile.hip", "qj_group.hip", "qj_mfma.hip"]
+SOURCES = ["qj_runtime.hip", "qj_gates.hip", "qj_ops.hip", "qj_tile.hip", "qj_group.hip", "qj_mfma.hip",
+           "qj_clifford.hip"]
 
 CFLAGS = [
     f"--offload-arch={ARCH}",
@@ -39,7 +40,7 @@ CFLAGS = [
 # stay unstructured, which keeps register-resident amplitudes from being
 # copied around every branch.
 _UNIFORM_CFG = ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
-EXTRA = {"qj_tile.hip": _UNIFORM_CFG, "qj_group.hip": _UNIFORM_CFG}
+EXTRA = {"qj_tile.hip": _UNIFORM_CFG, "qj_group.hip": _UNIFORM_CFG, "qj_clifford.hip": _UNIFORM_CFG}
 
 
 def _run(cmd):

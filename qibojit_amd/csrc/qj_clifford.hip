@@ -1,0 +1,893 @@
+// Stabilizer-tableau kernels on a column-major, bit-packed tableau.
+//
+// The tableau of n qubits has C = 2n+1 rows (destabilizers 0..n-1,
+// stabilizers n..2n-1, scratch 2n) and C columns (x bits 0..n-1, z bits
+// n..2n-1, sign r 2n).  Here it is stored by column: column c is a bit
+// vector over the rows, W = ceil(C/32) 32-bit words, row j in bit j%32 of
+// word j/32, and columns follow each other at a stride of ws = W rounded up
+// to a multiple of 4 words.  Bits of rows >= C are zero and stay zero under
+// every rule.
+//
+// Every Clifford rule rewrites a row from that row's own bits, so a gate on
+// qubit q touches only the columns x_q, z_q (x_t, z_t) and r, and one
+// bitwise operation on a word updates 32 rows.  A gate list runs in one
+// launch: each lane owns one word of every column (32 rows), keeps r in a
+// register and, where the workgroup's slice fits, works out of LDS; the host
+// groups gates on disjoint qubits so that the loads of a group issue
+// together.  Measurement (rowsum) works on the same layout: its phase sums
+// run bit-sliced over 32 rows at a time with a 2-bit counter per row.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "qj_common.h"
+
+namespace qj {
+namespace {
+
+enum : int {
+  kcH = QJ_CLIFF_H, kcS = QJ_CLIFF_S, kcZ = QJ_CLIFF_Z, kcX = QJ_CLIFF_X, kcY = QJ_CLIFF_Y,
+  kcSX = QJ_CLIFF_SX, kcSDG = QJ_CLIFF_SDG, kcSXDG = QJ_CLIFF_SXDG, kcRYpi = QJ_CLIFF_RY_PI,
+  kcRY3pi2 = QJ_CLIFF_RY_3PI_2, kcCNOT = QJ_CLIFF_CNOT, kcCZ = QJ_CLIFF_CZ, kcSWAP = QJ_CLIFF_SWAP,
+  kciSWAP = QJ_CLIFF_ISWAP, kcCY = QJ_CLIFF_CY, kcNop = 15
+};
+constexpr int kcFirstTwo = kcCNOT;
+constexpr int kCliffGroup = 8;          // gates per group (pairwise disjoint qubits)
+constexpr int kCliffMaxQubits = 1 << 19;
+constexpr int kMaxColumnChunks = 64;   // column chunks of a sliced rowsum
+constexpr size_t kCliffLdsPerCu = 160 * 1024;
+
+inline int64_t cliff_words(int n) { return (2 * (int64_t)n + 1 + 31) / 32; }
+inline int64_t cliff_stride(int n) { return (cliff_words(n) + 3) & ~(int64_t)3; }
+
+// The Clifford rules (src/qibojit/backends/clifford_operations_cpu.py:
+// 18-248) on the words of one row block: a, b = x, z of qubit qa (the
+// control), c, d = x, z of qubit qb (the target), all read before the gate.
+// The columns a gate changes are stored through col (column c of the block
+// at col[c * stride]; pa = qa * stride, pb = qb * stride, zoff = n * stride)
+// and the sign update is returned, to be XORed into r.  Each sign update is
+// the reference's Boolean function of the row's bits in a shorter form with
+// the same truth table: (x&z)^(x&(z^x)) = x for Z, z for X, x^z for Y,
+// z&~x for SX and RY(3pi/2), x&~z for SDG and RY(pi/2); of the two-qubit
+// rules SWAP's is identically 0, CZ's a&c&(b^d), iSWAP's
+// (~a&c&d)^(a&b&~c), CY's a&(b^c)&~(b^d) (the 16-entry tables of the
+// reference's expressions, checked against the oracle bit for bit).
+template <typename P>
+__device__ __forceinline__ uint32_t cliff_apply(int kind, P* col, int64_t pa, int64_t pb, int64_t zoff, uint32_t a,
+                                                uint32_t b, uint32_t c, uint32_t d) {
+  switch (kind) {
+    case kcH: col[pa] = b; col[zoff + pa] = a; return a & b;
+    case kcS: col[zoff + pa] = a ^ b; return a & b;
+    case kcZ: return a;
+    case kcX: return b;
+    case kcY: return a ^ b;
+    case kcSX: col[pa] = a ^ b; return b & ~a;
+    case kcSDG: col[zoff + pa] = a ^ b; return a & ~b;
+    case kcSXDG: col[pa] = a ^ b; return a & b;
+    case kcRYpi: col[pa] = b; col[zoff + pa] = a; return a & ~b;
+    case kcRY3pi2: col[pa] = b; col[zoff + pa] = a; return b & ~a;
+    case kcCNOT: col[pb] = c ^ a; col[zoff + pa] = b ^ d; return a & d & ~(b ^ c);
+    case kcCZ: col[zoff + pa] = b ^ c; col[zoff + pb] = d ^ a; return a & c & (b ^ d);
+    case kcSWAP: col[pa] = c; col[pb] = a; col[zoff + pa] = d; col[zoff + pb] = b; return 0u;
+    case kciSWAP:
+      col[pa] = c;
+      col[pb] = a;
+      col[zoff + pa] = c ^ d ^ a;
+      col[zoff + pb] = c ^ b ^ a;
+      return (~a & c & d) ^ (a & b & ~c);
+    case kcCY: col[pb] = a ^ c; col[zoff + pa] = b ^ d ^ c; col[zoff + pb] = d ^ a; return a & (b ^ c) & ~(b ^ d);
+    default: return 0u;  // empty slot
+  }
+}
+
+__device__ __forceinline__ uint32_t bit_of(const uint32_t* col, int64_t row) {
+  return (col[row >> 5] >> (row & 31)) & 1u;
+}
+
+// Mask of the rows of word w that are below `limit`.
+__device__ __forceinline__ uint32_t rows_below(int64_t w, int64_t limit) {
+  const int64_t lo = w * 32;
+  if (limit <= lo) return 0u;
+  if (limit >= lo + 32) return ~0u;
+  return (1u << (limit - lo)) - 1u;
+}
+
+// ---- layout conversion ----
+
+// dense: C x C bytes, row-major, nonzero = 1.  Thread per (column, word).
+__global__ void k_cliff_pack(const uint8_t* __restrict__ dense, int64_t C, int64_t W, int64_t ws,
+                             uint32_t* __restrict__ tab) {
+  const int64_t total = C * ws;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e % C, w = e / C;
+    uint32_t v = 0;
+    if (w < W) {
+      const int64_t r0 = w * 32;
+      const int nb = (int)std::min<int64_t>(32, C - r0);
+      for (int b = 0; b < nb; ++b) v |= (dense[(r0 + b) * C + c] != 0 ? 1u : 0u) << b;
+    }
+    tab[c * ws + w] = v;
+  }
+}
+
+// Thread per byte of the dense C x C matrix.
+__global__ void k_cliff_unpack(const uint32_t* __restrict__ tab, int64_t C, int64_t ws, uint8_t* __restrict__ dense) {
+  const int64_t total = C * C;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = e / C, c = e - row * C;
+    dense[e] = (uint8_t)bit_of(tab + c * ws, row);
+  }
+}
+
+// ---- gates ----
+
+// One gate, thread per word.
+__global__ void k_cliff_gate(uint32_t* __restrict__ tab, int n, int64_t ws, int64_t W, int kind, int qa, int qb) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  uint32_t* col = tab + w;
+  const int64_t pa = (int64_t)qa * ws, zoff = (int64_t)n * ws;
+  const int64_t pb = kind >= kcFirstTwo ? (int64_t)qb * ws : pa;
+  const uint32_t a = col[pa], b = col[zoff + pa], c = col[pb], d = col[zoff + pb];
+  const uint32_t t = cliff_apply(kind, col, pa, pb, zoff, a, b, c, d);
+  col[2 * zoff] ^= t;
+}
+
+// A gate list, one wave per workgroup, lane l owning word w0 + l of every
+// column (lanes past the slice repeat its last word, so every lane runs the
+// same code and stores the same values).  LDS = true stages the slice
+// (C x T words, column-major, lanes adjacent) in LDS for the whole list.
+// desc: ngroups x kCliffGroup slots of two words {qa, qb}; slot 0 carries
+// the group's kind in bits 24.. of its first word and its gate count in
+// bits 24.. of its second.  The
+// descriptors pass through an LDS ring of two chunks of kDescChunk groups:
+// the next chunk is loaded from global memory while the current one runs,
+// and the next group's words are read while the current group's columns
+// are in flight, so no group waits on global memory for its descriptors.
+constexpr int kDescWords = 2 * kCliffGroup;            // words per group
+constexpr int kDescChunk = 64;                          // groups per chunk
+constexpr int kDescChunkWords = kDescChunk * kDescWords;  // 1024
+constexpr int kDescPerLane = kDescChunkWords / 64;      // 16
+constexpr size_t kDescRingBytes = 2 * kDescChunkWords * sizeof(uint32_t);
+constexpr uint32_t kSlotMask = (1u << 24) - 1;  // qubit field of a slot word
+
+__device__ __forceinline__ void read_group(const uint32_t* p, uint32_t (&dw)[kDescWords]) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+  for (int k = 0; k < kDescWords / 4; ++k) {
+    const uint4 v = q[k];
+    dw[4 * k] = v.x;
+    dw[4 * k + 1] = v.y;
+    dw[4 * k + 2] = v.z;
+    dw[4 * k + 3] = v.w;
+  }
+}
+
+// One group of up to kCliffGroup gates of one kind on pairwise disjoint
+// qubits: every load first, then the rule slot by slot.  Slots past `cnt`
+// repeat slot 0 (same loads, same values stored) and their sign updates are
+// masked off, so the group runs without branches.  The next group's
+// descriptor words are read from LDS while this group's columns load.
+template <int KIND>
+__device__ __forceinline__ uint32_t run_group(uint32_t* col0, int64_t zoff, const int64_t (&oa)[kCliffGroup],
+                                              const int64_t (&ob)[kCliffGroup], int cnt, const uint32_t* cur, int g,
+                                              int ng, uint32_t (&dw)[kDescWords]) {
+  constexpr bool two = KIND >= kcFirstTwo;
+  uint32_t a[kCliffGroup], b[kCliffGroup], c[kCliffGroup], d[kCliffGroup];
+#pragma unroll
+  for (int j = 0; j < kCliffGroup; ++j) {
+    a[j] = col0[oa[j]];
+    b[j] = col0[zoff + oa[j]];
+    if (two) {
+      c[j] = col0[ob[j]];
+      d[j] = col0[zoff + ob[j]];
+    } else {
+      c[j] = a[j];
+      d[j] = b[j];
+    }
+  }
+  if (g + 1 < ng) read_group(cur + (g + 1) * kDescWords, dw);
+  uint32_t t = 0;
+#pragma unroll
+  for (int j = 0; j < kCliffGroup; ++j) {
+    const uint32_t tj = cliff_apply(KIND, col0, oa[j], ob[j], zoff, a[j], b[j], c[j], d[j]);
+    t ^= j < cnt ? tj : 0u;
+  }
+  return t;
+}
+
+template <bool LDS>
+__global__ __launch_bounds__(64) void k_cliff_circuit(uint32_t* __restrict__ tab, int n, int64_t ws, int64_t W,
+                                                      int T, const uint32_t* __restrict__ desc, int64_t ngroups) {
+  extern __shared__ __align__(16) uint32_t lds_words[];
+  uint32_t* ring = lds_words;
+  uint32_t* slice = lds_words + 2 * kDescChunkWords;
+  const int lane = threadIdx.x;
+  const int64_t w0 = (int64_t)blockIdx.x * T;
+  const int nact = (int)std::min<int64_t>(T, W - w0);
+  const int my = lane < nact ? lane : nact - 1;
+  const int64_t C = 2 * (int64_t)n + 1;
+  uint32_t* col0;
+  int64_t cs;
+  if (LDS) {
+    const int64_t total = C * T;
+    for (int64_t e = lane; e < total; e += 64) {
+      const int64_t c = e / T;
+      const int tt = (int)(e - c * T);
+      slice[e] = tt < nact ? tab[c * ws + w0 + tt] : 0u;
+    }
+    col0 = slice + my;
+    cs = T;
+  } else {
+    col0 = tab + w0 + my;
+    cs = ws;
+  }
+  const int64_t total_words = ngroups * kDescWords;
+  const int64_t nchunks = (ngroups + kDescChunk - 1) / kDescChunk;
+  // chunk 0 into ring buffer 0
+#pragma unroll
+  for (int k = 0; k < kDescPerLane; ++k) {
+    const int64_t e = (int64_t)k * 64 + lane;
+    ring[e] = e < total_words ? desc[e] : 0u;
+  }
+  __syncthreads();
+  const int64_t zoff = (int64_t)n * cs;
+  const int64_t roff = 2 * (int64_t)n * cs;
+  uint32_t r = col0[roff];
+  for (int64_t ch = 0; ch < nchunks; ++ch) {
+    const uint32_t* cur = ring + (ch & 1) * kDescChunkWords;
+    uint32_t nx[kDescPerLane];
+    const bool more = ch + 1 < nchunks;
+    if (more) {
+#pragma unroll
+      for (int k = 0; k < kDescPerLane; ++k) {
+        const int64_t e = (ch + 1) * kDescChunkWords + (int64_t)k * 64 + lane;
+        nx[k] = e < total_words ? desc[e] : 0u;
+      }
+    }
+    const int ng = (int)std::min<int64_t>(kDescChunk, ngroups - ch * kDescChunk);
+    uint32_t dw[kDescWords];
+    read_group(cur, dw);
+    for (int g = 0; g < ng; ++g) {
+      const uint32_t h0 = __builtin_amdgcn_readfirstlane(dw[0]);
+      const uint32_t h1 = __builtin_amdgcn_readfirstlane(dw[1]);
+      const int kind = (int)(h0 >> 24);
+      const int cnt = (int)(h1 >> 24);
+      int64_t oa[kCliffGroup], ob[kCliffGroup];
+#pragma unroll
+      for (int j = 0; j < kCliffGroup; ++j) {
+        oa[j] = (int64_t)(__builtin_amdgcn_readfirstlane(dw[2 * j]) & kSlotMask) * cs;
+        ob[j] = (int64_t)(__builtin_amdgcn_readfirstlane(dw[2 * j + 1]) & kSlotMask) * cs;
+      }
+      uint32_t t;
+      switch (kind) {
+        case kcH: t = run_group<kcH>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcS: t = run_group<kcS>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcZ: t = run_group<kcZ>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcX: t = run_group<kcX>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcY: t = run_group<kcY>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcSX: t = run_group<kcSX>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcSDG: t = run_group<kcSDG>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcSXDG: t = run_group<kcSXDG>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcRYpi: t = run_group<kcRYpi>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcRY3pi2: t = run_group<kcRY3pi2>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcCNOT: t = run_group<kcCNOT>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcCZ: t = run_group<kcCZ>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcSWAP: t = run_group<kcSWAP>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kciSWAP: t = run_group<kciSWAP>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        default: t = run_group<kcCY>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+      }
+      r ^= t;
+    }
+    if (more) {
+      uint32_t* nxt = ring + ((ch + 1) & 1) * kDescChunkWords;
+#pragma unroll
+      for (int k = 0; k < kDescPerLane; ++k) nxt[k * 64 + lane] = nx[k];
+      __syncthreads();
+    }
+  }
+  col0[roff] = r;
+  if (LDS) {
+    __syncthreads();
+    const int64_t total = C * T;
+    for (int64_t e = lane; e < total; e += 64) {
+      const int64_t c = e / T;
+      const int tt = (int)(e - c * T);
+      if (tt < nact) tab[c * ws + w0 + tt] = slice[e];
+    }
+  }
+}
+
+// ---- measurement ----
+
+// First stabilizer row (n <= row < 2n) with x_q = 1, as row - n, or -1.
+__global__ __launch_bounds__(256) void k_cliff_first_stab(const uint32_t* __restrict__ tab, int n, int64_t ws, int q,
+                                                          int32_t* __restrict__ out) {
+  __shared__ int best;
+  if (threadIdx.x == 0) best = 0x7fffffff;
+  __syncthreads();
+  const uint32_t* col = tab + (int64_t)q * ws;
+  const int64_t wlo = (int64_t)n >> 5, whi = (2 * (int64_t)n - 1) >> 5;
+  int mine = 0x7fffffff;
+  for (int64_t w = wlo + threadIdx.x; w <= whi; w += blockDim.x) {
+    const uint32_t m = col[w] & rows_below(w, 2 * (int64_t)n) & ~rows_below(w, n);
+    if (m) {
+      mine = (int)(w * 32 + __builtin_ctz(m));
+      break;  // words are visited in increasing order per thread
+    }
+  }
+  if (mine != 0x7fffffff) atomicMin(&best, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = best == 0x7fffffff ? -1 : best - n;
+}
+
+// hm[w] = x_q bits of rows < 2n (the rows a random outcome multiplies).
+__global__ void k_cliff_hmask(const uint32_t* __restrict__ tab, int64_t ws, int64_t W, int n, int q,
+                              uint32_t* __restrict__ hm) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w < W) hm[w] = tab[(int64_t)q * ws + w] & rows_below(w, 2 * (int64_t)n);
+}
+
+// Rows h in hm (minus p) <- P_p P_h, bit-sliced: thread (w, chunk) updates
+// the x/z words of columns [k0, k1) and sums the exponents of i of those
+// columns into a 2-bit counter per row (c0 + 2 c1 mod 4), written to
+// part[(chunk * W + w) * 2 + {0, 1}].  Row p is not rewritten, so its bits
+// read by every thread stay those of the state before the call.
+__global__ void k_cliff_rowsum_sliced(uint32_t* __restrict__ tab, int n, int64_t ws, int64_t W, int64_t p,
+                                      const uint32_t* __restrict__ hm, int kchunk, uint32_t* __restrict__ part) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  uint32_t h = hm[w];
+  if (w == (p >> 5)) h &= ~(1u << (p & 31));
+  uint32_t c0 = 0, c1 = 0;
+  const int k0 = blockIdx.y * kchunk;
+  const int k1 = std::min(n, k0 + kchunk);
+  if (h) {
+    for (int k = k0; k < k1; ++k) {
+      uint32_t* xc = tab + (int64_t)k * ws;
+      uint32_t* zc = tab + ((int64_t)n + k) * ws;
+      const uint32_t xp = bit_of(xc, p), zp = bit_of(zc, p);
+      if (!(xp | zp)) continue;
+      const uint32_t X = xc[w], Z = zc[w];
+      uint32_t P, M;  // rows whose exponent is +1 / -1 (clifford_operations_cpu.py:361-365)
+      if (xp && !zp) {
+        P = X & Z;
+        M = ~X & Z;
+      } else if (!xp) {
+        P = X & ~Z;
+        M = X & Z;
+      } else {
+        P = ~X & Z;
+        M = X & ~Z;
+      }
+      c1 ^= c0 & P;
+      c0 ^= P;
+      c1 ^= ~c0 & M;
+      c0 ^= M;
+      if (xp) xc[w] = X ^ h;
+      if (zp) zc[w] = Z ^ h;
+    }
+  }
+  part[(blockIdx.y * W + w) * 2] = c0;
+  part[(blockIdx.y * W + w) * 2 + 1] = c1;
+}
+
+// r_h = (2 r_h + 2 r_p + sum) mod 4 != 0 for the rows of hm (minus p).
+__global__ void k_cliff_rowsum_sign(uint32_t* __restrict__ tab, int n, int64_t ws, int64_t W, int64_t p,
+                                    const uint32_t* __restrict__ hm, int nchunks, const uint32_t* __restrict__ part) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  uint32_t h = hm[w];
+  if (w == (p >> 5)) h &= ~(1u << (p & 31));
+  if (!h) return;
+  uint32_t c0 = 0, c1 = 0;
+#pragma unroll 8
+  for (int y = 0; y < nchunks; ++y) {
+    const uint32_t a0 = part[(y * W + w) * 2], a1 = part[(y * W + w) * 2 + 1];
+    c1 ^= a1 ^ (c0 & a0);
+    c0 ^= a0;
+  }
+  uint32_t* rc = tab + 2 * (int64_t)n * ws;
+  const uint32_t r = rc[w];
+  const uint32_t rp = bit_of(rc, p) ? ~0u : 0u;
+  const uint32_t rnew = c0 | (c1 ^ r ^ rp);
+  rc[w] = (r & ~h) | (rnew & h);
+}
+
+// Row p - n <- row p; row p <- (r = outcome, z_q = 1, zero elsewhere).
+// Thread per column.
+__global__ void k_cliff_set_rows(uint32_t* __restrict__ tab, int n, int64_t ws, int64_t p, int q, int outcome) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t C = 2 * (int64_t)n + 1;
+  if (c >= C) return;
+  uint32_t* col = tab + c * ws;
+  const int64_t d = p - n;
+  const uint32_t b = bit_of(col, p);
+  col[d >> 5] = (col[d >> 5] & ~(1u << (d & 31))) | (b << (d & 31));
+  const uint32_t nb = c == (int64_t)n + q ? 1u : (c == C - 1 ? (uint32_t)(outcome & 1) : 0u);
+  col[p >> 5] = (col[p >> 5] & ~(1u << (p & 31))) | (nb << (p & 31));
+}
+
+// Determined outcome (clifford_operations_gpu.py:628-640 with the determined
+// rowsum of clifford_operations_cpu.py:337-385): the scratch row becomes the
+// XOR of the stabilizer rows n+i with destabilizer bit x_q(i) = 1, its sign
+// the XOR of their signs.  Workgroup (one wave) per column; the scratch bit
+// of column c is written by its own wave only.
+__global__ __launch_bounds__(64) void k_cliff_determined(uint32_t* __restrict__ tab, int n, int64_t ws, int q,
+                                                         int32_t* __restrict__ out) {
+  const int64_t c = blockIdx.x;
+  uint32_t* col = tab + c * ws;
+  const uint32_t* xq = tab + (int64_t)q * ws;
+  const int64_t nw = ((int64_t)n + 31) >> 5;
+  const int sh = n & 31;
+  uint32_t acc = 0;
+  for (int64_t j = threadIdx.x; j < nw; j += 64) {
+    const uint32_t m = xq[j] & rows_below(j, n);
+    if (!m) continue;
+    const int64_t a = (n + j * 32) >> 5;  // bits n+32j .. n+32j+31 of the column
+    uint32_t v = col[a] >> sh;
+    if (sh && a + 1 < ws) v |= col[a + 1] << (32 - sh);
+    acc ^= v & m;
+  }
+  const uint64_t odd = __ballot(__builtin_popcount(acc) & 1);
+  const uint32_t parity = (uint32_t)(__builtin_popcountll(odd) & 1);
+  if (threadIdx.x == 0) {
+    const int64_t s = 2 * (int64_t)n;
+    col[s >> 5] = (col[s >> 5] & ~(1u << (s & 31))) | (parity << (s & 31));
+    if (c == s) out[0] = (int32_t)parity;
+  }
+}
+
+// ---- general rowsum (clifford_operations_cpu.py:337-385) ----
+
+// New sign of every pair from the state before the call: rnew[j], and
+// flip[j] = rnew[j] ^ r_h[j].  Thread per pair.
+__global__ void k_cliff_pair_sign(const uint32_t* __restrict__ tab, int n, int64_t ws, const int64_t* __restrict__ h,
+                                  const int64_t* __restrict__ i, int64_t m, uint8_t* __restrict__ rnew,
+                                  uint8_t* __restrict__ flip) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int64_t hj = h[j], ij = i[j];
+  int e = 0;
+  for (int k = 0; k < n; ++k) {
+    const uint32_t* xc = tab + (int64_t)k * ws;
+    const uint32_t* zc = tab + ((int64_t)n + k) * ws;
+    const int xi = (int)bit_of(xc, ij), zi = (int)bit_of(zc, ij);
+    const int xh = (int)bit_of(xc, hj), zh = (int)bit_of(zc, hj);
+    e += 2 * (xi * xh * (zh - zi) + zi * zh * (xi - xh)) - xi * zh + xh * zi;
+  }
+  const uint32_t* rc = tab + 2 * (int64_t)n * ws;
+  const int rh = (int)bit_of(rc, hj), ri = (int)bit_of(rc, ij);
+  const int v = ((2 * rh + 2 * ri + e) % 4 + 4) % 4;
+  rnew[j] = v != 0;
+  flip[j] = (uint8_t)((v != 0) ^ rh);
+}
+
+// Rows h[j] <- x/z of i[j] XOR h[j], sign rnew[j]: thread per (pair,
+// column), XOR atomics on the words of the rows h (pairwise distinct, none of
+// them an i row, so the bits read are never the ones written).
+__global__ void k_cliff_pair_apply(uint32_t* __restrict__ tab, int n, int64_t ws, const int64_t* __restrict__ h,
+                                   const int64_t* __restrict__ i, int64_t m, const uint8_t* __restrict__ flip) {
+  const int64_t C = 2 * (int64_t)n + 1;
+  const int64_t total = m * C;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / C, c = e - j * C;
+    uint32_t* col = tab + c * ws;
+    const int64_t hj = h[j];
+    const uint32_t t = c < C - 1 ? bit_of(col, i[j]) : (uint32_t)flip[j];
+    if (t) atomicXor(col + (hj >> 5), 1u << (hj & 31));
+  }
+}
+
+// Determined mode: row h0 ^= XOR_j (row i[j] ^ row h0) on x/z, sign ^=
+// XOR_j rnew[j].  Thread per column.
+__global__ void k_cliff_pair_accumulate(uint32_t* __restrict__ tab, int n, int64_t ws, int64_t h0,
+                                        const int64_t* __restrict__ i, int64_t m, const uint8_t* __restrict__ rnew) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t C = 2 * (int64_t)n + 1;
+  if (c >= C) return;
+  uint32_t* col = tab + c * ws;
+  const uint32_t orig = bit_of(col, h0);
+  uint32_t acc = 0;
+  if (c < C - 1) {
+    for (int64_t j = 0; j < m; ++j) acc ^= bit_of(col, i[j]) ^ orig;
+  } else {
+    for (int64_t j = 0; j < m; ++j) acc ^= rnew[j];
+  }
+  if (acc) col[h0 >> 5] ^= 1u << (h0 & 31);
+}
+
+// ---- numpy-format bit packing of 0/1 byte matrices ----
+
+// out[r, b] = sum_k (in[r, 8b + k] != 0) << (7 - k)   (numpy.packbits, big bit order)
+__global__ void k_packbits_rows(const uint8_t* __restrict__ in, int64_t rows, int64_t cols, uint8_t* __restrict__ out) {
+  const int64_t pc = (cols + 7) >> 3;
+  const int64_t total = rows * pc;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / pc, b = e - r * pc;
+    uint32_t v = 0;
+    for (int k = 0; k < 8; ++k) {
+      const int64_t c = 8 * b + k;
+      if (c < cols && in[r * cols + c]) v |= 1u << (7 - k);
+    }
+    out[e] = (uint8_t)v;
+  }
+}
+
+// out[r, c] = bit 7 - c%8 of in[r, c/8] for c < count   (numpy.unpackbits)
+__global__ void k_unpackbits_rows(const uint8_t* __restrict__ in, int64_t rows, int64_t pcols, int64_t count,
+                                  uint8_t* __restrict__ out) {
+  const int64_t total = rows * count;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / count, c = e - r * count;
+    out[e] = (uint8_t)((in[r * pcols + (c >> 3)] >> (7 - (c & 7))) & 1u);
+  }
+}
+
+// ---- host side ----
+
+int check_tab(const void* tab, int n, const char* what) {
+  if (!tab) return set_error(QJ_EVALUE, "%s: null tableau", what);
+  if (n < 1 || n > kCliffMaxQubits) return set_error(QJ_EVALUE, "%s: nqubits %d out of range", what, n);
+  return QJ_OK;
+}
+
+unsigned blocks_for(int64_t items, int per) {
+  return (unsigned)grid_for((u64)std::max<int64_t>(items, 1), (u64)per);
+}
+
+void device_lds(int* cus, size_t* max_lds) {
+  static int ncu[64] = {0};
+  static int mlds[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = -1;
+  if (dev >= 0 && !ncu[dev]) {
+    int a = 0, b = 0;
+    if (hipDeviceGetAttribute(&a, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || a <= 0) a = 256;
+    if (hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || b <= 0) b = 65536;
+    ncu[dev] = a;
+    mlds[dev] = b;
+  }
+  *cus = dev >= 0 ? ncu[dev] : 256;
+  *max_lds = dev >= 0 ? (size_t)mlds[dev] : 65536;
+}
+
+// Groups of up to kCliffGroup gates of one kind on pairwise disjoint
+// qubits, by list scheduling: a gate is ready once every earlier gate that
+// shares a qubit with it has been placed in a group, ready gates act on
+// disjoint qubits (of two gates on one qubit only the earlier can be ready),
+// and each group takes the ready gates of the kind with the most of them.
+// This only moves gates past gates on other qubits: those commute, and so do
+// their XORs into r, so the tableau is the same bit for bit.  Returns the
+// descriptor words (ngroups x kCliffGroup x 2).
+int schedule_gates(const int32_t* gates, int64_t ngates, int n, std::vector<uint32_t>& desc) {
+  std::vector<int64_t> count(n + 1, 0);
+  for (int64_t g = 0; g < ngates; ++g) {
+    const int kind = gates[3 * g], a = gates[3 * g + 1], b = gates[3 * g + 2];
+    if (kind < 0 || kind >= kcNop) return set_error(QJ_EVALUE, "clifford circuit: gate %lld has bad kind %d", (long long)g, kind);
+    if (a < 0 || a >= n) return set_error(QJ_EVALUE, "clifford circuit: gate %lld qubit %d out of range", (long long)g, a);
+    ++count[a + 1];
+    if (kind >= kcFirstTwo) {
+      if (b < 0 || b >= n) return set_error(QJ_EVALUE, "clifford circuit: gate %lld qubit %d out of range", (long long)g, b);
+      if (b == a) return set_error(QJ_EVALUE, "clifford circuit: gate %lld acts twice on qubit %d", (long long)g, a);
+      ++count[b + 1];
+    }
+  }
+  // per-qubit gate lists in circuit order (CSR)
+  for (int q = 0; q < n; ++q) count[q + 1] += count[q];
+  std::vector<int64_t> lists(count[n]);
+  std::vector<int64_t> fillp(count.begin(), count.end() - 1);
+  for (int64_t g = 0; g < ngates; ++g) {
+    lists[fillp[gates[3 * g + 1]]++] = g;
+    if (gates[3 * g] >= kcFirstTwo) lists[fillp[gates[3 * g + 2]]++] = g;
+  }
+  std::vector<int64_t> head(count.begin(), count.end() - 1);
+  std::vector<uint8_t> hits(ngates, 0);
+  std::vector<std::vector<int64_t>> ready(kcNop);
+  std::vector<size_t> rpos(kcNop, 0);
+  int64_t nready = 0;
+  auto hit = [&](int64_t g) {
+    const int need = gates[3 * g] >= kcFirstTwo ? 2 : 1;
+    if (++hits[g] == need) {
+      ready[gates[3 * g]].push_back(g);
+      ++nready;
+    }
+  };
+  for (int q = 0; q < n; ++q)
+    if (head[q] < count[q + 1]) hit(lists[head[q]]);
+  desc.clear();
+  desc.reserve((size_t)(ngates / kCliffGroup + 16) * 2 * kCliffGroup);
+  int64_t placed = 0;
+  while (nready > 0) {
+    int best = 0;
+    size_t most = 0;
+    for (int k = 0; k < kcNop; ++k) {
+      const size_t avail = ready[k].size() - rpos[k];
+      if (avail > most) {
+        most = avail;
+        best = k;
+      }
+    }
+    const int cnt = (int)std::min<size_t>(most, kCliffGroup);
+    uint32_t slot[2 * kCliffGroup];
+    std::vector<int64_t> taken(ready[best].begin() + rpos[best], ready[best].begin() + rpos[best] + cnt);
+    rpos[best] += cnt;
+    nready -= cnt;
+    if (rpos[best] == ready[best].size()) {
+      ready[best].clear();
+      rpos[best] = 0;
+    }
+    for (int j = 0; j < cnt; ++j) {
+      const int64_t g = taken[j];
+      const int a = gates[3 * g + 1];
+      const int b = best >= kcFirstTwo ? gates[3 * g + 2] : a;
+      slot[2 * j] = (uint32_t)a;
+      slot[2 * j + 1] = (uint32_t)b;
+    }
+    for (int j = cnt; j < kCliffGroup; ++j) {  // padding: copies of slot 0
+      slot[2 * j] = slot[0];
+      slot[2 * j + 1] = slot[1];
+    }
+    slot[0] |= (uint32_t)best << 24;
+    slot[1] |= (uint32_t)cnt << 24;
+    desc.insert(desc.end(), slot, slot + 2 * kCliffGroup);
+    placed += cnt;
+    // advance the qubits of the placed gates; their next gates get a hit
+    for (int j = 0; j < cnt; ++j) {
+      const int64_t g = taken[j];
+      const int qs[2] = {gates[3 * g + 1], gates[3 * g + 2]};
+      for (int u = 0; u < (best >= kcFirstTwo ? 2 : 1); ++u) {
+        const int q = qs[u];
+        ++head[q];
+        if (head[q] < count[q + 1]) hit(lists[head[q]]);
+      }
+    }
+  }
+  if (placed != ngates) return set_error(QJ_EVALUE, "clifford circuit: scheduling placed %lld of %lld gates",
+                                         (long long)placed, (long long)ngates);
+  return QJ_OK;
+}
+
+}  // namespace
+}  // namespace qj
+
+using namespace qj;
+
+extern "C" {
+
+int64_t qj_clifford_column_words(int nqubits) {
+  if (nqubits < 1 || nqubits > kCliffMaxQubits) return -1;
+  return cliff_stride(nqubits);
+}
+
+int qj_clifford_pack(const uint8_t* dense, int nqubits, uint32_t* tab, void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_pack");
+  if (rc) return rc;
+  if (!dense) return set_error(QJ_EVALUE, "clifford_pack: null matrix");
+  const int64_t C = 2 * (int64_t)nqubits + 1;
+  k_cliff_pack<<<dim3(blocks_for(C * cliff_stride(nqubits), kBlock)), dim3(kBlock), 0, (hipStream_t)stream>>>(
+      dense, C, cliff_words(nqubits), cliff_stride(nqubits), tab);
+  return check_launch("clifford_pack launch");
+}
+
+int qj_clifford_unpack(const uint32_t* tab, int nqubits, uint8_t* dense, void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_unpack");
+  if (rc) return rc;
+  if (!dense) return set_error(QJ_EVALUE, "clifford_unpack: null matrix");
+  const int64_t C = 2 * (int64_t)nqubits + 1;
+  k_cliff_unpack<<<dim3(blocks_for(C * C, kBlock)), dim3(kBlock), 0, (hipStream_t)stream>>>(
+      tab, C, cliff_stride(nqubits), dense);
+  return check_launch("clifford_unpack launch");
+}
+
+int qj_clifford_apply_gate(uint32_t* tab, int nqubits, int kind, int q0, int q1, void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_apply_gate");
+  if (rc) return rc;
+  if (kind < 0 || kind >= kcNop) return set_error(QJ_EVALUE, "clifford_apply_gate: bad kind %d", kind);
+  if (q0 < 0 || q0 >= nqubits) return set_error(QJ_EVALUE, "clifford_apply_gate: qubit %d out of range", q0);
+  if (kind >= kcFirstTwo) {
+    if (q1 < 0 || q1 >= nqubits) return set_error(QJ_EVALUE, "clifford_apply_gate: qubit %d out of range", q1);
+    if (q1 == q0) return set_error(QJ_EVALUE, "clifford_apply_gate: control and target are both %d", q0);
+  }
+  const int64_t W = cliff_words(nqubits);
+  k_cliff_gate<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, (hipStream_t)stream>>>(
+      tab, nqubits, cliff_stride(nqubits), W, kind, q0, q1);
+  return check_launch("clifford_apply_gate launch");
+}
+
+int qj_clifford_apply_circuit(uint32_t* tab, int nqubits, const int32_t* gates, int64_t ngates, void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_apply_circuit");
+  if (rc) return rc;
+  if (ngates < 0) return set_error(QJ_EVALUE, "clifford_apply_circuit: negative gate count");
+  if (ngates == 0) return QJ_OK;
+  if (!gates) return set_error(QJ_EVALUE, "clifford_apply_circuit: null gate list");
+  std::vector<uint32_t> desc;
+  rc = schedule_gates(gates, ngates, nqubits, desc);
+  if (rc) return rc;
+  const int64_t ngroups = (int64_t)desc.size() / (2 * kCliffGroup);
+  hipStream_t s = (hipStream_t)stream;
+  void* ddesc = nullptr;
+  int ticket = -1;
+  rc = ring_upload(desc.data(), desc.size() * sizeof(uint32_t), s, &ddesc, &ticket);
+  if (rc) return rc;
+  const int64_t W = cliff_words(nqubits), ws = cliff_stride(nqubits);
+  const int64_t C = 2 * (int64_t)nqubits + 1;
+  int cus = 0;
+  size_t max_lds = 0;
+  device_lds(&cus, &max_lds);
+  const size_t cap = std::min(kCliffLdsPerCu, std::max<size_t>(max_lds, 65536)) - kDescRingBytes;
+  int64_t T = (int64_t)(cap / (4 * (size_t)C));
+  const char* kern = std::getenv("QJ_CLIFFORD_KERNEL");
+  const bool force_global = kern && std::strcmp(kern, "global") == 0;
+  // LDS slices when every workgroup is resident at once; otherwise the
+  // global variant (its columns stay in L2), which measures as fast as the
+  // LDS one from 2000 qubits up and needs no second round of workgroups.
+  bool use_lds = T >= 1 && !force_global;
+  int64_t nwg = 0;
+  if (use_lds) {
+    T = std::min<int64_t>({T, 64, W});
+    nwg = (W + T - 1) / T;
+    T = (W + nwg - 1) / nwg;  // even split
+    const size_t bytes = kDescRingBytes + (size_t)T * C * 4;
+    const int64_t per_cu = std::max<int64_t>(1, (int64_t)(kCliffLdsPerCu / bytes));
+    use_lds = nwg <= (int64_t)cus * per_cu;
+  }
+  if (use_lds) {
+    const size_t lds = kDescRingBytes + (size_t)T * C * 4;
+    if (lds > 65536)
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cliff_circuit<true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_cliff_circuit<true><<<dim3((unsigned)nwg), dim3(64), lds, s>>>(tab, nqubits, ws, W, (int)T,
+                                                                     static_cast<const uint32_t*>(ddesc), ngroups);
+  } else {
+    const int64_t gwg = (W + 63) / 64;
+    k_cliff_circuit<false><<<dim3((unsigned)gwg), dim3(64), kDescRingBytes, s>>>(tab, nqubits, ws, W, 64,
+                                                                    static_cast<const uint32_t*>(ddesc), ngroups);
+  }
+  rc = check_launch("clifford_apply_circuit launch");
+  ring_release(ticket, s);
+  return rc;
+}
+
+int qj_clifford_first_stabilizer(const uint32_t* tab, int nqubits, int q, int32_t* p_out, void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_first_stabilizer");
+  if (rc) return rc;
+  if (!p_out) return set_error(QJ_EVALUE, "clifford_first_stabilizer: null output");
+  if (q < 0 || q >= nqubits) return set_error(QJ_EVALUE, "clifford_first_stabilizer: qubit %d out of range", q);
+  k_cliff_first_stab<<<dim3(1), dim3(256), 0, (hipStream_t)stream>>>(tab, nqubits, cliff_stride(nqubits), q, p_out);
+  return check_launch("clifford_first_stabilizer launch");
+}
+
+int qj_clifford_random_outcome(uint32_t* tab, int nqubits, int p, int q, int outcome, void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_random_outcome");
+  if (rc) return rc;
+  if (q < 0 || q >= nqubits) return set_error(QJ_EVALUE, "clifford_random_outcome: qubit %d out of range", q);
+  if (p < 0 || p >= nqubits) return set_error(QJ_EVALUE, "clifford_random_outcome: stabilizer %d out of range", p);
+  if (outcome != 0 && outcome != 1) return set_error(QJ_EVALUE, "clifford_random_outcome: outcome must be 0 or 1");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t W = cliff_words(nqubits), ws = cliff_stride(nqubits);
+  const int64_t prow = (int64_t)nqubits + p;
+  // column chunks: enough (word, chunk) threads to keep every CU busy
+  int cus = 0;
+  size_t max_lds = 0;
+  device_lds(&cus, &max_lds);
+  int64_t nchunks = std::max<int64_t>(1, ((int64_t)cus * 1024) / std::max<int64_t>(W, 1));
+  nchunks = std::min<int64_t>(nchunks, std::min<int64_t>(nqubits, kMaxColumnChunks));
+  const int kchunk = (int)((nqubits + nchunks - 1) / nchunks);
+  nchunks = (nqubits + kchunk - 1) / kchunk;
+  DeviceScratch* sc = nullptr;
+  rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  rc = scratch_reserve_big(sc, (size_t)(W + 2 * nchunks * W) * sizeof(uint32_t), s);
+  if (rc) return rc;
+  uint32_t* hm = static_cast<uint32_t*>(sc->big);
+  uint32_t* part = hm + W;
+  k_cliff_hmask<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(tab, ws, W, nqubits, q, hm);
+  k_cliff_rowsum_sliced<<<dim3(blocks_for(W, kBlock), (unsigned)nchunks), dim3(kBlock), 0, s>>>(
+      tab, nqubits, ws, W, prow, hm, kchunk, part);
+  k_cliff_rowsum_sign<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, W, prow, hm,
+                                                                           (int)nchunks, part);
+  const int64_t C = 2 * (int64_t)nqubits + 1;
+  k_cliff_set_rows<<<dim3(blocks_for(C, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, prow, q, outcome);
+  return check_launch("clifford_random_outcome launch");
+}
+
+int qj_clifford_determined_outcome(uint32_t* tab, int nqubits, int q, int32_t* outcome_out, void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_determined_outcome");
+  if (rc) return rc;
+  if (!outcome_out) return set_error(QJ_EVALUE, "clifford_determined_outcome: null output");
+  if (q < 0 || q >= nqubits) return set_error(QJ_EVALUE, "clifford_determined_outcome: qubit %d out of range", q);
+  const int64_t C = 2 * (int64_t)nqubits + 1;
+  k_cliff_determined<<<dim3((unsigned)C), dim3(64), 0, (hipStream_t)stream>>>(tab, nqubits, cliff_stride(nqubits), q,
+                                                                              outcome_out);
+  return check_launch("clifford_determined_outcome launch");
+}
+
+int qj_clifford_rowsum(uint32_t* tab, int nqubits, const int64_t* h, const int64_t* i, int64_t nrows, int determined,
+                       void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_rowsum");
+  if (rc) return rc;
+  if (nrows < 0) return set_error(QJ_EVALUE, "clifford_rowsum: negative row count");
+  if (nrows == 0) return QJ_OK;
+  if (!h || !i) return set_error(QJ_EVALUE, "clifford_rowsum: null row list");
+  const int64_t C = 2 * (int64_t)nqubits + 1;
+  for (int64_t j = 0; j < nrows; ++j) {
+    if (h[j] < 0 || h[j] >= C || i[j] < 0 || i[j] >= C)
+      return set_error(QJ_EVALUE, "clifford_rowsum: row index out of range at %lld", (long long)j);
+  }
+  if (determined) {
+    for (int64_t j = 0; j < nrows; ++j) {
+      if (h[j] != h[0]) return set_error(QJ_EVALUE, "clifford_rowsum: determined mode needs a single target row");
+      if (i[j] == h[0]) return set_error(QJ_EVALUE, "clifford_rowsum: source row equals the target row");
+    }
+  } else {
+    std::vector<uint8_t> seen(C, 0);
+    for (int64_t j = 0; j < nrows; ++j) {
+      if (seen[h[j]] & 1) return set_error(QJ_EVALUE, "clifford_rowsum: target row %lld repeated", (long long)h[j]);
+      seen[h[j]] |= 1;
+    }
+    for (int64_t j = 0; j < nrows; ++j) {
+      if (seen[i[j]] & 1) return set_error(QJ_EVALUE, "clifford_rowsum: row %lld is both a source and a target",
+                                           (long long)i[j]);
+    }
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<int64_t> hi(2 * nrows);
+  std::memcpy(hi.data(), h, nrows * sizeof(int64_t));
+  std::memcpy(hi.data() + nrows, i, nrows * sizeof(int64_t));
+  void* dhi = nullptr;
+  int ticket = -1;
+  rc = ring_upload(hi.data(), hi.size() * sizeof(int64_t), s, &dhi, &ticket);
+  if (rc) return rc;
+  DeviceScratch* sc = nullptr;
+  rc = scratch_for_current_device(&sc);
+  if (!rc) rc = scratch_reserve_big(sc, (size_t)2 * nrows, s);
+  if (rc) {
+    ring_release(ticket, s);
+    return rc;
+  }
+  const int64_t* dh = static_cast<const int64_t*>(dhi);
+  const int64_t* di = dh + nrows;
+  uint8_t* rnew = static_cast<uint8_t*>(sc->big);
+  uint8_t* flip = rnew + nrows;
+  const int64_t ws = cliff_stride(nqubits);
+  k_cliff_pair_sign<<<dim3(blocks_for(nrows, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, dh, di, nrows, rnew,
+                                                                             flip);
+  if (determined)
+    k_cliff_pair_accumulate<<<dim3(blocks_for(C, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, h[0], di, nrows,
+                                                                                 rnew);
+  else
+    k_cliff_pair_apply<<<dim3(blocks_for(nrows * C, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, dh, di, nrows,
+                                                                                    flip);
+  rc = check_launch("clifford_rowsum launch");
+  ring_release(ticket, s);
+  return rc;
+}
+
+int qj_packbits_rows(const uint8_t* in, int64_t rows, int64_t cols, uint8_t* out, void* stream) {
+  if (rows < 0 || cols < 0) return set_error(QJ_EVALUE, "packbits_rows: negative size");
+  if (rows == 0 || cols == 0) return QJ_OK;
+  if (!in || !out) return set_error(QJ_EVALUE, "packbits_rows: null pointer");
+  k_packbits_rows<<<dim3(blocks_for(rows * ((cols + 7) >> 3), kBlock)), dim3(kBlock), 0, (hipStream_t)stream>>>(
+      in, rows, cols, out);
+  return check_launch("packbits_rows launch");
+}
+
+int qj_unpackbits_rows(const uint8_t* in, int64_t rows, int64_t packed_cols, int64_t count, uint8_t* out,
+                       void* stream) {
+  if (rows < 0 || packed_cols < 0 || count < 0) return set_error(QJ_EVALUE, "unpackbits_rows: negative size");
+  if (count > 8 * packed_cols) return set_error(QJ_EVALUE, "unpackbits_rows: count exceeds the packed bits");
+  if (rows == 0 || count == 0) return QJ_OK;
+  if (!in || !out) return set_error(QJ_EVALUE, "unpackbits_rows: null pointer");
+  k_unpackbits_rows<<<dim3(blocks_for(rows * count, kBlock)), dim3(kBlock), 0, (hipStream_t)stream>>>(
+      in, rows, packed_cols, count, out);
+  return check_launch("unpackbits_rows launch");
+}
+
+}  // extern "C"

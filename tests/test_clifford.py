@@ -1,0 +1,256 @@
+"""Stabilizer-tableau oracle pinned by brute force, and the host side of the
+Clifford engine (no GPU compute calls).
+
+The oracle's gate rules are checked against U P U^dagger of each gate's
+textbook unitary on every row of random tableaux; rowsum against the matrix
+product of the two rows' Pauli strings; the measurement functions against
+the state vector of the same Clifford circuit.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import clifford_ref as cr
+from qibojit_amd import _native
+
+I2 = np.eye(2, dtype=complex)
+PX = np.array([[0, 1], [1, 0]], dtype=complex)
+PY = np.array([[0, -1j], [1j, 0]])
+PZ = np.diag([1, -1]).astype(complex)
+PAULI = {(0, 0): I2, (1, 0): PX, (1, 1): PY, (0, 1): PZ}
+
+
+def _ry(t):
+    return np.array([[np.cos(t / 2), -np.sin(t / 2)], [np.sin(t / 2), np.cos(t / 2)]], dtype=complex)
+
+
+_SX = np.array([[1 + 1j, 1 - 1j], [1 - 1j, 1 + 1j]]) / 2
+ONE_QUBIT_U = {
+    "H": np.array([[1, 1], [1, -1]], dtype=complex) / np.sqrt(2),
+    "S": np.diag([1, 1j]),
+    "Z": PZ,
+    "X": PX,
+    "Y": PY,
+    "SX": _SX,
+    "SDG": np.diag([1, -1j]),
+    "SXDG": _SX.conj().T,
+    "RY_pi": _ry(np.pi / 2),
+    "RY_3pi_2": _ry(3 * np.pi / 2),
+}
+TWO_QUBIT_U = {
+    "CNOT": np.eye(4, dtype=complex)[[0, 1, 3, 2]],
+    "CZ": np.diag([1, 1, 1, -1]).astype(complex),
+    "SWAP": np.eye(4, dtype=complex)[[0, 2, 1, 3]],
+    "iSWAP": np.array([[1, 0, 0, 0], [0, 0, 1j, 0], [0, 1j, 0, 0], [0, 0, 0, 1]]),
+    "CY": np.block([[I2, np.zeros((2, 2))], [np.zeros((2, 2)), PY]]),
+}
+
+
+def pauli_of_row(row, n):
+    """(-1)^r prod_k P(x_k, z_k), qubit 0 the leftmost Kronecker factor."""
+    m = np.array([[1.0 + 0j]])
+    for k in range(n):
+        m = np.kron(m, PAULI[(int(row[k]), int(row[n + k]))])
+    return (-1) ** int(row[2 * n]) * m
+
+
+def embed(u, qubits, n):
+    """Full 2^n matrix of u acting on ``qubits`` (qubit 0 = most significant)."""
+    k = len(qubits)
+    dim = 2**n
+    full = np.zeros((dim, dim), dtype=complex)
+    for col in range(dim):
+        bits = [(col >> (n - 1 - q)) & 1 for q in range(n)]
+        sub = 0
+        for q in qubits:
+            sub = 2 * sub + bits[q]
+        for out in range(2**k):
+            amp = u[out, sub]
+            if amp == 0:
+                continue
+            nb = list(bits)
+            for j, q in enumerate(qubits):
+                nb[q] = (out >> (k - 1 - j)) & 1
+            idx = 0
+            for b in nb:
+                idx = 2 * idx + b
+            full[idx, col] += amp
+    return full
+
+
+def random_circuit(rng, n, ngates):
+    gates = []
+    for _ in range(ngates):
+        if n >= 2 and rng.random() < 0.4:
+            name = list(TWO_QUBIT_U)[rng.integers(len(TWO_QUBIT_U))]
+            gates.append((name, *[int(v) for v in rng.choice(n, 2, replace=False)]))
+        else:
+            name = list(ONE_QUBIT_U)[rng.integers(len(ONE_QUBIT_U))]
+            gates.append((name, int(rng.integers(n))))
+    return gates
+
+
+def run_circuit(gates, n):
+    sm = cr.zero_tableau(n)
+    psi = np.zeros(2**n, dtype=complex)
+    psi[0] = 1
+    for name, *qs in gates:
+        cr.apply(sm, name, tuple(qs), n)
+        u = ONE_QUBIT_U.get(name)
+        if u is None:
+            u = TWO_QUBIT_U[name]
+        psi = embed(u, qs, n) @ psi
+    return sm, psi
+
+
+@pytest.mark.parametrize("name", list(ONE_QUBIT_U) + list(TWO_QUBIT_U))
+@pytest.mark.parametrize("n", [2, 3])
+def test_gate_rule_is_conjugation(name, n):
+    rng = np.random.default_rng(abs(hash((name, n))) % 2**32)
+    u = ONE_QUBIT_U.get(name)
+    if u is None:
+        u = TWO_QUBIT_U[name]
+    k = 1 if name in ONE_QUBIT_U else 2
+    for _ in range(3):
+        qs = tuple(int(v) for v in rng.choice(n, size=k, replace=False))
+        sm = rng.integers(0, 2, size=(2 * n + 1, 2 * n + 1)).astype(np.uint8)
+        out = cr.apply(sm.copy(), name, qs, n)
+        uf = embed(u, qs, n)
+        for row in range(2 * n + 1):
+            np.testing.assert_allclose(uf @ pauli_of_row(sm[row], n) @ uf.conj().T, pauli_of_row(out[row], n),
+                                       atol=1e-12, err_msg=f"{name} on {qs}, row {row}")
+
+
+def test_planted_sign_slip_is_caught():
+    """A rule with its sign update dropped fails the conjugation check."""
+    n, q = 2, 1
+    sm = np.ones((2 * n + 1, 2 * n + 1), dtype=np.uint8)
+    out = sm.copy()
+    out[:, q], out[:, n + q] = sm[:, n + q], sm[:, q]  # H without r ^= x & z
+    uf = embed(ONE_QUBIT_U["H"], (q,), n)
+    assert not np.allclose(uf @ pauli_of_row(sm[0], n) @ uf.conj().T, pauli_of_row(out[0], n))
+
+
+def test_rowsum_sign_is_the_pauli_product():
+    rng = np.random.default_rng(5)
+    n = 3
+    checked = 0
+    while checked < 40:
+        sm = rng.integers(0, 2, size=(2 * n + 1, 2 * n + 1)).astype(np.uint8)
+        h, i = 1, 4
+        ph, pi_ = pauli_of_row(sm[h], n), pauli_of_row(sm[i], n)
+        if not np.allclose(ph @ pi_, pi_ @ ph):
+            continue  # tableau rows commute
+        out = cr.rowsum(sm.copy(), [h], [i], n)
+        np.testing.assert_allclose(pauli_of_row(out[h], n), pi_ @ ph, atol=1e-12)
+        np.testing.assert_array_equal(np.delete(out, h, axis=0), np.delete(sm, h, axis=0))
+        checked += 1
+
+
+def test_rowsum_determined_on_zero_row_is_xor():
+    rng = np.random.default_rng(6)
+    n = 4
+    sm = rng.integers(0, 2, size=(2 * n + 1, 2 * n + 1)).astype(np.uint8)
+    sm[-1] = 0
+    src = [5, 6, 7]
+    out = cr.rowsum(sm.copy(), [2 * n] * 3, src, n, determined=True)
+    np.testing.assert_array_equal(out[-1], sm[5] ^ sm[6] ^ sm[7])
+    np.testing.assert_array_equal(out[:-1], sm[:-1])
+
+
+def test_measurement_against_state_vector():
+    rng = np.random.default_rng(11)
+    nrandom = ndetermined = 0
+    for _ in range(120):
+        n = int(rng.integers(1, 5))
+        sm, psi = run_circuit(random_circuit(rng, n, int(rng.integers(0, 20))), n)
+        for row in range(n, 2 * n):
+            np.testing.assert_allclose(pauli_of_row(sm[row], n) @ psi, psi, atol=1e-12)
+        q = int(rng.integers(n))
+        one = ((np.arange(2**n) >> (n - 1 - q)) & 1) == 1
+        p1 = float(np.sum(np.abs(psi[one]) ** 2))
+        p = cr.first_stabilizer(sm, q, n)
+        if p >= 0:
+            outcome = int(rng.integers(2))
+            post, got = cr.random_outcome(sm.copy(), p, q, n, outcome)
+            assert got == outcome and abs(p1 - 0.5) < 1e-12
+            proj = psi * (one if outcome else ~one)
+            proj /= np.linalg.norm(proj)
+            for row in range(n, 2 * n):
+                np.testing.assert_allclose(pauli_of_row(post[row], n) @ proj, proj, atol=1e-12)
+            nrandom += 1
+        else:
+            _, got = cr.determined_outcome(sm.copy(), q, n)
+            if (got, p1) != (1, 0.0):  # see the next test for the reference's exception
+                assert abs(p1 - got) < 1e-12
+            ndetermined += 1
+    assert nrandom > 20 and ndetermined > 20
+
+
+# The reference's determined rowsum XORs the selected stabilizers in
+# parallel against the zeroed scratch row, so the phases between those rows
+# (the i's of their products) are not accumulated.  On this circuit (found by
+# a search over random 3-qubit circuits) qubit 1 is |0> with certainty but the
+# reference's rule answers 1; the oracle and the engine reproduce the rule.
+REFERENCE_DETERMINED_CASE = (3, [("CY", 0, 2), ("CNOT", 1, 2), ("Z", 2), ("H", 1), ("SX", 0), ("X", 1),
+                                 ("CZ", 0, 1), ("SWAP", 1, 2)], 1)
+
+
+def test_reference_determined_rule_case():
+    n, gates, q = REFERENCE_DETERMINED_CASE
+    sm, psi = run_circuit(gates, n)
+    assert cr.first_stabilizer(sm, q, n) == -1
+    one = ((np.arange(2**n) >> (n - 1 - q)) & 1) == 1
+    assert np.sum(np.abs(psi[one]) ** 2) < 1e-12
+    _, got = cr.determined_outcome(sm.copy(), q, n)
+    assert got == 1
+
+
+# ---- host side of the engine ----
+
+def test_encode_gates():
+    from qibojit_amd.backends import clifford_operations_hip as ch
+
+    arr = ch.encode_gates([("H", 2), ("CNOT", 0, 3), ("RY_3pi_2", 1)])
+    np.testing.assert_array_equal(arr, [[0, 2, -1], [10, 0, 3], [9, 1, -1]])
+    with pytest.raises(ValueError, match="not a Clifford gate"):
+        ch.encode_gates([("T", 0)])
+    with pytest.raises(ValueError, match="takes 2"):
+        ch.encode_gates([("CZ", 0)])
+
+
+def _fake():
+    return ctypes.c_void_p(0x1000)  # never dereferenced: validation fails first
+
+
+def test_clifford_abi_validation():
+    lib = _native.lib()
+    assert lib.qj_clifford_column_words(1) == 4
+    assert lib.qj_clifford_column_words(64) == 8  # 129 rows -> 5 words -> 8
+    assert lib.qj_clifford_column_words(0) == -1
+    with pytest.raises(ValueError, match="bad kind"):
+        _native.call("qj_clifford_apply_gate", _fake(), 4, 15, 0, 1, None)
+    with pytest.raises(ValueError, match="out of range"):
+        _native.call("qj_clifford_apply_gate", _fake(), 4, 0, 4, -1, None)
+    with pytest.raises(ValueError, match="both"):
+        _native.call("qj_clifford_apply_gate", _fake(), 4, 10, 2, 2, None)
+    gates = np.array([[10, 1, 1]], dtype=np.int32)
+    with pytest.raises(ValueError, match="twice"):
+        _native.call("qj_clifford_apply_circuit", _fake(), 4, gates.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                     1, None)
+    h = np.array([1, 1], dtype=np.int64)
+    i = np.array([5, 6], dtype=np.int64)
+    with pytest.raises(ValueError, match="repeated"):
+        _native.call("qj_clifford_rowsum", _fake(), 4, h.ctypes.data_as(ctypes.c_void_p),
+                     i.ctypes.data_as(ctypes.c_void_p), 2, 0, None)
+    h2 = np.array([1, 5], dtype=np.int64)
+    with pytest.raises(ValueError, match="both a source and a target"):
+        _native.call("qj_clifford_rowsum", _fake(), 4, h2.ctypes.data_as(ctypes.c_void_p),
+                     i.ctypes.data_as(ctypes.c_void_p), 2, 0, None)
+    with pytest.raises(ValueError, match="single target"):
+        _native.call("qj_clifford_rowsum", _fake(), 4, h2.ctypes.data_as(ctypes.c_void_p),
+                     i.ctypes.data_as(ctypes.c_void_p), 2, 1, None)
+    with pytest.raises(ValueError, match="outcome"):
+        _native.call("qj_clifford_random_outcome", _fake(), 4, 0, 0, 2, None)

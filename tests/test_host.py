@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "qjhip.h")
 
 def _declared_symbols():
     text = open(HEADER, encoding="utf-8").read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int)\s+(qj_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int64_t|int)\s+(qj_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_the_abi():
@@ -29,7 +29,8 @@ def test_library_exports_every_declared_symbol():
     lib = _native.lib()  # loads without a GPU
     for name in _declared_symbols():
         assert hasattr(lib, name), name
-    assert set(_native.SIGNATURES) | {"qj_last_error"} == set(_declared_symbols())
+    exported = set(_native.SIGNATURES) | set(_native.VALUE_SIGNATURES) | {"qj_last_error"}
+    assert exported == set(_declared_symbols())
 
 
 def test_abi_version_and_device_count():
