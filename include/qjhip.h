@@ -425,6 +425,17 @@ int qj_clifford_apply_gate(uint32_t* tab, int nqubits, int kind, int q0, int q1,
  * (kind, q0, q1).  The result equals the gates applied one by one in list
  * order.  No reference counterpart (the reference launches per gate). */
 int qj_clifford_apply_circuit(uint32_t* tab, int nqubits, const int32_t* gates, int64_t ngates, void* stream);
+/* The two halves of qj_clifford_apply_circuit, for a gate list applied many
+ * times.  qj_clifford_schedule orders the list on the host into groups of
+ * gates of one kind on disjoint qubits and writes *ngroups groups of
+ * QJ_CLIFF_GROUP_WORDS uint32 words to the host array `desc` (desc == NULL:
+ * only *ngroups; QJ_EVALUE if capacity, in groups, is smaller).
+ * qj_clifford_apply_schedule runs such an array, copied unchanged to device
+ * memory, on a tableau of the same nqubits.  No reference counterpart. */
+#define QJ_CLIFF_GROUP_WORDS 32
+int qj_clifford_schedule(const int32_t* gates, int64_t ngates, int nqubits, uint32_t* desc, int64_t capacity,
+                         int64_t* ngroups);
+int qj_clifford_apply_schedule(uint32_t* tab, int nqubits, const uint32_t* desc, int64_t ngroups, void* stream);
 /* *p_out (device int32) = first stabilizer p (0-based: row n+p) with x bit
  * 1 on qubit q, or -1: the test of qibo's measurement loop that chooses
  * between the random and the determined outcome. */

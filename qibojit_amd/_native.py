@@ -70,6 +70,8 @@ SIGNATURES = {
     "qj_clifford_unpack": [_vp, _i32, _vp, _vp],
     "qj_clifford_apply_gate": [_vp, _i32, _i32, _i32, _i32, _vp],
     "qj_clifford_apply_circuit": [_vp, _i32, _pi32, _i64, _vp],
+    "qj_clifford_schedule": [_pi32, _i64, _i32, _vp, _i64, ctypes.POINTER(ctypes.c_int64)],
+    "qj_clifford_apply_schedule": [_vp, _i32, _vp, _i64, _vp],
     "qj_clifford_first_stabilizer": [_vp, _i32, _i32, _vp, _vp],
     "qj_clifford_random_outcome": [_vp, _i32, _i32, _i32, _i32, _vp],
     "qj_clifford_determined_outcome": [_vp, _i32, _i32, _vp, _vp],

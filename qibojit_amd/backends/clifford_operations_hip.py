@@ -30,6 +30,7 @@ from qibojit_amd import _native
 KINDS = {"H": 0, "S": 1, "Z": 2, "X": 3, "Y": 4, "SX": 5, "SDG": 6, "SXDG": 7, "RY_pi": 8, "RY_3pi_2": 9,
          "CNOT": 10, "CZ": 11, "SWAP": 12, "iSWAP": 13, "CY": 14}
 TWO_QUBIT = ("CNOT", "CZ", "SWAP", "iSWAP", "CY")
+GROUP_WORDS = 32  # QJ_CLIFF_GROUP_WORDS
 
 
 def _stream(t):
@@ -179,6 +180,43 @@ def apply_circuit(symplectic_matrix, gates, nqubits):
     ptr = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     _native.call("qj_clifford_apply_circuit", _ptr(symplectic_matrix), int(nqubits), ptr, arr.shape[0],
                  _stream(symplectic_matrix))
+    return symplectic_matrix
+
+
+class CliffordSchedule:
+    """A gate list ordered into groups once (``compile_circuit``), held in
+    device memory, applied with ``apply_schedule`` as often as needed."""
+
+    def __init__(self, desc, ngroups, ngates, nqubits):
+        self.desc = desc
+        self.ngroups = ngroups
+        self.ngates = ngates
+        self.nqubits = nqubits
+
+
+def compile_circuit(gates, nqubits):
+    """Orders a gate list (as for ``apply_circuit``) into its device schedule."""
+    arr = gates if isinstance(gates, np.ndarray) and gates.dtype == np.int32 else encode_gates(gates)
+    arr = np.ascontiguousarray(arr.reshape(-1, 3))
+    ptr = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    ngroups = ctypes.c_int64(0)
+    _native.call("qj_clifford_schedule", ptr, arr.shape[0], int(nqubits), None, 0, ctypes.byref(ngroups))
+    words = np.empty(max(1, ngroups.value) * GROUP_WORDS, dtype=np.uint32)
+    _native.call("qj_clifford_schedule", ptr, arr.shape[0], int(nqubits), words.ctypes.data_as(ctypes.c_void_p),
+                 ngroups.value, ctypes.byref(ngroups))
+    desc = torch.from_numpy(words.view(np.int32)).to(_device())
+    return CliffordSchedule(desc, ngroups.value, arr.shape[0], int(nqubits))
+
+
+def apply_schedule(symplectic_matrix, schedule, nqubits):
+    """Applies a ``CliffordSchedule`` in one launch."""
+    _check_tableau(symplectic_matrix, nqubits)
+    if schedule.nqubits != int(nqubits):
+        raise ValueError(f"schedule compiled for {schedule.nqubits} qubits, tableau has {nqubits}")
+    if schedule.desc.device != symplectic_matrix.device:
+        raise ValueError("schedule and tableau are on different devices")
+    _native.call("qj_clifford_apply_schedule", _ptr(symplectic_matrix), int(nqubits), _ptr(schedule.desc),
+                 schedule.ngroups, _stream(symplectic_matrix))
     return symplectic_matrix
 
 

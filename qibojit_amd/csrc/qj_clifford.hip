@@ -33,7 +33,7 @@ enum : int {
   kciSWAP = QJ_CLIFF_ISWAP, kcCY = QJ_CLIFF_CY, kcNop = 15
 };
 constexpr int kcFirstTwo = kcCNOT;
-constexpr int kCliffGroup = 8;          // gates per group (pairwise disjoint qubits)
+constexpr int kCliffGroup = 16;         // gates per group (pairwise disjoint qubits)
 constexpr int kCliffMaxQubits = 1 << 19;
 constexpr int kMaxColumnChunks = 64;   // column chunks of a sliced rowsum
 constexpr size_t kCliffLdsPerCu = 160 * 1024;
@@ -148,7 +148,8 @@ __global__ void k_cliff_gate(uint32_t* __restrict__ tab, int n, int64_t ws, int6
 // and the next group's words are read while the current group's columns
 // are in flight, so no group waits on global memory for its descriptors.
 constexpr int kDescWords = 2 * kCliffGroup;            // words per group
-constexpr int kDescChunk = 64;                          // groups per chunk
+static_assert(kDescWords == QJ_CLIFF_GROUP_WORDS, "descriptor words per group");
+constexpr int kDescChunk = 1024 / kDescWords;            // groups per chunk
 constexpr int kDescChunkWords = kDescChunk * kDescWords;  // 1024
 constexpr int kDescPerLane = kDescChunkWords / 64;      // 16
 constexpr size_t kDescRingBytes = 2 * kDescChunkWords * sizeof(uint32_t);
@@ -167,18 +168,25 @@ __device__ __forceinline__ void read_group(const uint32_t* p, uint32_t (&dw)[kDe
 }
 
 // One group of up to kCliffGroup gates of one kind on pairwise disjoint
-// qubits: every load first, then the rule slot by slot.  Slots past `cnt`
-// repeat slot 0 (same loads, same values stored) and their sign updates are
-// masked off, so the group runs without branches.  The next group's
-// descriptor words are read from LDS while this group's columns load.
-template <int KIND>
-__device__ __forceinline__ uint32_t run_group(uint32_t* col0, int64_t zoff, const int64_t (&oa)[kCliffGroup],
-                                              const int64_t (&ob)[kCliffGroup], int cnt, const uint32_t* cur, int g,
-                                              int ng, uint32_t (&dw)[kDescWords]) {
+// qubits, run with S >= cnt slots: every load first, then the rule slot by
+// slot.  Slots past `cnt` repeat slot 0 (same loads, same values stored) and
+// their sign updates are masked off, so the group runs without branches.
+// The next group's descriptor words are read from LDS while this group's
+// columns load.
+template <int KIND, int S>
+__device__ __forceinline__ uint32_t run_group(uint32_t* col0, int64_t zoff, int64_t cs, uint32_t qmax, int cnt,
+                                              const uint32_t* cur, int g, int ng, uint32_t (&dw)[kDescWords]) {
   constexpr bool two = KIND >= kcFirstTwo;
-  uint32_t a[kCliffGroup], b[kCliffGroup], c[kCliffGroup], d[kCliffGroup];
+  int64_t oa[S], ob[S];
 #pragma unroll
-  for (int j = 0; j < kCliffGroup; ++j) {
+  for (int j = 0; j < S; ++j) {
+    // clamped: a descriptor array from outside stays inside the tableau
+    oa[j] = (int64_t)std::min(__builtin_amdgcn_readfirstlane(dw[2 * j]) & kSlotMask, qmax) * cs;
+    ob[j] = (int64_t)std::min(__builtin_amdgcn_readfirstlane(dw[2 * j + 1]) & kSlotMask, qmax) * cs;
+  }
+  uint32_t a[S], b[S], c[S], d[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
     a[j] = col0[oa[j]];
     b[j] = col0[zoff + oa[j]];
     if (two) {
@@ -192,11 +200,21 @@ __device__ __forceinline__ uint32_t run_group(uint32_t* col0, int64_t zoff, cons
   if (g + 1 < ng) read_group(cur + (g + 1) * kDescWords, dw);
   uint32_t t = 0;
 #pragma unroll
-  for (int j = 0; j < kCliffGroup; ++j) {
+  for (int j = 0; j < S; ++j) {
     const uint32_t tj = cliff_apply(KIND, col0, oa[j], ob[j], zoff, a[j], b[j], c[j], d[j]);
     t ^= j < cnt ? tj : 0u;
   }
   return t;
+}
+
+// Smallest slot count that holds the group (2, 4, 8 or kCliffGroup).
+template <int KIND>
+__device__ __forceinline__ uint32_t run_sized(uint32_t* col0, int64_t zoff, int64_t cs, uint32_t qmax, int cnt,
+                                              const uint32_t* cur, int g, int ng, uint32_t (&dw)[kDescWords]) {
+  if (cnt <= 2) return run_group<KIND, 2>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw);
+  if (cnt <= 4) return run_group<KIND, 4>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw);
+  if (cnt <= 8) return run_group<KIND, 8>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw);
+  return run_group<KIND, kCliffGroup>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw);
 }
 
 template <bool LDS>
@@ -236,6 +254,7 @@ __global__ __launch_bounds__(64) void k_cliff_circuit(uint32_t* __restrict__ tab
   __syncthreads();
   const int64_t zoff = (int64_t)n * cs;
   const int64_t roff = 2 * (int64_t)n * cs;
+  const uint32_t qmax = (uint32_t)(n - 1);
   uint32_t r = col0[roff];
   for (int64_t ch = 0; ch < nchunks; ++ch) {
     const uint32_t* cur = ring + (ch & 1) * kDescChunkWords;
@@ -252,33 +271,25 @@ __global__ __launch_bounds__(64) void k_cliff_circuit(uint32_t* __restrict__ tab
     uint32_t dw[kDescWords];
     read_group(cur, dw);
     for (int g = 0; g < ng; ++g) {
-      const uint32_t h0 = __builtin_amdgcn_readfirstlane(dw[0]);
-      const uint32_t h1 = __builtin_amdgcn_readfirstlane(dw[1]);
-      const int kind = (int)(h0 >> 24);
-      const int cnt = (int)(h1 >> 24);
-      int64_t oa[kCliffGroup], ob[kCliffGroup];
-#pragma unroll
-      for (int j = 0; j < kCliffGroup; ++j) {
-        oa[j] = (int64_t)(__builtin_amdgcn_readfirstlane(dw[2 * j]) & kSlotMask) * cs;
-        ob[j] = (int64_t)(__builtin_amdgcn_readfirstlane(dw[2 * j + 1]) & kSlotMask) * cs;
-      }
+      const int kind = (int)(__builtin_amdgcn_readfirstlane(dw[0]) >> 24);
+      const int cnt = (int)(__builtin_amdgcn_readfirstlane(dw[1]) >> 24);
       uint32_t t;
       switch (kind) {
-        case kcH: t = run_group<kcH>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcS: t = run_group<kcS>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcZ: t = run_group<kcZ>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcX: t = run_group<kcX>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcY: t = run_group<kcY>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcSX: t = run_group<kcSX>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcSDG: t = run_group<kcSDG>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcSXDG: t = run_group<kcSXDG>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcRYpi: t = run_group<kcRYpi>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcRY3pi2: t = run_group<kcRY3pi2>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcCNOT: t = run_group<kcCNOT>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcCZ: t = run_group<kcCZ>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kcSWAP: t = run_group<kcSWAP>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        case kciSWAP: t = run_group<kciSWAP>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
-        default: t = run_group<kcCY>(col0, zoff, oa, ob, cnt, cur, g, ng, dw); break;
+        case kcH: t = run_sized<kcH>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcS: t = run_sized<kcS>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcZ: t = run_sized<kcZ>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcX: t = run_sized<kcX>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcY: t = run_sized<kcY>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcSX: t = run_sized<kcSX>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcSDG: t = run_sized<kcSDG>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcSXDG: t = run_sized<kcSXDG>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcRYpi: t = run_sized<kcRYpi>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcRY3pi2: t = run_sized<kcRY3pi2>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcCNOT: t = run_sized<kcCNOT>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcCZ: t = run_sized<kcCZ>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcSWAP: t = run_sized<kcSWAP>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kciSWAP: t = run_sized<kciSWAP>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        default: t = run_sized<kcCY>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
       }
       r ^= t;
     }
@@ -654,6 +665,43 @@ int schedule_gates(const int32_t* gates, int64_t ngates, int n, std::vector<uint
   return QJ_OK;
 }
 
+// Launches k_cliff_circuit on a descriptor array in device memory: LDS
+// slices when every workgroup is resident at once; otherwise the global
+// variant (its columns stay in L2), which measures as fast as the LDS one
+// from 2000 qubits up and needs no second round of workgroups.
+void launch_circuit(uint32_t* tab, int nqubits, const uint32_t* ddesc, int64_t ngroups, hipStream_t s) {
+  const int64_t W = cliff_words(nqubits), ws = cliff_stride(nqubits);
+  const int64_t C = 2 * (int64_t)nqubits + 1;
+  int cus = 0;
+  size_t max_lds = 0;
+  device_lds(&cus, &max_lds);
+  const size_t cap = std::min(kCliffLdsPerCu, std::max<size_t>(max_lds, 65536)) - kDescRingBytes;
+  int64_t T = (int64_t)(cap / (4 * (size_t)C));
+  const char* kern = std::getenv("QJ_CLIFFORD_KERNEL");
+  const bool force_global = kern && std::strcmp(kern, "global") == 0;
+  bool use_lds = T >= 1 && !force_global;
+  int64_t nwg = 0;
+  if (use_lds) {
+    T = std::min<int64_t>({T, 64, W});
+    nwg = (W + T - 1) / T;
+    T = (W + nwg - 1) / nwg;  // even split
+    const size_t bytes = kDescRingBytes + (size_t)T * C * 4;
+    const int64_t per_cu = std::max<int64_t>(1, (int64_t)(kCliffLdsPerCu / bytes));
+    use_lds = nwg <= (int64_t)cus * per_cu;
+  }
+  if (use_lds) {
+    const size_t lds = kDescRingBytes + (size_t)T * C * 4;
+    if (lds > 65536)
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cliff_circuit<true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_cliff_circuit<true><<<dim3((unsigned)nwg), dim3(64), lds, s>>>(tab, nqubits, ws, W, (int)T, ddesc, ngroups);
+  } else {
+    const int64_t gwg = (W + 63) / 64;
+    k_cliff_circuit<false><<<dim3((unsigned)gwg), dim3(64), kDescRingBytes, s>>>(tab, nqubits, ws, W, 64, ddesc,
+                                                                                ngroups);
+  }
+}
+
 }  // namespace
 }  // namespace qj
 
@@ -701,6 +749,37 @@ int qj_clifford_apply_gate(uint32_t* tab, int nqubits, int kind, int q0, int q1,
   return check_launch("clifford_apply_gate launch");
 }
 
+int qj_clifford_schedule(const int32_t* gates, int64_t ngates, int nqubits, uint32_t* desc, int64_t capacity,
+                         int64_t* ngroups) {
+  if (nqubits < 1 || nqubits > kCliffMaxQubits)
+    return set_error(QJ_EVALUE, "clifford_schedule: nqubits %d out of range", nqubits);
+  if (!ngroups) return set_error(QJ_EVALUE, "clifford_schedule: null group count");
+  if (ngates < 0) return set_error(QJ_EVALUE, "clifford_schedule: negative gate count");
+  if (ngates > 0 && !gates) return set_error(QJ_EVALUE, "clifford_schedule: null gate list");
+  std::vector<uint32_t> words;
+  if (ngates > 0) {
+    const int rc = schedule_gates(gates, ngates, nqubits, words);
+    if (rc) return rc;
+  }
+  *ngroups = (int64_t)words.size() / kDescWords;
+  if (!desc) return QJ_OK;  // size query
+  if (capacity < *ngroups)
+    return set_error(QJ_EVALUE, "clifford_schedule: %lld groups do not fit a capacity of %lld", (long long)*ngroups,
+                     (long long)capacity);
+  std::memcpy(desc, words.data(), words.size() * sizeof(uint32_t));
+  return QJ_OK;
+}
+
+int qj_clifford_apply_schedule(uint32_t* tab, int nqubits, const uint32_t* desc, int64_t ngroups, void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_apply_schedule");
+  if (rc) return rc;
+  if (ngroups < 0) return set_error(QJ_EVALUE, "clifford_apply_schedule: negative group count");
+  if (ngroups == 0) return QJ_OK;
+  if (!desc) return set_error(QJ_EVALUE, "clifford_apply_schedule: null schedule");
+  launch_circuit(tab, nqubits, desc, ngroups, (hipStream_t)stream);
+  return check_launch("clifford_apply_schedule launch");
+}
+
 int qj_clifford_apply_circuit(uint32_t* tab, int nqubits, const int32_t* gates, int64_t ngates, void* stream) {
   int rc = check_tab(tab, nqubits, "clifford_apply_circuit");
   if (rc) return rc;
@@ -710,46 +789,13 @@ int qj_clifford_apply_circuit(uint32_t* tab, int nqubits, const int32_t* gates, 
   std::vector<uint32_t> desc;
   rc = schedule_gates(gates, ngates, nqubits, desc);
   if (rc) return rc;
-  const int64_t ngroups = (int64_t)desc.size() / (2 * kCliffGroup);
+  const int64_t ngroups = (int64_t)desc.size() / kDescWords;
   hipStream_t s = (hipStream_t)stream;
   void* ddesc = nullptr;
   int ticket = -1;
   rc = ring_upload(desc.data(), desc.size() * sizeof(uint32_t), s, &ddesc, &ticket);
   if (rc) return rc;
-  const int64_t W = cliff_words(nqubits), ws = cliff_stride(nqubits);
-  const int64_t C = 2 * (int64_t)nqubits + 1;
-  int cus = 0;
-  size_t max_lds = 0;
-  device_lds(&cus, &max_lds);
-  const size_t cap = std::min(kCliffLdsPerCu, std::max<size_t>(max_lds, 65536)) - kDescRingBytes;
-  int64_t T = (int64_t)(cap / (4 * (size_t)C));
-  const char* kern = std::getenv("QJ_CLIFFORD_KERNEL");
-  const bool force_global = kern && std::strcmp(kern, "global") == 0;
-  // LDS slices when every workgroup is resident at once; otherwise the
-  // global variant (its columns stay in L2), which measures as fast as the
-  // LDS one from 2000 qubits up and needs no second round of workgroups.
-  bool use_lds = T >= 1 && !force_global;
-  int64_t nwg = 0;
-  if (use_lds) {
-    T = std::min<int64_t>({T, 64, W});
-    nwg = (W + T - 1) / T;
-    T = (W + nwg - 1) / nwg;  // even split
-    const size_t bytes = kDescRingBytes + (size_t)T * C * 4;
-    const int64_t per_cu = std::max<int64_t>(1, (int64_t)(kCliffLdsPerCu / bytes));
-    use_lds = nwg <= (int64_t)cus * per_cu;
-  }
-  if (use_lds) {
-    const size_t lds = kDescRingBytes + (size_t)T * C * 4;
-    if (lds > 65536)
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cliff_circuit<true>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    k_cliff_circuit<true><<<dim3((unsigned)nwg), dim3(64), lds, s>>>(tab, nqubits, ws, W, (int)T,
-                                                                     static_cast<const uint32_t*>(ddesc), ngroups);
-  } else {
-    const int64_t gwg = (W + 63) / 64;
-    k_cliff_circuit<false><<<dim3((unsigned)gwg), dim3(64), kDescRingBytes, s>>>(tab, nqubits, ws, W, 64,
-                                                                    static_cast<const uint32_t*>(ddesc), ngroups);
-  }
+  launch_circuit(tab, nqubits, static_cast<const uint32_t*>(ddesc), ngroups, s);
   rc = check_launch("clifford_apply_circuit launch");
   ring_release(ticket, s);
   return rc;

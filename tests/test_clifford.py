@@ -162,7 +162,7 @@ def test_rowsum_determined_on_zero_row_is_xor():
 
 def test_measurement_against_state_vector():
     rng = np.random.default_rng(11)
-    nrandom = ndetermined = 0
+    nrandom = ndetermined = mismatches = 0
     for _ in range(120):
         n = int(rng.integers(1, 5))
         sm, psi = run_circuit(random_circuit(rng, n, int(rng.integers(0, 20))), n)
@@ -183,10 +183,22 @@ def test_measurement_against_state_vector():
             nrandom += 1
         else:
             _, got = cr.determined_outcome(sm.copy(), q, n)
-            if (got, p1) != (1, 0.0):  # see the next test for the reference's exception
-                assert abs(p1 - got) < 1e-12
+            # the stabilizers the rule selects multiply to +-Z_q, whose sign is
+            # the state's certain outcome
+            sel = [n + i for i in range(n) if sm[i, q]]
+            prod = np.eye(2**n, dtype=complex)
+            for row in sel:
+                prod = prod @ pauli_of_row(sm[row], n)
+            zq = embed(PZ, (q,), n)
+            assert np.allclose(prod, zq) or np.allclose(prod, -zq)
+            certain = 0 if np.allclose(prod, zq) else 1
+            assert abs(p1 - certain) < 1e-12
+            # the reference's rule combines their signs by XOR (see below)
+            assert got == (int(np.bitwise_xor.reduce(sm[sel, -1])) if sel else 0)
+            mismatches += got != certain
             ndetermined += 1
     assert nrandom > 20 and ndetermined > 20
+    assert mismatches <= ndetermined // 10
 
 
 # The reference's determined rowsum XORs the selected stabilizers in
@@ -219,6 +231,47 @@ def test_encode_gates():
         ch.encode_gates([("T", 0)])
     with pytest.raises(ValueError, match="takes 2"):
         ch.encode_gates([("CZ", 0)])
+
+
+def _schedule(arr, n):
+    ngroups = ctypes.c_int64(0)
+    ptr = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    _native.call("qj_clifford_schedule", ptr, arr.shape[0], n, None, 0, ctypes.byref(ngroups))
+    words = np.zeros(ngroups.value * 32, dtype=np.uint32)
+    _native.call("qj_clifford_schedule", ptr, arr.shape[0], n, words.ctypes.data_as(ctypes.c_void_p), ngroups.value,
+                 ctypes.byref(ngroups))
+    return words.reshape(ngroups.value, 16, 2)
+
+
+@pytest.mark.parametrize("n,ngates", [(1, 40), (3, 300), (9, 500), (40, 800)])
+def test_schedule_groups_reproduce_the_gate_list(n, ngates):
+    """The host scheduler's groups (one kind, disjoint qubits), applied in
+    group order by the oracle, equal the gate list applied one by one."""
+    from qibojit_amd.backends import clifford_operations_hip as ch
+
+    rng = np.random.default_rng(n * ngates)
+    gates = random_circuit(rng, n, ngates)
+    groups = _schedule(ch.encode_gates(gates), n)
+    names = {v: k for k, v in ch.KINDS.items()}
+    mask = (1 << 24) - 1
+    sm0 = rng.integers(0, 2, size=(2 * n + 1, 2 * n + 1)).astype(np.uint8)
+    got = sm0.copy()
+    placed = 0
+    for grp in groups:
+        kind, cnt = int(grp[0, 0] >> 24), int(grp[0, 1] >> 24)
+        assert 1 <= cnt <= 16
+        slots = [(int(a & mask), int(b & mask)) for a, b in grp]
+        assert all(s == slots[0] for s in slots[cnt:])  # padding repeats slot 0
+        used = [q for a, b in slots[:cnt] for q in ({a, b} if kind >= 10 else {a})]
+        assert len(used) == len(set(used))  # disjoint qubits
+        for a, b in slots[:cnt]:
+            cr.apply(got, names[kind], (a, b) if kind >= 10 else (a,), n)
+        placed += cnt
+    assert placed == ngates
+    want = sm0.copy()
+    for name, *qs in gates:
+        cr.apply(want, name, tuple(qs), n)
+    np.testing.assert_array_equal(got, want)
 
 
 def _fake():

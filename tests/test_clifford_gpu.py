@@ -91,6 +91,22 @@ def test_circuit_matches_gate_by_gate(n, ngates, kernel, monkeypatch):
     np.testing.assert_array_equal(_to_host(tab, n), _oracle_run(sm.copy(), gates, n))
 
 
+def test_compiled_schedule_applied_twice():
+    rng = np.random.default_rng(44)
+    n = 60
+    gates = _random_gates(rng, n, 2000)
+    sm = _dense(rng, n)
+    sched = ch.compile_circuit(gates, n)
+    assert sched.ngates == 2000 and sched.ngroups >= 2000 // 16
+    tab = ch._clifford_pre_execution_reshape(sm)
+    ch.apply_schedule(tab, sched, n)
+    ch.apply_schedule(tab, sched, n)
+    want = _oracle_run(_oracle_run(sm.copy(), gates, n), gates, n)
+    np.testing.assert_array_equal(_to_host(tab, n), want)
+    with pytest.raises(ValueError, match="compiled for"):
+        ch.apply_schedule(ch.zero_state(n + 1), sched, n + 1)
+
+
 def test_circuit_of_dependent_chain():
     """Gates that all share a qubit: one gate per group, order kept."""
     rng = np.random.default_rng(3)
