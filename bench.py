@@ -670,13 +670,15 @@ def _clifford_gates(n, ngates, seed):
 
 
 def run_clifford(sizes=((100, 100000), (1000, 100000), (10000, 20000)), reps=3, nmeas=64, cpu_seconds=2.0,
-                 seed=77):
+                 seed=77, nshots=100000, loop_shots=5):
     """Stabilizer tableaux: a seeded random Clifford gate list from |0...0>
     applied in one launch (k_cliff_circuit; schedule compiled once, launches
     timed with HIP events on the launch stream), the same list one launch per
     gate (k_cliff_gate, as the reference launches), the oracle's per-gate
     NumPy rules on the host (bounded sample), bit-exact check of the launch
-    against the oracle, and collapsing measurements of `nmeas` qubits."""
+    against the oracle, collapsing measurements of `nmeas` qubits, and
+    `nshots` shots of 32 qubits (affine-map sampling) against the
+    shot-by-shot measurement loop."""
     import torch
 
     from oracle import clifford_ref as cr
@@ -759,6 +761,23 @@ def run_clifford(sizes=((100, 100000), (1000, 100000), (10000, 20000)), reps=3, 
         torch.cuda.synchronize()
         rec["measure_seconds_per_qubit"] = (time.perf_counter() - t0) / len(qubits)
         rec["measured_ones"] = int(sum(sample))
+        # many shots of 32 qubits not measured above: affine-map sampling vs
+        # the shot-by-shot loop
+        measured = set(qubits)
+        sq = [q for q in range(n) if q not in measured][:32]
+        np.random.seed(seed)
+        t0 = time.perf_counter()
+        shots = ch.sample_shots(tab, sq, n, nshots)
+        torch.cuda.synchronize()
+        rec["sample_shots"] = {"qubits": len(sq), "shots": nshots, "seconds": time.perf_counter() - t0}
+        t0 = time.perf_counter()
+        for _ in range(loop_shots):
+            ch.M(tab, sq, n)
+        torch.cuda.synchronize()
+        per_shot = (time.perf_counter() - t0) / loop_shots
+        rec["sample_shots"]["shot_by_shot_seconds_per_shot"] = per_shot
+        rec["sample_shots"]["speedup_vs_shot_by_shot"] = per_shot * nshots / rec["sample_shots"]["seconds"]
+        rec["sample_shots"]["mean"] = float(shots.float().mean())
         out.append(rec)
         del tab
         torch.cuda.empty_cache()

@@ -460,6 +460,16 @@ int qj_clifford_determined_outcome(uint32_t* tab, int nqubits, int q, int32_t* o
  * with every (row i[j] XOR row h[0]) and its sign with every new sign. */
 int qj_clifford_rowsum(uint32_t* tab, int nqubits, const int64_t* h, const int64_t* i, int64_t nrows,
                        int determined, void* stream);
+/* Many measurement shots of one tableau: for a fixed state the measured
+ * bits are affine over GF(2) in the random outcomes drawn on the way, so
+ * shot s is s0 ^ (u[s] . D).  out[s * m + q] = s0[q] ^ parity(u[s] & dt[q])
+ * with u (nshots x kw uint64 words: shot s's random outcome bits, bit j of
+ * the row = j-th random outcome), dt (m x kw words: row q = the outcomes
+ * whose flip flips measured qubit q), s0 (m bytes: the sample with every
+ * random outcome 0); all device arrays, out uint8 (nshots x m).  No
+ * reference counterpart (qibo samples a Clifford state shot by shot). */
+int qj_gf2_samples(const uint64_t* u, const uint64_t* dt, const uint8_t* s0, int64_t nshots, int64_t m, int64_t kw,
+                   uint8_t* out, void* stream);
 /* numpy.packbits / numpy.unpackbits(count=...) along the last axis of a
  * row-major byte matrix (device), big bit order: the reference's _packbits /
  * _unpackbits (clifford_operations_cpu.py:256-327). */

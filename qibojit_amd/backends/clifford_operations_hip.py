@@ -283,6 +283,65 @@ def M(state, qubits, nqubits, collapse=False):
     return sample
 
 
+def _measure_forced(state, qubits, nqubits, flips):
+    """The measurement loop of ``M`` on ``state`` (collapsed in place) with
+    the random outcomes given instead of drawn: the t-th random outcome is 1
+    iff t is in ``flips``.  Returns the sample and the positions (indices
+    into ``qubits``) of the random outcomes."""
+    sample, random_at = [], []
+    for pos, q in enumerate(qubits):
+        p = first_stabilizer(state, q, nqubits)
+        if p >= 0:
+            bit = int(len(random_at) in flips)
+            _native.call("qj_clifford_random_outcome", _ptr(state), int(nqubits), p, int(q), bit, _stream(state))
+            random_at.append(pos)
+            sample.append(bit)
+        else:
+            _, outcome = _determined_outcome(state, q, nqubits)
+            sample.append(int(outcome))
+    return np.array(sample, dtype=np.uint8), random_at
+
+
+def sample_shots(state, qubits, nqubits, nshots):
+    """``nshots`` measurements of ``qubits`` on copies of ``state`` (left
+    unchanged): a (nshots, len(qubits)) uint8 device tensor equal to ``nshots``
+    calls of ``M(state, qubits, nqubits)``, with the same draws from NumPy's
+    generator (one per random outcome, shot after shot).
+
+    Which measurements are random, and every x/z bit on the way, do not
+    depend on the outcomes; the signs, and with them every measured bit, are
+    affine over GF(2) in the random outcomes.  So one run with every random
+    outcome 0 (s0) and one run per random outcome j with only j set give the
+    map, and shot s is s0 ^ (u_s . D), evaluated by ``qj_gf2_samples``:
+    len(qubits) x (k + 1) measurements for any ``nshots``, k the number of
+    random outcomes."""
+    _check_tableau(state, nqubits)
+    qubits = [int(q) for q in qubits]
+    m, nshots = len(qubits), int(nshots)
+    s0, random_at = _measure_forced(state.clone(), qubits, nqubits, set())
+    k = len(random_at)
+    kw = max(1, (k + 63) // 64)
+    dt = np.zeros((m, kw), dtype=np.uint64)
+    for j in range(k):
+        sj, at = _measure_forced(state.clone(), qubits, nqubits, {j})
+        if at != random_at:
+            raise RuntimeError("clifford sampling: random outcomes moved between runs")
+        flip = np.nonzero(sj ^ s0)[0]
+        dt[flip, j // 64] |= np.uint64(1) << np.uint64(j % 64)
+    draws = np.random.randint(2, size=(nshots, k), dtype=np.uint).astype(np.uint8)
+    u = np.zeros((nshots, kw * 8), dtype=np.uint8)
+    if k:
+        packed = np.packbits(draws, axis=1, bitorder="little")
+        u[:, :packed.shape[1]] = packed
+    dev = state.device
+    u_d = torch.from_numpy(u.view(np.int64).reshape(nshots, kw)).to(dev)
+    dt_d = torch.from_numpy(dt.view(np.int64)).to(dev)
+    s0_d = torch.from_numpy(s0).to(dev)
+    out = torch.empty((nshots, m), dtype=torch.uint8, device=dev)
+    _native.call("qj_gf2_samples", _ptr(u_d), _ptr(dt_d), _ptr(s0_d), nshots, m, kw, _ptr(out), _stream(state))
+    return out
+
+
 # ---- numpy-format bit packing (clifford_operations_cpu.py:250-335) ----
 
 def _packed_size(n):

@@ -541,6 +541,26 @@ __global__ void k_unpackbits_rows(const uint8_t* __restrict__ in, int64_t rows, 
   }
 }
 
+// ---- many shots: samples as affine functions of the random outcomes ----
+
+// out[s, q] = s0[q] ^ parity(u[s] & dt[q]): u holds each shot's random
+// outcome bits (kw 64-bit words), dt row q the outcomes that flip measured
+// qubit q.  Thread per (shot, measured qubit).
+__global__ void k_gf2_samples(const uint64_t* __restrict__ u, const uint64_t* __restrict__ dt,
+                              const uint8_t* __restrict__ s0, int64_t nshots, int64_t m, int64_t kw,
+                              uint8_t* __restrict__ out) {
+  const int64_t total = nshots * m;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = e / m, q = e - s * m;
+    const uint64_t* us = u + s * kw;
+    const uint64_t* dq = dt + q * kw;
+    uint64_t acc = 0;
+    for (int64_t w = 0; w < kw; ++w) acc ^= us[w] & dq[w];
+    out[e] = (uint8_t)((s0[q] ^ __builtin_popcountll(acc)) & 1u);
+  }
+}
+
 // ---- host side ----
 
 int check_tab(const void* tab, int n, const char* what) {
@@ -914,6 +934,16 @@ int qj_clifford_rowsum(uint32_t* tab, int nqubits, const int64_t* h, const int64
   rc = check_launch("clifford_rowsum launch");
   ring_release(ticket, s);
   return rc;
+}
+
+int qj_gf2_samples(const uint64_t* u, const uint64_t* dt, const uint8_t* s0, int64_t nshots, int64_t m, int64_t kw,
+                   uint8_t* out, void* stream) {
+  if (nshots < 0 || m < 0 || kw < 0) return set_error(QJ_EVALUE, "gf2_samples: negative size");
+  if (nshots == 0 || m == 0) return QJ_OK;
+  if (!s0 || !out || (kw > 0 && (!u || !dt))) return set_error(QJ_EVALUE, "gf2_samples: null pointer");
+  k_gf2_samples<<<dim3(blocks_for(nshots * m, kBlock)), dim3(kBlock), 0, (hipStream_t)stream>>>(u, dt, s0, nshots, m,
+                                                                                              kw, out);
+  return check_launch("gf2_samples launch");
 }
 
 int qj_packbits_rows(const uint8_t* in, int64_t rows, int64_t cols, uint8_t* out, void* stream) {

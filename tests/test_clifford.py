@@ -274,6 +274,18 @@ def test_schedule_groups_reproduce_the_gate_list(n, ngates):
     np.testing.assert_array_equal(got, want)
 
 
+def test_vector_draws_equal_scalar_draws():
+    """sample_shots draws its random outcomes as one (nshots, k) array: the
+    legacy generator gives the same bits, and the same state after, as the
+    scalar draws of the shot-by-shot loop."""
+    np.random.seed(12)
+    scalar = [np.random.randint(2, size=None, dtype=np.uint) for _ in range(600)]
+    after = np.random.random_sample()
+    np.random.seed(12)
+    vector = np.random.randint(2, size=(20, 30), dtype=np.uint).ravel()
+    assert np.array_equal(scalar, vector) and np.random.random_sample() == after
+
+
 def _fake():
     return ctypes.c_void_p(0x1000)  # never dereferenced: validation fails first
 

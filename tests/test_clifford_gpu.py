@@ -236,6 +236,36 @@ def test_non_collapsing_measurement_keeps_the_state():
     assert torch.equal(tab, before)
 
 
+@pytest.mark.parametrize("n,nmeas", [(1, 1), (5, 5), (24, 20), (90, 40)])
+def test_sample_shots_equals_shot_by_shot(n, nmeas):
+    """Affine-map sampling == nshots measurement loops on copies, draw for draw."""
+    rng = np.random.default_rng(n)
+    tab = ch.zero_state(n)
+    ch.apply_circuit(tab, random_circuit(rng, n, 10 * n), n)
+    qubits = [int(v) for v in rng.permutation(n)[:nmeas]]
+    before = tab.clone()
+    np.random.seed(7)
+    got = ch.sample_shots(tab, qubits, n, 25).cpu().numpy()
+    _, keys, pos, *_ = np.random.get_state()
+    assert torch.equal(tab, before)
+    np.random.seed(7)
+    want = np.array([ch.M(tab, qubits, n) for _ in range(25)], dtype=np.uint8)
+    np.testing.assert_array_equal(got, want)
+    _, keys2, pos2, *_ = np.random.get_state()
+    assert np.array_equal(keys, keys2) and pos == pos2
+
+
+def test_sample_shots_of_a_ghz_state():
+    """GHZ on 40 qubits: every shot is all zeros or all ones, about half each."""
+    n = 40
+    tab = ch.zero_state(n)
+    ch.apply_circuit(tab, [("H", 0)] + [("CNOT", 0, q) for q in range(1, n)], n)
+    np.random.seed(3)
+    shots = ch.sample_shots(tab, list(range(n)), n, 4000).cpu().numpy()
+    assert np.all((shots.sum(axis=1) == 0) | (shots.sum(axis=1) == n))
+    assert 1800 < int(shots[:, 0].sum()) < 2200
+
+
 def test_reference_determined_rule_case():
     n, gates, q = REFERENCE_DETERMINED_CASE
     tab = ch.zero_state(n)
