@@ -50,6 +50,7 @@ def parse():
     p.add_argument("--only-batch", action="store_true", help="many-circuit ansatz benchmark only")
     p.add_argument("--only-precision", action="store_true", help="complex64 vs complex128 runs and the MFMA kernel")
     p.add_argument("--only-clifford", action="store_true", help="stabilizer-tableau benchmark only")
+    p.add_argument("--only-mfma", action="store_true", help="the 5-qubit block kernels alone (for counter runs)")
     return p.parse_args()
 
 
@@ -861,6 +862,9 @@ def main():
         return
     if args.only_batch:
         print(json.dumps({"batch_1024": run_batch(backend)}), flush=True)
+        return
+    if args.only_mfma:
+        print(json.dumps({"mfma_kernel_32q": run_mfma_kernel(backend)}), flush=True)
         return
     if args.only_clifford:
         print(json.dumps({"clifford": run_clifford()}), flush=True)
