@@ -457,7 +457,8 @@ class HipBackend(Backend):
                  mat.ctypes.data_as(ctypes.c_void_p), self._stream())
             del tkeep
             return state
-        tptr, tkeep, nt = i32_array([nqubits - t - 1 for t in targets])
+        tbits = [nqubits - t - 1 for t in targets]
+        tptr, tkeep, nt = i32_array(tbits)
         qptr, qkeep, nq = i32_array(qseq)
         call(
             "qj_apply_multi_qubit",

@@ -162,6 +162,15 @@ def test_apply_multiqubit_gate_large(backend, port_backend, n, targets, controls
     _check(backend, port_backend, gates.Unitary(u, *targets).controlled_by(*controls), n, dtype, rng)
 
 
+@pytest.mark.parametrize("targets,controls", [([21, 20, 19, 18, 17], []), ([17, 21, 18, 20], [3]),
+                                              ([19, 20, 21, 2, 9], [0, 16]), ([20, 5, 21, 8], [])])
+def test_low_target_blocks(backend, port_backend, targets, controls, dtype, rng):
+    """4/5 targets among the lowest bits (inside one cache line)."""
+    k = len(targets)
+    u = gm.haar_unitary(2**k, rng)
+    _check(backend, port_backend, gates.Unitary(u, *targets).controlled_by(*controls), 22, dtype, rng)
+
+
 @pytest.mark.parametrize("n,targets", cases.MULTI_QUBIT_BASE)
 @pytest.mark.parametrize("use_qubits", [False, True])
 def test_apply_multi_qubit_base(backend, port_backend, n, targets, use_qubits, dtype, rng):
