@@ -779,6 +779,18 @@ def run_clifford(sizes=((100, 100000), (1000, 100000), (10000, 20000)), reps=3, 
         rec["sample_shots"]["shot_by_shot_seconds_per_shot"] = per_shot
         rec["sample_shots"]["speedup_vs_shot_by_shot"] = per_shot * nshots / rec["sample_shots"]["seconds"]
         rec["sample_shots"]["mean"] = float(shots.float().mean())
+        # every qubit up to 1000 of them, 10000 shots: one symbolic pass
+        wide = list(range(min(n, 1000)))
+        np.random.seed(seed)
+        t0 = time.perf_counter()
+        forms, k = ch.measurement_forms(tab, wide, n)
+        t_forms = time.perf_counter() - t0
+        np.random.seed(seed)
+        t0 = time.perf_counter()
+        shots = ch.sample_shots(tab, wide, n, 10000)
+        torch.cuda.synchronize()
+        rec["sample_shots_wide"] = {"qubits": len(wide), "random_outcomes": k, "shots": 10000,
+                                    "forms_seconds": t_forms, "seconds": time.perf_counter() - t0}
         out.append(rec)
         del tab
         torch.cuda.empty_cache()

@@ -460,6 +460,22 @@ int qj_clifford_determined_outcome(uint32_t* tab, int nqubits, int q, int32_t* o
  * with every (row i[j] XOR row h[0]) and its sign with every new sign. */
 int qj_clifford_rowsum(uint32_t* tab, int nqubits, const int64_t* h, const int64_t* i, int64_t nrows,
                        int determined, void* stream);
+/* Symbolic measurement pass (the map of qj_gf2_samples in one pass).
+ * `signs` (device, (nvars + 1) * ws words) holds sign columns: column j of
+ * row r = the coefficient of term j in row r's sign as an affine form over
+ * GF(2) of the random outcomes (j = 0 the constant, j >= 1 variable j).
+ * qj_clifford_sym_init sets column 0 to the tableau's sign column and the
+ * others to 0.  qj_clifford_sym_random is qj_clifford_random_outcome on the
+ * x/z bits of `tab`, with the signs carried as forms and row n+p's sign set
+ * to variable `var` (1 <= var <= nvars < 65535).  qj_clifford_sym_determined
+ * writes the form of qibo's determined outcome on q (the XOR of the selected
+ * stabilizers' signs) to form[0..nvars] (device bytes).  The tableau's own
+ * sign column is not maintained.  No reference counterpart. */
+int qj_clifford_sym_init(const uint32_t* tab, int nqubits, uint32_t* signs, int nvars, void* stream);
+int qj_clifford_sym_random(uint32_t* tab, int nqubits, uint32_t* signs, int nvars, int p, int q, int var,
+                           void* stream);
+int qj_clifford_sym_determined(const uint32_t* tab, int nqubits, const uint32_t* signs, int nvars, int q,
+                               uint8_t* form, void* stream);
 /* Many measurement shots of one tableau: for a fixed state the measured
  * bits are affine over GF(2) in the random outcomes drawn on the way, so
  * shot s is s0 ^ (u[s] . D).  out[s * m + q] = s0[q] ^ parity(u[s] & dt[q])

@@ -302,41 +302,62 @@ def _measure_forced(state, qubits, nqubits, flips):
     return np.array(sample, dtype=np.uint8), random_at
 
 
+def measurement_forms(state, qubits, nqubits):
+    """Affine forms over GF(2) of the bits ``M(state, qubits, nqubits)``
+    would return: a (len(qubits), k + 1) uint8 array F and k, the number of
+    random outcomes, such that the sample for random outcomes u_1..u_k is
+    F[:, 0] ^ (F[:, 1:] . u mod 2).  One symbolic measurement pass (the
+    state is left unchanged).
+
+    For a fixed state, which measurements are random, and every x/z bit on
+    the way, do not depend on the outcomes (``first_stabilizer`` and the
+    rowsum's x/z updates read x/z only).  The signs do, affinely, so they are
+    carried as forms: sign column j of the device buffer holds every row's
+    coefficient of term j."""
+    _check_tableau(state, nqubits)
+    qubits = [int(q) for q in qubits]
+    m, n = len(qubits), int(nqubits)
+    tab = state.clone()
+    signs = torch.empty(((m + 1), column_words(n)), dtype=torch.int32, device=state.device)
+    forms = torch.zeros((m, m + 1), dtype=torch.uint8, device=state.device)
+    stream = _stream(state)
+    _native.call("qj_clifford_sym_init", _ptr(tab), n, _ptr(signs), m, stream)
+    k = 0
+    for i, q in enumerate(qubits):
+        p = first_stabilizer(tab, q, n)
+        if p >= 0:
+            k += 1
+            _native.call("qj_clifford_sym_random", _ptr(tab), n, _ptr(signs), m, p, q, k, stream)
+            forms[i, k] = 1
+        else:
+            _native.call("qj_clifford_sym_determined", _ptr(tab), n, _ptr(signs), m, q, _ptr(forms[i]), stream)
+    return forms[:, : k + 1].cpu().numpy(), k
+
+
 def sample_shots(state, qubits, nqubits, nshots):
     """``nshots`` measurements of ``qubits`` on copies of ``state`` (left
     unchanged): a (nshots, len(qubits)) uint8 device tensor equal to ``nshots``
     calls of ``M(state, qubits, nqubits)``, with the same draws from NumPy's
-    generator (one per random outcome, shot after shot).
-
-    Which measurements are random, and every x/z bit on the way, do not
-    depend on the outcomes; the signs, and with them every measured bit, are
-    affine over GF(2) in the random outcomes.  So one run with every random
-    outcome 0 (s0) and one run per random outcome j with only j set give the
-    map, and shot s is s0 ^ (u_s . D), evaluated by ``qj_gf2_samples``:
-    len(qubits) x (k + 1) measurements for any ``nshots``, k the number of
-    random outcomes."""
-    _check_tableau(state, nqubits)
-    qubits = [int(q) for q in qubits]
-    m, nshots = len(qubits), int(nshots)
-    s0, random_at = _measure_forced(state.clone(), qubits, nqubits, set())
-    k = len(random_at)
+    generator (one per random outcome, shot after shot).  The affine forms
+    of the measured bits come from one symbolic pass
+    (``measurement_forms``); shot s is F[:, 0] ^ (F[:, 1:] . u_s), evaluated
+    by ``qj_gf2_samples`` for all shots at once."""
+    forms, k = measurement_forms(state, qubits, nqubits)
+    m, nshots = forms.shape[0], int(nshots)
     kw = max(1, (k + 63) // 64)
-    dt = np.zeros((m, kw), dtype=np.uint64)
-    for j in range(k):
-        sj, at = _measure_forced(state.clone(), qubits, nqubits, {j})
-        if at != random_at:
-            raise RuntimeError("clifford sampling: random outcomes moved between runs")
-        flip = np.nonzero(sj ^ s0)[0]
-        dt[flip, j // 64] |= np.uint64(1) << np.uint64(j % 64)
+    dt = np.zeros((m, kw * 8), dtype=np.uint8)
+    if k:
+        packed = np.packbits(forms[:, 1:], axis=1, bitorder="little")
+        dt[:, : packed.shape[1]] = packed
     draws = np.random.randint(2, size=(nshots, k), dtype=np.uint).astype(np.uint8)
     u = np.zeros((nshots, kw * 8), dtype=np.uint8)
     if k:
         packed = np.packbits(draws, axis=1, bitorder="little")
-        u[:, :packed.shape[1]] = packed
+        u[:, : packed.shape[1]] = packed
     dev = state.device
-    u_d = torch.from_numpy(u.view(np.int64).reshape(nshots, kw)).to(dev)
+    u_d = torch.from_numpy(u.view(np.int64)).to(dev)
     dt_d = torch.from_numpy(dt.view(np.int64)).to(dev)
-    s0_d = torch.from_numpy(s0).to(dev)
+    s0_d = torch.from_numpy(np.ascontiguousarray(forms[:, 0])).to(dev)
     out = torch.empty((nshots, m), dtype=torch.uint8, device=dev)
     _native.call("qj_gf2_samples", _ptr(u_d), _ptr(dt_d), _ptr(s0_d), nshots, m, kw, _ptr(out), _stream(state))
     return out

@@ -541,6 +541,88 @@ __global__ void k_unpackbits_rows(const uint8_t* __restrict__ in, int64_t rows, 
   }
 }
 
+__global__ void k_fill_words(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = v;
+}
+
+// ---- symbolic signs: the measured bits as affine forms over GF(2) ----
+//
+// For a fixed state the x/z bits met by a sequence of measurements do not
+// depend on the outcomes; the signs do, affinely in the random outcomes
+// u_1..u_k.  Sign column j (j = 0: the constant term, j >= 1: variable j)
+// holds, for every row, the coefficient of that term in the row's sign.
+
+// Combined 2-bit exponent counters of a sliced rowsum: cc[2w], cc[2w+1].
+__global__ void k_sym_counters(int64_t W, int nchunks, const uint32_t* __restrict__ part, uint32_t* __restrict__ cc) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  uint32_t c0 = 0, c1 = 0;
+#pragma unroll 8
+  for (int y = 0; y < nchunks; ++y) {
+    const uint32_t a0 = part[(y * W + w) * 2], a1 = part[(y * W + w) * 2 + 1];
+    c1 ^= a1 ^ (c0 & a0);
+    c0 ^= a0;
+  }
+  cc[2 * w] = c0;
+  cc[2 * w + 1] = c1;
+}
+
+// Sign forms of the rows h (hm minus p) after h <- P_p P_h: an odd exponent
+// sum makes the sign the constant 1, an even one r_h ^ r_p ^ c1.  Thread
+// per (word, sign column); bit p of every column is read, never written.
+__global__ void k_sym_rowsum_signs(uint32_t* __restrict__ signs, int64_t ws, int64_t W, int64_t p,
+                                   const uint32_t* __restrict__ hm, const uint32_t* __restrict__ cc) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  uint32_t h = hm[w];
+  if (w == (p >> 5)) h &= ~(1u << (p & 31));
+  if (!h) return;
+  const int64_t j = blockIdx.y;
+  uint32_t* col = signs + j * ws;
+  const uint32_t c0 = cc[2 * w], c1 = cc[2 * w + 1];
+  const uint32_t old = col[w];
+  const uint32_t rp = bit_of(col, p) ? ~0u : 0u;
+  const uint32_t even = old ^ rp ^ (j == 0 ? c1 : 0u);
+  const uint32_t odd = j == 0 ? ~0u : 0u;
+  col[w] = (old & ~h) | (h & ~c0 & even) | (h & c0 & odd);
+}
+
+// Sign columns of rows p - n and p after a random outcome that becomes
+// variable `var`: row p - n <- row p, row p <- u_var.  Thread per column.
+__global__ void k_sym_set_rows(uint32_t* __restrict__ signs, int64_t ws, int nterms, int64_t p, int64_t d, int var) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nterms) return;
+  uint32_t* col = signs + (int64_t)j * ws;
+  const uint32_t b = bit_of(col, p);
+  col[d >> 5] = (col[d >> 5] & ~(1u << (d & 31))) | (b << (d & 31));
+  const uint32_t nb = j == var ? 1u : 0u;
+  col[p >> 5] = (col[p >> 5] & ~(1u << (p & 31))) | (nb << (p & 31));
+}
+
+// Form of a determined outcome: for every sign column, the parity over the
+// stabilizer rows n+i with destabilizer bit x_q(i) = 1 (the XOR rule of the
+// determined rowsum).  Workgroup (one wave) per sign column.
+__global__ __launch_bounds__(64) void k_sym_determined(const uint32_t* __restrict__ tab, const uint32_t* __restrict__ signs,
+                                                       int n, int64_t ws, int q, uint8_t* __restrict__ form) {
+  const int64_t j = blockIdx.x;
+  const uint32_t* col = signs + j * ws;
+  const uint32_t* xq = tab + (int64_t)q * ws;
+  const int64_t nw = ((int64_t)n + 31) >> 5;
+  const int sh = n & 31;
+  uint32_t acc = 0;
+  for (int64_t i = threadIdx.x; i < nw; i += 64) {
+    const uint32_t m = xq[i] & rows_below(i, n);
+    if (!m) continue;
+    const int64_t a = (n + i * 32) >> 5;
+    uint32_t v = col[a] >> sh;
+    if (sh && a + 1 < ws) v |= col[a + 1] << (32 - sh);
+    acc ^= v & m;
+  }
+  const uint64_t odd = __ballot(__builtin_popcount(acc) & 1);
+  if (threadIdx.x == 0) form[j] = (uint8_t)(__builtin_popcountll(odd) & 1);
+}
+
 // ---- many shots: samples as affine functions of the random outcomes ----
 
 // out[s, q] = s0[q] ^ parity(u[s] & dt[q]): u holds each shot's random
@@ -587,6 +669,19 @@ void device_lds(int* cus, size_t* max_lds) {
   }
   *cus = dev >= 0 ? ncu[dev] : 256;
   *max_lds = dev >= 0 ? (size_t)mlds[dev] : 65536;
+}
+
+// Column chunks of a sliced rowsum: enough (word, chunk) threads to keep
+// every CU busy, at most kMaxColumnChunks partial counters per word.
+void column_chunks(int n, int64_t W, int* nchunks_out, int* kchunk_out) {
+  int cus = 0;
+  size_t max_lds = 0;
+  device_lds(&cus, &max_lds);
+  int64_t nchunks = std::max<int64_t>(1, ((int64_t)cus * 1024) / std::max<int64_t>(W, 1));
+  nchunks = std::min<int64_t>(nchunks, std::min<int64_t>(n, kMaxColumnChunks));
+  const int kchunk = (int)((n + nchunks - 1) / nchunks);
+  *kchunk_out = kchunk;
+  *nchunks_out = (int)((n + kchunk - 1) / kchunk);
 }
 
 // Groups of up to kCliffGroup gates of one kind on pairwise disjoint
@@ -840,13 +935,8 @@ int qj_clifford_random_outcome(uint32_t* tab, int nqubits, int p, int q, int out
   const int64_t W = cliff_words(nqubits), ws = cliff_stride(nqubits);
   const int64_t prow = (int64_t)nqubits + p;
   // column chunks: enough (word, chunk) threads to keep every CU busy
-  int cus = 0;
-  size_t max_lds = 0;
-  device_lds(&cus, &max_lds);
-  int64_t nchunks = std::max<int64_t>(1, ((int64_t)cus * 1024) / std::max<int64_t>(W, 1));
-  nchunks = std::min<int64_t>(nchunks, std::min<int64_t>(nqubits, kMaxColumnChunks));
-  const int kchunk = (int)((nqubits + nchunks - 1) / nchunks);
-  nchunks = (nqubits + kchunk - 1) / kchunk;
+  int nchunks = 0, kchunk = 0;
+  column_chunks(nqubits, W, &nchunks, &kchunk);
   DeviceScratch* sc = nullptr;
   rc = scratch_for_current_device(&sc);
   if (rc) return rc;
@@ -934,6 +1024,70 @@ int qj_clifford_rowsum(uint32_t* tab, int nqubits, const int64_t* h, const int64
   rc = check_launch("clifford_rowsum launch");
   ring_release(ticket, s);
   return rc;
+}
+
+int qj_clifford_sym_init(const uint32_t* tab, int nqubits, uint32_t* signs, int nvars, void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_sym_init");
+  if (rc) return rc;
+  if (!signs) return set_error(QJ_EVALUE, "clifford_sym_init: null sign columns");
+  if (nvars < 0) return set_error(QJ_EVALUE, "clifford_sym_init: negative variable count");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t ws = cliff_stride(nqubits);
+  rc = hip_status(hipMemcpyAsync(signs, tab + 2 * (int64_t)nqubits * ws, ws * sizeof(uint32_t),
+                                 hipMemcpyDeviceToDevice, s), "clifford_sym_init copy");
+  if (rc) return rc;
+  if (nvars > 0) {
+    k_fill_words<<<dim3(blocks_for(nvars * ws, kBlock)), dim3(kBlock), 0, s>>>(signs + ws, (int64_t)nvars * ws, 0u);
+    return check_launch("clifford_sym_init launch");
+  }
+  return QJ_OK;
+}
+
+int qj_clifford_sym_random(uint32_t* tab, int nqubits, uint32_t* signs, int nvars, int p, int q, int var,
+                           void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_sym_random");
+  if (rc) return rc;
+  if (!signs) return set_error(QJ_EVALUE, "clifford_sym_random: null sign columns");
+  if (q < 0 || q >= nqubits) return set_error(QJ_EVALUE, "clifford_sym_random: qubit %d out of range", q);
+  if (p < 0 || p >= nqubits) return set_error(QJ_EVALUE, "clifford_sym_random: stabilizer %d out of range", p);
+  if (var < 1 || var > nvars) return set_error(QJ_EVALUE, "clifford_sym_random: variable %d out of range", var);
+  if (nvars >= 65535) return set_error(QJ_EVALUE, "clifford_sym_random: at most 65534 variables");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t W = cliff_words(nqubits), ws = cliff_stride(nqubits);
+  const int64_t prow = (int64_t)nqubits + p;
+  int nchunks = 0, kchunk = 0;
+  column_chunks(nqubits, W, &nchunks, &kchunk);
+  DeviceScratch* sc = nullptr;
+  rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  rc = scratch_reserve_big(sc, (size_t)(3 * W + 2 * (int64_t)nchunks * W) * sizeof(uint32_t), s);
+  if (rc) return rc;
+  uint32_t* hm = static_cast<uint32_t*>(sc->big);
+  uint32_t* cc = hm + W;
+  uint32_t* part = cc + 2 * W;
+  k_cliff_hmask<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(tab, ws, W, nqubits, q, hm);
+  k_cliff_rowsum_sliced<<<dim3(blocks_for(W, kBlock), (unsigned)nchunks), dim3(kBlock), 0, s>>>(
+      tab, nqubits, ws, W, prow, hm, kchunk, part);
+  k_sym_counters<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(W, nchunks, part, cc);
+  k_sym_rowsum_signs<<<dim3(blocks_for(W, kBlock), (unsigned)(nvars + 1)), dim3(kBlock), 0, s>>>(signs, ws, W, prow,
+                                                                                                  hm, cc);
+  const int64_t C = 2 * (int64_t)nqubits + 1;
+  k_cliff_set_rows<<<dim3(blocks_for(C, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, prow, q, 0);
+  k_sym_set_rows<<<dim3(blocks_for(nvars + 1, kBlock)), dim3(kBlock), 0, s>>>(signs, ws, nvars + 1, prow,
+                                                                             prow - nqubits, var);
+  return check_launch("clifford_sym_random launch");
+}
+
+int qj_clifford_sym_determined(const uint32_t* tab, int nqubits, const uint32_t* signs, int nvars, int q,
+                               uint8_t* form, void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_sym_determined");
+  if (rc) return rc;
+  if (!signs || !form) return set_error(QJ_EVALUE, "clifford_sym_determined: null pointer");
+  if (q < 0 || q >= nqubits) return set_error(QJ_EVALUE, "clifford_sym_determined: qubit %d out of range", q);
+  if (nvars < 0) return set_error(QJ_EVALUE, "clifford_sym_determined: negative variable count");
+  k_sym_determined<<<dim3((unsigned)(nvars + 1)), dim3(64), 0, (hipStream_t)stream>>>(tab, signs, nqubits,
+                                                                                    cliff_stride(nqubits), q, form);
+  return check_launch("clifford_sym_determined launch");
 }
 
 int qj_gf2_samples(const uint64_t* u, const uint64_t* dt, const uint8_t* s0, int64_t nshots, int64_t m, int64_t kw,

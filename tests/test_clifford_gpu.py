@@ -255,6 +255,24 @@ def test_sample_shots_equals_shot_by_shot(n, nmeas):
     assert np.array_equal(keys, keys2) and pos == pos2
 
 
+@pytest.mark.parametrize("n,nmeas", [(6, 6), (30, 18)])
+def test_measurement_forms_against_forced_outcomes(n, nmeas):
+    """The symbolic pass's forms equal concrete measurement runs with the
+    random outcomes forced: all 0, and each one alone set to 1."""
+    rng = np.random.default_rng(100 + n)
+    tab = ch.zero_state(n)
+    ch.apply_circuit(tab, random_circuit(rng, n, 12 * n), n)
+    qubits = [int(v) for v in rng.permutation(n)[:nmeas]]
+    forms, k = ch.measurement_forms(tab, qubits, n)
+    assert forms.shape == (nmeas, k + 1) and k >= 2
+    s0, at = ch._measure_forced(tab.clone(), qubits, n, set())
+    assert len(at) == k
+    np.testing.assert_array_equal(forms[:, 0], s0)
+    for j in range(k):
+        sj, _ = ch._measure_forced(tab.clone(), qubits, n, {j})
+        np.testing.assert_array_equal(forms[:, 0] ^ forms[:, j + 1], sj, err_msg=f"variable {j + 1}")
+
+
 def test_sample_shots_of_a_ghz_state():
     """GHZ on 40 qubits: every shot is all zeros or all ones, about half each."""
     n = 40
