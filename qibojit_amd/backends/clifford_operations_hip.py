@@ -283,16 +283,18 @@ def M(state, qubits, nqubits, collapse=False):
     return sample
 
 
-def _measure_forced(state, qubits, nqubits, flips):
+def measure_with_outcomes(state, qubits, nqubits, ones):
     """The measurement loop of ``M`` on ``state`` (collapsed in place) with
-    the random outcomes given instead of drawn: the t-th random outcome is 1
-    iff t is in ``flips``.  Returns the sample and the positions (indices
-    into ``qubits``) of the random outcomes."""
+    the random outcomes chosen instead of drawn (post-selection): the t-th
+    random outcome (t = 0, 1, ...) is 1 iff t is in ``ones``.  Returns the
+    sample (uint8 array) and the positions in ``qubits`` of the random
+    outcomes."""
+    _check_tableau(state, nqubits)
     sample, random_at = [], []
     for pos, q in enumerate(qubits):
         p = first_stabilizer(state, q, nqubits)
         if p >= 0:
-            bit = int(len(random_at) in flips)
+            bit = int(len(random_at) in ones)
             _native.call("qj_clifford_random_outcome", _ptr(state), int(nqubits), p, int(q), bit, _stream(state))
             random_at.append(pos)
             sample.append(bit)

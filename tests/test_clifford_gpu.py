@@ -265,11 +265,11 @@ def test_measurement_forms_against_forced_outcomes(n, nmeas):
     qubits = [int(v) for v in rng.permutation(n)[:nmeas]]
     forms, k = ch.measurement_forms(tab, qubits, n)
     assert forms.shape == (nmeas, k + 1) and k >= 2
-    s0, at = ch._measure_forced(tab.clone(), qubits, n, set())
+    s0, at = ch.measure_with_outcomes(tab.clone(), qubits, n, set())
     assert len(at) == k
     np.testing.assert_array_equal(forms[:, 0], s0)
     for j in range(k):
-        sj, _ = ch._measure_forced(tab.clone(), qubits, n, {j})
+        sj, _ = ch.measure_with_outcomes(tab.clone(), qubits, n, {j})
         np.testing.assert_array_equal(forms[:, 0] ^ forms[:, j + 1], sj, err_msg=f"variable {j + 1}")
 
 
