@@ -405,20 +405,17 @@ __device__ __forceinline__ void lds_get(const cplx<float>* sm, cplx<float> (&x)[
   for (int r = 0; r < 16; ++r) x[r] = sm[tile_swz(tile_t<PH>(threadIdx.x, r))];
 }
 
-// Moves x from layout P to layout Q through the LDS buffer (and y after it,
-// through the same buffer, in adjoint mode).
+// Moves x from layout P to layout Q through the LDS buffer (and y at the
+// same time through a second tile buffer, in adjoint mode: two barriers per
+// transpose for both states).
 template <bool ADJ, int P, int Q>
 __device__ __forceinline__ void tile_transpose(cplx<float>* sm, cplx<float> (&x)[16], cplx<float> (&y)[16]) {
   lds_put<P>(sm, x);
+  if (ADJ) lds_put<P>(sm + (1 << kTileBits), y);
   __syncthreads();
   lds_get<Q>(sm, x);
-  if (ADJ) {
-    __syncthreads();
-    lds_put<P>(sm, y);
-    __syncthreads();
-    lds_get<Q>(sm, y);
-  }
-  __syncthreads();  // the buffer is free again
+  if (ADJ) lds_get<Q>(sm + (1 << kTileBits), y);
+  __syncthreads();  // the buffers are free again
 }
 
 template <bool ADJ, int NROT>
@@ -426,8 +423,8 @@ __global__ void __launch_bounds__(kTileBlock)
     k_rot_tile(const cplx<float>* psi_in, cplx<float>* psi_out, const cplx<float>* lam_in, cplx<float>* lam_out,
                const int32_t* __restrict__ sel, const cplx<float>* __restrict__ mats,
                double* __restrict__ partials, u64 ntiles, RotTileArgs a) {
-  // one 32 KiB tile buffer (the kernel's only LDS); the gradient reduction
-  // reuses it after the last tile
+  // one 32 KiB tile buffer per state (the kernel's only LDS); the gradient
+  // reduction reuses it after the last tile
   extern __shared__ __align__(16) unsigned char tile_raw[];
   cplx<float>* sm = reinterpret_cast<cplx<float>*>(tile_raw);
   double* wsum = reinterpret_cast<double*>(tile_raw);
@@ -805,7 +802,7 @@ int qj_rot_tile_pass(const void* psi_in, void* psi_out, const void* lam_in, void
   const u64 ntiles = (u64(1) << nqubits) >> kTileBits;
   dim3 gd((unsigned)ncols, (unsigned)nrows);
   hipStream_t st = (hipStream_t)stream;
-  const size_t lds = (size_t)8 << kTileBits;  // one complex64 tile
+  const size_t lds = (size_t)(adj ? 2 : 1) * (8 << kTileBits);  // one complex64 tile per state
   const cplx<float>* m = static_cast<const cplx<float>*>(mats);
   const cplx<float>* pi = (const cplx<float>*)psi_in;
   cplx<float>* po = (cplx<float>*)psi_out;
