@@ -276,7 +276,10 @@ __global__ void __launch_bounds__(kRotBlock)
 // with `ladder` set H' = L^+ H L for the CNOT ladder L (psi is the state
 // before the ladder): there the diagonal reads bits of P(k) and X on bit q
 // flips bits q and q-1.  out == NULL computes the energy only.
-template <typename R>
+// Each thread takes V consecutive amplitudes: a flip of bits at or above
+// log2(V) reads the partner group with one vector load, a flip inside the
+// group pairs amplitudes already in registers.
+template <typename R, int V>
 __global__ void __launch_bounds__(kRotBlock)
     k_tfim_batch(const cplx<R>* psi, cplx<R>* out, const int32_t* __restrict__ sel, u64 stride, int nq, double g,
                  int ladder, double* __restrict__ partials) {
@@ -288,34 +291,61 @@ __global__ void __launch_bounds__(kRotBlock)
   cplx<R>* o = out ? out + s * stride : nullptr;
   const u64 zzmask = (u64(1) << (nq - 1)) - 1;
   double acc = 0.0;
-  const u64 step = (u64)gridDim.x * kRotBlock;
-  for (u64 k = (u64)blockIdx.x * kRotBlock + threadIdx.x; k < n; k += step) {
-    const cplx<R> x = p[k];
-    const u64 kd = ladder ? suffix_parity(k) : k;
-    const int pc = __popcll((kd ^ (kd >> 1)) & zzmask);
-    const double zz = (double)(nq - 1) - 2.0 * (double)pc;
-    const double xr = x.re, xi = x.im;
+  const u64 step = (u64)gridDim.x * kRotBlock * V;
+  for (u64 k0 = ((u64)blockIdx.x * kRotBlock + threadIdx.x) * V; k0 < n; k0 += step) {
+    const cvec<R, V> xv = ldv<R, V>(p + k0);
+    double zz[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      const u64 kd = ladder ? suffix_parity(k0 + j) : k0 + j;
+      zz[j] = (double)(nq - 1) - 2.0 * (double)__popcll((kd ^ (kd >> 1)) & zzmask);
+    }
     if (o) {
-      double re = -zz * xr, im = -zz * xi;
-      for (int q = 0; q < nq; ++q) {
-        const u64 f = (u64(1) << q) | (ladder && q ? u64(1) << (q - 1) : 0);
-        const cplx<R> y = p[k ^ f];
-        re -= g * (double)y.re;
-        im -= g * (double)y.im;
+      double re[V], im[V];
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        re[j] = -zz[j] * (double)xv.v[j].re;
+        im[j] = -zz[j] * (double)xv.v[j].im;
       }
-      o[k] = mk<R>((R)re, (R)im);
-      acc += xr * re + xi * im;
-    } else {
-      // each unordered pair {k, k^f} once, from its lower index
-      double xs = 0.0;
       for (int q = 0; q < nq; ++q) {
         const u64 f = (u64(1) << q) | (ladder && q ? u64(1) << (q - 1) : 0);
-        if (!(k & (u64(1) << q))) {
-          const cplx<R> y = p[k ^ f];
-          xs += xr * (double)y.re + xi * (double)y.im;
+        const u64 fo = f & ~u64(V - 1);
+        const int fi = (int)(f & (V - 1));
+        const cvec<R, V> yv = fo ? ldv<R, V>(p + (k0 ^ fo)) : xv;
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+          re[j] -= g * (double)yv.v[j ^ fi].re;
+          im[j] -= g * (double)yv.v[j ^ fi].im;
         }
       }
-      acc += -zz * (xr * xr + xi * xi) - 2.0 * g * xs;
+      cvec<R, V> ov;
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        ov.v[j] = mk<R>((R)re[j], (R)im[j]);
+        acc += (double)xv.v[j].re * re[j] + (double)xv.v[j].im * im[j];
+      }
+      stv<R, V>(o + k0, ov);
+    } else {
+      // each unordered pair {k, k^f} once, from its lower index
+      double xs[V];
+#pragma unroll
+      for (int j = 0; j < V; ++j) xs[j] = 0.0;
+      for (int q = 0; q < nq; ++q) {
+        const u64 f = (u64(1) << q) | (ladder && q ? u64(1) << (q - 1) : 0);
+        const u64 fo = f & ~u64(V - 1);
+        const int fi = (int)(f & (V - 1));
+        if (fo && ((k0 >> q) & 1)) continue;  // bit q (>= log2 V) is 1 for the whole group
+        const cvec<R, V> yv = fo ? ldv<R, V>(p + (k0 ^ fo)) : xv;
+#pragma unroll
+        for (int j = 0; j < V; ++j)
+          if (fo || !((j >> q) & 1))
+            xs[j] += (double)xv.v[j].re * (double)yv.v[j ^ fi].re + (double)xv.v[j].im * (double)yv.v[j ^ fi].im;
+      }
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        const double xr = xv.v[j].re, xi = xv.v[j].im;
+        acc += -zz[j] * (xr * xr + xi * xi) - 2.0 * g * xs[j];
+      }
     }
   }
   acc = rot_block_sum(acc, wsum);
@@ -832,14 +862,27 @@ int qj_tfim_batch(const void* psi, void* out, int dtype, int nqubits, uint64_t s
   if (psi == out) return set_error(QJ_EVALUE, "tfim_batch: out must not alias psi");
   hipStream_t st = (hipStream_t)stream;
   dim3 gd((unsigned)ncols, (unsigned)nrows);
-  if (dtype == QJ_COMPLEX64)
-    k_tfim_batch<float><<<gd, dim3(kRotBlock), 0, st>>>((const cplx<float>*)psi, (cplx<float>*)out, sel, stride,
-                                                        nqubits, g, ladder != 0, partials);
-  else if (dtype == QJ_COMPLEX128)
-    k_tfim_batch<double><<<gd, dim3(kRotBlock), 0, st>>>((const cplx<double>*)psi, (cplx<double>*)out, sel,
-                                                         stride, nqubits, g, ladder != 0, partials);
-  else
+  // vector groups of 32 bytes when every state starts on one (the state
+  // pointers are at least 256-byte aligned)
+  if (dtype == QJ_COMPLEX64) {
+    if (stride % 4 == 0 && (reinterpret_cast<uintptr_t>(psi) & 31) == 0 &&
+        (!out || (reinterpret_cast<uintptr_t>(out) & 31) == 0))
+      k_tfim_batch<float, 4><<<gd, dim3(kRotBlock), 0, st>>>((const cplx<float>*)psi, (cplx<float>*)out, sel,
+                                                             stride, nqubits, g, ladder != 0, partials);
+    else
+      k_tfim_batch<float, 1><<<gd, dim3(kRotBlock), 0, st>>>((const cplx<float>*)psi, (cplx<float>*)out, sel,
+                                                             stride, nqubits, g, ladder != 0, partials);
+  } else if (dtype == QJ_COMPLEX128) {
+    if (stride % 2 == 0 && (reinterpret_cast<uintptr_t>(psi) & 31) == 0 &&
+        (!out || (reinterpret_cast<uintptr_t>(out) & 31) == 0))
+      k_tfim_batch<double, 2><<<gd, dim3(kRotBlock), 0, st>>>((const cplx<double>*)psi, (cplx<double>*)out, sel,
+                                                              stride, nqubits, g, ladder != 0, partials);
+    else
+      k_tfim_batch<double, 1><<<gd, dim3(kRotBlock), 0, st>>>((const cplx<double>*)psi, (cplx<double>*)out, sel,
+                                                              stride, nqubits, g, ladder != 0, partials);
+  } else {
     return set_error(QJ_EVALUE, "tfim_batch: dtype must be complex64/complex128");
+  }
   return check_launch("tfim_batch launch");
 }
 
