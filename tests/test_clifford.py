@@ -274,6 +274,26 @@ def test_schedule_groups_reproduce_the_gate_list(n, ngates):
     np.testing.assert_array_equal(got, want)
 
 
+def test_schedule_size_query_and_capacity():
+    from qibojit_amd.backends import clifford_operations_hip as ch
+
+    arr = ch.encode_gates([("H", 0), ("H", 1), ("CNOT", 0, 1), ("S", 1), ("CZ", 1, 2), ("X", 0)])
+    ptr = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    ngroups = ctypes.c_int64(-1)
+    _native.call("qj_clifford_schedule", ptr, arr.shape[0], 3, None, 0, ctypes.byref(ngroups))
+    assert 1 <= ngroups.value <= arr.shape[0]
+    small = np.zeros(32, dtype=np.uint32)
+    if ngroups.value > 1:
+        with pytest.raises(ValueError, match="capacity"):
+            _native.call("qj_clifford_schedule", ptr, arr.shape[0], 3, small.ctypes.data_as(ctypes.c_void_p), 1,
+                         ctypes.byref(ngroups))
+    empty = ctypes.c_int64(-1)
+    _native.call("qj_clifford_schedule", ptr, 0, 3, None, 0, ctypes.byref(empty))
+    assert empty.value == 0
+    with pytest.raises(ValueError, match="out of range"):
+        _native.call("qj_clifford_schedule", ptr, arr.shape[0], 2, None, 0, ctypes.byref(empty))
+
+
 def test_vector_draws_equal_scalar_draws():
     """sample_shots draws its random outcomes as one (nshots, k) array: the
     legacy generator gives the same bits, and the same state after, as the
