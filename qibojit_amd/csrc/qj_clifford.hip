@@ -33,6 +33,10 @@ enum : int {
   kciSWAP = QJ_CLIFF_ISWAP, kcCY = QJ_CLIFF_CY, kcNop = 15
 };
 constexpr int kcFirstTwo = kcCNOT;
+// Gate lists only: a composed one-qubit Clifford, slot word 1 holding its
+// code (bits 0-3: m00 m01 m10 m11 of x' = m00 x ^ m01 z, z' = m10 x ^ m11 z;
+// bits 4-6: a, b, c of the sign update a x ^ b z ^ c xz).
+constexpr int kcOne = 15;
 constexpr int kCliffGroup = 16;         // gates per group (pairwise disjoint qubits)
 constexpr int kCliffMaxQubits = 1 << 19;
 constexpr int kMaxColumnChunks = 64;   // column chunks of a sliced rowsum
@@ -176,13 +180,15 @@ __device__ __forceinline__ void read_group(const uint32_t* p, uint32_t (&dw)[kDe
 template <int KIND, int S>
 __device__ __forceinline__ uint32_t run_group(uint32_t* col0, int64_t zoff, int64_t cs, uint32_t qmax, int cnt,
                                               const uint32_t* cur, int g, int ng, uint32_t (&dw)[kDescWords]) {
-  constexpr bool two = KIND >= kcFirstTwo;
+  constexpr bool two = KIND >= kcFirstTwo && KIND != kcOne;
   int64_t oa[S], ob[S];
+  uint32_t code[S];
 #pragma unroll
   for (int j = 0; j < S; ++j) {
     // clamped: a descriptor array from outside stays inside the tableau
     oa[j] = (int64_t)std::min(__builtin_amdgcn_readfirstlane(dw[2 * j]) & kSlotMask, qmax) * cs;
-    ob[j] = (int64_t)std::min(__builtin_amdgcn_readfirstlane(dw[2 * j + 1]) & kSlotMask, qmax) * cs;
+    code[j] = __builtin_amdgcn_readfirstlane(dw[2 * j + 1]) & kSlotMask;
+    ob[j] = (int64_t)std::min(code[j], qmax) * cs;
   }
   uint32_t a[S], b[S], c[S], d[S];
 #pragma unroll
@@ -201,7 +207,18 @@ __device__ __forceinline__ uint32_t run_group(uint32_t* col0, int64_t zoff, int6
   uint32_t t = 0;
 #pragma unroll
   for (int j = 0; j < S; ++j) {
-    const uint32_t tj = cliff_apply(KIND, col0, oa[j], ob[j], zoff, a[j], b[j], c[j], d[j]);
+    uint32_t tj;
+    if constexpr (KIND == kcOne) {
+      const uint32_t k = code[j];
+      const uint32_t m00 = 0u - (k & 1u), m01 = 0u - ((k >> 1) & 1u), m10 = 0u - ((k >> 2) & 1u),
+                     m11 = 0u - ((k >> 3) & 1u), sa = 0u - ((k >> 4) & 1u), sb = 0u - ((k >> 5) & 1u),
+                     sc = 0u - ((k >> 6) & 1u);
+      col0[oa[j]] = (a[j] & m00) ^ (b[j] & m01);
+      col0[zoff + oa[j]] = (a[j] & m10) ^ (b[j] & m11);
+      tj = (a[j] & sa) ^ (b[j] & sb) ^ (a[j] & b[j] & sc);
+    } else {
+      tj = cliff_apply(KIND, col0, oa[j], ob[j], zoff, a[j], b[j], c[j], d[j]);
+    }
     t ^= j < cnt ? tj : 0u;
   }
   return t;
@@ -289,6 +306,7 @@ __global__ __launch_bounds__(64) void k_cliff_circuit(uint32_t* __restrict__ tab
         case kcCZ: t = run_sized<kcCZ>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
         case kcSWAP: t = run_sized<kcSWAP>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
         case kciSWAP: t = run_sized<kciSWAP>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
+        case kcOne: t = run_sized<kcOne>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
         default: t = run_sized<kcCY>(col0, zoff, cs, qmax, cnt, cur, g, ng, dw); break;
       }
       r ^= t;
@@ -684,26 +702,99 @@ void column_chunks(int n, int64_t W, int* nchunks_out, int* kchunk_out) {
   *nchunks_out = (int)((n + kchunk - 1) / kchunk);
 }
 
+// One-qubit rules as (m, s): m = bits m00 m01 m10 m11 of the GF(2) map of
+// (x, z), s = bits a b c of the sign update a x ^ b z ^ c xz (the same
+// functions as cliff_apply's one-qubit cases).
+constexpr uint8_t kOneM[10] = {0x6, 0xD, 0x9, 0x9, 0x9, 0xB, 0xD, 0xB, 0x6, 0x6};
+constexpr uint8_t kOneS[10] = {0x4, 0x4, 0x1, 0x2, 0x3, 0x6, 0x5, 0x4, 0x5, 0x6};
+
+inline int bit_of_host(uint32_t v, int i) { return (v >> i) & 1; }
+
+// Rule (m2, s2) after rule (m1, s1), as one rule.  With x1 = m00 x ^ m01 z,
+// z1 = m10 x ^ m11 z: x1 z1 = m00 m10 x ^ m01 m11 z ^ (m00 m11 ^ m01 m10) xz
+// for bits, so the second sign update stays of the form a x ^ b z ^ c xz.
+inline void compose_one(uint8_t m1, uint8_t s1, uint8_t m2, uint8_t s2, uint8_t* m, uint8_t* sgn) {
+  const int m00 = bit_of_host(m1, 0), m01 = bit_of_host(m1, 1), m10 = bit_of_host(m1, 2), m11 = bit_of_host(m1, 3);
+  const int n00 = bit_of_host(m2, 0), n01 = bit_of_host(m2, 1), n10 = bit_of_host(m2, 2), n11 = bit_of_host(m2, 3);
+  const int a2 = bit_of_host(s2, 0), b2 = bit_of_host(s2, 1), c2 = bit_of_host(s2, 2);
+  const int ax = (a2 & m00) ^ (b2 & m10) ^ (c2 & m00 & m10);
+  const int bz = (a2 & m01) ^ (b2 & m11) ^ (c2 & m01 & m11);
+  const int cxz = c2 & ((m00 & m11) ^ (m01 & m10));
+  *sgn = (uint8_t)(s1 ^ (ax | (bz << 1) | (cxz << 2)));
+  const int p00 = (n00 & m00) ^ (n01 & m10), p01 = (n00 & m01) ^ (n01 & m11);
+  const int p10 = (n10 & m00) ^ (n11 & m10), p11 = (n10 & m01) ^ (n11 & m11);
+  *m = (uint8_t)(p00 | (p01 << 1) | (p10 << 2) | (p11 << 3));
+}
+
+inline bool is_two(int kind) { return kind >= kcFirstTwo && kind != kcOne; }
+
+// Validates the list and composes every qubit's runs of one-qubit gates
+// (the gates between two gates that touch the qubit otherwise) into one
+// kcOne gate (kind, qubit, code), dropping identities.  A run is emitted
+// just before the next two-qubit gate on its qubit, or at the end: it only
+// moves past gates on other qubits.
+int compose_runs(const int32_t* gates, int64_t ngates, int n, std::vector<int32_t>& out) {
+  std::vector<uint8_t> pm(n, 0x9), ps(n, 0), live(n, 0);
+  out.clear();
+  out.reserve((size_t)ngates * 3);
+  auto flush = [&](int q) {
+    if (live[q] && !(pm[q] == 0x9 && ps[q] == 0)) {
+      int kind = kcOne;  // a named rule when the composite is one (its code is shorter)
+      for (int k = 0; k < kcFirstTwo; ++k)
+        if (kOneM[k] == pm[q] && kOneS[k] == ps[q]) kind = k;
+      out.push_back(kind);
+      out.push_back(q);
+      out.push_back(kind == kcOne ? (pm[q] | (ps[q] << 4)) : q);
+    }
+    pm[q] = 0x9;
+    ps[q] = 0;
+    live[q] = 0;
+  };
+  for (int64_t g = 0; g < ngates; ++g) {
+    const int kind = gates[3 * g], a = gates[3 * g + 1], b = gates[3 * g + 2];
+    if (kind < 0 || kind >= kcNop)
+      return set_error(QJ_EVALUE, "clifford circuit: gate %lld has bad kind %d", (long long)g, kind);
+    if (a < 0 || a >= n) return set_error(QJ_EVALUE, "clifford circuit: gate %lld qubit %d out of range", (long long)g, a);
+    if (kind < kcFirstTwo) {
+      uint8_t m, sg;
+      compose_one(pm[a], ps[a], kOneM[kind], kOneS[kind], &m, &sg);
+      pm[a] = m;
+      ps[a] = sg;
+      live[a] = 1;
+      continue;
+    }
+    if (b < 0 || b >= n) return set_error(QJ_EVALUE, "clifford circuit: gate %lld qubit %d out of range", (long long)g, b);
+    if (b == a) return set_error(QJ_EVALUE, "clifford circuit: gate %lld acts twice on qubit %d", (long long)g, a);
+    flush(a);
+    flush(b);
+    out.push_back(kind);
+    out.push_back(a);
+    out.push_back(b);
+  }
+  for (int q = 0; q < n; ++q) flush(q);
+  return QJ_OK;
+}
+
 // Groups of up to kCliffGroup gates of one kind on pairwise disjoint
 // qubits, by list scheduling: a gate is ready once every earlier gate that
 // shares a qubit with it has been placed in a group, ready gates act on
 // disjoint qubits (of two gates on one qubit only the earlier can be ready),
 // and each group takes the ready gates of the kind with the most of them.
 // This only moves gates past gates on other qubits: those commute, and so do
-// their XORs into r, so the tableau is the same bit for bit.  Returns the
-// descriptor words (ngroups x kCliffGroup x 2).
-int schedule_gates(const int32_t* gates, int64_t ngates, int n, std::vector<uint32_t>& desc) {
+// their XORs into r, so the tableau is the same bit for bit.  One-qubit
+// gates are composed per run first (compose_runs), so they share one kind.
+// Returns the descriptor words (ngroups x kCliffGroup x 2).
+int schedule_gates(const int32_t* in_gates, int64_t in_ngates, int n, std::vector<uint32_t>& desc) {
+  std::vector<int32_t> composed;
+  int rc = compose_runs(in_gates, in_ngates, n, composed);
+  if (rc) return rc;
+  const int32_t* gates = composed.data();
+  const int64_t ngates = (int64_t)composed.size() / 3;
+  constexpr int kKinds = kcOne + 1;
   std::vector<int64_t> count(n + 1, 0);
   for (int64_t g = 0; g < ngates; ++g) {
-    const int kind = gates[3 * g], a = gates[3 * g + 1], b = gates[3 * g + 2];
-    if (kind < 0 || kind >= kcNop) return set_error(QJ_EVALUE, "clifford circuit: gate %lld has bad kind %d", (long long)g, kind);
-    if (a < 0 || a >= n) return set_error(QJ_EVALUE, "clifford circuit: gate %lld qubit %d out of range", (long long)g, a);
-    ++count[a + 1];
-    if (kind >= kcFirstTwo) {
-      if (b < 0 || b >= n) return set_error(QJ_EVALUE, "clifford circuit: gate %lld qubit %d out of range", (long long)g, b);
-      if (b == a) return set_error(QJ_EVALUE, "clifford circuit: gate %lld acts twice on qubit %d", (long long)g, a);
-      ++count[b + 1];
-    }
+    ++count[gates[3 * g + 1] + 1];
+    if (is_two(gates[3 * g])) ++count[gates[3 * g + 2] + 1];
   }
   // per-qubit gate lists in circuit order (CSR)
   for (int q = 0; q < n; ++q) count[q + 1] += count[q];
@@ -711,15 +802,15 @@ int schedule_gates(const int32_t* gates, int64_t ngates, int n, std::vector<uint
   std::vector<int64_t> fillp(count.begin(), count.end() - 1);
   for (int64_t g = 0; g < ngates; ++g) {
     lists[fillp[gates[3 * g + 1]]++] = g;
-    if (gates[3 * g] >= kcFirstTwo) lists[fillp[gates[3 * g + 2]]++] = g;
+    if (is_two(gates[3 * g])) lists[fillp[gates[3 * g + 2]]++] = g;
   }
   std::vector<int64_t> head(count.begin(), count.end() - 1);
   std::vector<uint8_t> hits(ngates, 0);
-  std::vector<std::vector<int64_t>> ready(kcNop);
-  std::vector<size_t> rpos(kcNop, 0);
+  std::vector<std::vector<int64_t>> ready(kKinds);
+  std::vector<size_t> rpos(kKinds, 0);
   int64_t nready = 0;
   auto hit = [&](int64_t g) {
-    const int need = gates[3 * g] >= kcFirstTwo ? 2 : 1;
+    const int need = is_two(gates[3 * g]) ? 2 : 1;
     if (++hits[g] == need) {
       ready[gates[3 * g]].push_back(g);
       ++nready;
@@ -733,7 +824,7 @@ int schedule_gates(const int32_t* gates, int64_t ngates, int n, std::vector<uint
   while (nready > 0) {
     int best = 0;
     size_t most = 0;
-    for (int k = 0; k < kcNop; ++k) {
+    for (int k = 0; k < kKinds; ++k) {
       const size_t avail = ready[k].size() - rpos[k];
       if (avail > most) {
         most = avail;
@@ -752,9 +843,10 @@ int schedule_gates(const int32_t* gates, int64_t ngates, int n, std::vector<uint
     for (int j = 0; j < cnt; ++j) {
       const int64_t g = taken[j];
       const int a = gates[3 * g + 1];
-      const int b = best >= kcFirstTwo ? gates[3 * g + 2] : a;
+      // second word: the target of a two-qubit gate, the code of a composed
+      // one-qubit gate
       slot[2 * j] = (uint32_t)a;
-      slot[2 * j + 1] = (uint32_t)b;
+      slot[2 * j + 1] = (uint32_t)gates[3 * g + 2];
     }
     for (int j = cnt; j < kCliffGroup; ++j) {  // padding: copies of slot 0
       slot[2 * j] = slot[0];
@@ -768,7 +860,7 @@ int schedule_gates(const int32_t* gates, int64_t ngates, int n, std::vector<uint
     for (int j = 0; j < cnt; ++j) {
       const int64_t g = taken[j];
       const int qs[2] = {gates[3 * g + 1], gates[3 * g + 2]};
-      for (int u = 0; u < (best >= kcFirstTwo ? 2 : 1); ++u) {
+      for (int u = 0; u < (is_two(best) ? 2 : 1); ++u) {
         const int q = qs[u];
         ++head[q];
         if (head[q] < count[q + 1]) hit(lists[head[q]]);

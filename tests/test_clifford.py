@@ -243,6 +243,16 @@ def _schedule(arr, n):
     return words.reshape(ngroups.value, 16, 2)
 
 
+def _apply_composed(sm, q, code, n):
+    """A composed one-qubit rule (code bits: m00 m01 m10 m11 a b c) on every
+    row: x' = m00 x ^ m01 z, z' = m10 x ^ m11 z, r ^= a x ^ b z ^ c x z."""
+    bit = [(code >> i) & 1 for i in range(7)]
+    x, z = sm[:, q].copy(), sm[:, n + q].copy()
+    sm[:, -1] ^= (bit[4] & x) ^ (bit[5] & z) ^ (bit[6] & x & z)
+    sm[:, q] = (bit[0] & x) ^ (bit[1] & z)
+    sm[:, n + q] = (bit[2] & x) ^ (bit[3] & z)
+
+
 @pytest.mark.parametrize("n,ngates", [(1, 40), (3, 300), (9, 500), (40, 800)])
 def test_schedule_groups_reproduce_the_gate_list(n, ngates):
     """The host scheduler's groups (one kind, disjoint qubits), applied in
@@ -256,18 +266,21 @@ def test_schedule_groups_reproduce_the_gate_list(n, ngates):
     mask = (1 << 24) - 1
     sm0 = rng.integers(0, 2, size=(2 * n + 1, 2 * n + 1)).astype(np.uint8)
     got = sm0.copy()
-    placed = 0
     for grp in groups:
         kind, cnt = int(grp[0, 0] >> 24), int(grp[0, 1] >> 24)
         assert 1 <= cnt <= 16
         slots = [(int(a & mask), int(b & mask)) for a, b in grp]
         assert all(s == slots[0] for s in slots[cnt:])  # padding repeats slot 0
-        used = [q for a, b in slots[:cnt] for q in ({a, b} if kind >= 10 else {a})]
+        two = 10 <= kind < 15
+        used = [q for a, b in slots[:cnt] for q in ({a, b} if two else {a})]
         assert len(used) == len(set(used))  # disjoint qubits
-        for a, b in slots[:cnt]:
-            cr.apply(got, names[kind], (a, b) if kind >= 10 else (a,), n)
-        placed += cnt
-    assert placed == ngates
+        for a, code in slots[:cnt]:
+            if two:
+                cr.apply(got, names[kind], (a, code), n)
+            elif kind == 15:
+                _apply_composed(got, a, code, n)
+            else:
+                cr.apply(got, names[kind], (a,), n)
     want = sm0.copy()
     for name, *qs in gates:
         cr.apply(want, name, tuple(qs), n)
