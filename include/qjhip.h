@@ -452,6 +452,14 @@ int qj_clifford_random_outcome(uint32_t* tab, int nqubits, int p, int q, int out
  * their signs (the reference's determined rowsum); *outcome_out (device
  * int32) = that sign. */
 int qj_clifford_determined_outcome(uint32_t* tab, int nqubits, int q, int32_t* outcome_out, void* stream);
+/* qj_clifford_determined_outcome, then the sign of the selected stabilizers'
+ * product with its phases accumulated (the sequential rowsum of Aaronson and
+ * Gottesman): outcome = (sum r_j + sum_{j<l} z_j.x_l + sum |x_j & z_j| / 2)
+ * mod 2 over the selected rows in row order, into the scratch row's sign and
+ * *outcome_out.  Always the state's certain outcome; the reference's XOR rule
+ * can differ from it.  No reference counterpart (an opt-in beside
+ * clifford_operations_gpu.py:628-640). */
+int qj_clifford_determined_outcome_exact(uint32_t* tab, int nqubits, int q, int32_t* outcome_out, void* stream);
 /* rowsum of clifford_operations_cpu.py:337-385 on host arrays h, i of
  * nrows row indices: row h[j] <- P_{i[j]} P_{h[j]} (x, z XORed; r = (2 r_h +
  * 2 r_i + sum of the exponents of i) mod 4 != 0), every product from the
@@ -476,6 +484,11 @@ int qj_clifford_sym_random(uint32_t* tab, int nqubits, uint32_t* signs, int nvar
                            void* stream);
 int qj_clifford_sym_determined(const uint32_t* tab, int nqubits, const uint32_t* signs, int nvars, int q,
                                uint8_t* form, void* stream);
+/* qj_clifford_sym_determined for the exact outcome of
+ * qj_clifford_determined_outcome_exact: the phase term (set by the x/z bits
+ * only) XORed into form[0].  No reference counterpart. */
+int qj_clifford_sym_determined_exact(const uint32_t* tab, int nqubits, const uint32_t* signs, int nvars, int q,
+                                     uint8_t* form, void* stream);
 /* Many measurement shots of one tableau: for a fixed state the measured
  * bits are affine over GF(2) in the random outcomes drawn on the way, so
  * shot s is s0 ^ (u[s] . D).  out[s * m + q] = s0[q] ^ parity(u[s] & dt[q])

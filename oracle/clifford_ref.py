@@ -24,7 +24,13 @@ How each function is pinned (tests/test_clifford.py):
     built from |0...0> by random Clifford circuits against the state vector
     of the same circuit (every stabilizer fixes the state, a random outcome
     has probability 1/2, the collapsed state is fixed by the new
-    stabilizers, a determined outcome is the state's certain outcome).
+    stabilizers); for a determined outcome, the selected stabilizers
+    multiply to +-Z_q, whose sign is the state's certain outcome, and the
+    XOR rule equals the XOR of their signs (it misses the certain outcome on
+    a recorded 3-qubit circuit, as the reference does);
+  * ``determined_outcome_exact``: equal to the certain outcome on every
+    determined case of those circuits (150 more at 2..6 qubits), including
+    the case where the XOR rule misses it.
 """
 
 import numpy as np
@@ -245,6 +251,26 @@ def determined_outcome(sm, q, n):
         return sm, int(sm[-1, -1])
     rowsum(sm, [2 * n] * len(idx), idx, n, True)
     return sm, int(sm[-1, -1])
+
+
+def determined_outcome_exact(sm, q, n):
+    """Determined outcome with the phases of the product accumulated (the
+    sequential Aaronson-Gottesman rowsum): the selected stabilizers S (rows
+    n+i with destabilizer bit x_q(i) = 1, in row order) multiply to
+    (-1)^(sum r_j + sum_{j<l} z_j.x_l) i^(sum |x_j & z_j|) Z_q, with Y = iXZ
+    per qubit.  Not the reference's rule (which XORs the signs); the scratch
+    row is left as ``determined_outcome`` leaves it, its sign set to the
+    exact outcome."""
+    sm, _ = determined_outcome(sm, q, n)
+    rows = np.nonzero(sm[:n, q])[0] + n
+    x = sm[rows, :n].astype(np.int64)
+    z = sm[rows, n:-1].astype(np.int64)
+    before = np.cumsum(z, axis=0) - z  # row l: sum of z_j over the rows j < l
+    cross = int((before * x).sum())
+    ys = int((x & z).sum())
+    out = (int(sm[rows, -1].sum()) + cross + ys // 2) & 1
+    sm[-1, -1] = out
+    return sm, out
 
 
 def zero_tableau(n):

@@ -75,10 +75,12 @@ SIGNATURES = {
     "qj_clifford_first_stabilizer": [_vp, _i32, _i32, _vp, _vp],
     "qj_clifford_random_outcome": [_vp, _i32, _i32, _i32, _i32, _vp],
     "qj_clifford_determined_outcome": [_vp, _i32, _i32, _vp, _vp],
+    "qj_clifford_determined_outcome_exact": [_vp, _i32, _i32, _vp, _vp],
     "qj_clifford_rowsum": [_vp, _i32, _vp, _vp, _i64, _i32, _vp],
     "qj_clifford_sym_init": [_vp, _i32, _vp, _i32, _vp],
     "qj_clifford_sym_random": [_vp, _i32, _vp, _i32, _i32, _i32, _i32, _vp],
     "qj_clifford_sym_determined": [_vp, _i32, _vp, _i32, _i32, _vp, _vp],
+    "qj_clifford_sym_determined_exact": [_vp, _i32, _vp, _i32, _i32, _vp, _vp],
     "qj_gf2_samples": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp],
     "qj_packbits_rows": [_vp, _i64, _i64, _vp, _vp],
     "qj_unpackbits_rows": [_vp, _i64, _i64, _i64, _vp, _vp],

@@ -254,11 +254,15 @@ def _random_outcome(state, p, q, nqubits):
     return state, outcome
 
 
-def _determined_outcome(state, q, nqubits):
-    """Determined outcome of qubit q (clifford_operations_gpu.py:628-640)."""
+def _determined_outcome(state, q, nqubits, exact=False):
+    """Determined outcome of qubit q (clifford_operations_gpu.py:628-640):
+    the XOR of the selected stabilizers' signs.  ``exact=True`` (no reference
+    counterpart) accumulates the phases of their product instead, which is
+    always the state's certain outcome; the XOR rule can differ from it."""
     _check_tableau(state, nqubits)
     out = torch.empty(1, dtype=torch.int32, device=state.device)
-    _native.call("qj_clifford_determined_outcome", _ptr(state), int(nqubits), int(q), _ptr(out), _stream(state))
+    name = "qj_clifford_determined_outcome_exact" if exact else "qj_clifford_determined_outcome"
+    _native.call(name, _ptr(state), int(nqubits), int(q), _ptr(out), _stream(state))
     return state, np.uint8(out.item())
 
 
@@ -268,9 +272,10 @@ def _init_state_for_measurements(state, nqubits, collapse):
     return state if collapse else state.clone()
 
 
-def M(state, qubits, nqubits, collapse=False):
+def M(state, qubits, nqubits, collapse=False, exact=False):
     """Measures ``qubits`` one after another (the measurement loop of qibo's
-    Clifford backend, which calls this module's outcome functions)."""
+    Clifford backend, which calls this module's outcome functions);
+    ``exact`` selects the determined-outcome rule of ``_determined_outcome``."""
     sample = []
     state = _init_state_for_measurements(state, nqubits, collapse)
     for q in qubits:
@@ -278,12 +283,12 @@ def M(state, qubits, nqubits, collapse=False):
         if p >= 0:
             state, outcome = _random_outcome(state, [p], q, nqubits)
         else:
-            state, outcome = _determined_outcome(state, q, nqubits)
+            state, outcome = _determined_outcome(state, q, nqubits, exact)
         sample.append(int(outcome))
     return sample
 
 
-def measure_with_outcomes(state, qubits, nqubits, ones):
+def measure_with_outcomes(state, qubits, nqubits, ones, exact=False):
     """The measurement loop of ``M`` on ``state`` (collapsed in place) with
     the random outcomes chosen instead of drawn (post-selection): the t-th
     random outcome (t = 0, 1, ...) is 1 iff t is in ``ones``.  Returns the
@@ -299,13 +304,13 @@ def measure_with_outcomes(state, qubits, nqubits, ones):
             random_at.append(pos)
             sample.append(bit)
         else:
-            _, outcome = _determined_outcome(state, q, nqubits)
+            _, outcome = _determined_outcome(state, q, nqubits, exact)
             sample.append(int(outcome))
     return np.array(sample, dtype=np.uint8), random_at
 
 
-def measurement_forms(state, qubits, nqubits):
-    """Affine forms over GF(2) of the bits ``M(state, qubits, nqubits)``
+def measurement_forms(state, qubits, nqubits, exact=False):
+    """Affine forms over GF(2) of the bits ``M(state, qubits, nqubits, exact=exact)``
     would return: a (len(qubits), k + 1) uint8 array F and k, the number of
     random outcomes, such that the sample for random outcomes u_1..u_k is
     F[:, 0] ^ (F[:, 1:] . u mod 2).  One symbolic measurement pass (the
@@ -332,19 +337,20 @@ def measurement_forms(state, qubits, nqubits):
             _native.call("qj_clifford_sym_random", _ptr(tab), n, _ptr(signs), m, p, q, k, stream)
             forms[i, k] = 1
         else:
-            _native.call("qj_clifford_sym_determined", _ptr(tab), n, _ptr(signs), m, q, _ptr(forms[i]), stream)
+            name = "qj_clifford_sym_determined_exact" if exact else "qj_clifford_sym_determined"
+            _native.call(name, _ptr(tab), n, _ptr(signs), m, q, _ptr(forms[i]), stream)
     return forms[:, : k + 1].cpu().numpy(), k
 
 
-def sample_shots(state, qubits, nqubits, nshots):
+def sample_shots(state, qubits, nqubits, nshots, exact=False):
     """``nshots`` measurements of ``qubits`` on copies of ``state`` (left
     unchanged): a (nshots, len(qubits)) uint8 device tensor equal to ``nshots``
-    calls of ``M(state, qubits, nqubits)``, with the same draws from NumPy's
+    calls of ``M(state, qubits, nqubits, exact=exact)``, with the same draws from NumPy's
     generator (one per random outcome, shot after shot).  The affine forms
     of the measured bits come from one symbolic pass
     (``measurement_forms``); shot s is F[:, 0] ^ (F[:, 1:] . u_s), evaluated
     by ``qj_gf2_samples`` for all shots at once."""
-    forms, k = measurement_forms(state, qubits, nqubits)
+    forms, k = measurement_forms(state, qubits, nqubits, exact)
     m, nshots = forms.shape[0], int(nshots)
     kw = max(1, (k + 63) // 64)
     dt = np.zeros((m, kw * 8), dtype=np.uint8)

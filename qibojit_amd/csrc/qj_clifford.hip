@@ -470,6 +470,82 @@ __global__ __launch_bounds__(64) void k_cliff_determined(uint32_t* __restrict__ 
   }
 }
 
+// Accumulated phase of the product of the stabilizers a determined outcome
+// selects (rows n+i with destabilizer bit x_q(i) = 1, in row order j < l):
+// with Y = iXZ per qubit, prod_j P_j = (-1)^(sum r_j + sum_{j<l} z_j.x_l)
+// i^(sum |x_j & z_j|) Z_q (the sequential rowsum of Aaronson and Gottesman,
+// Phys. Rev. A 70, 052328).  Workgroup (one wave) per qubit column k; lane
+// per 32-row word, 64 words per pass.  The cross term takes an exclusive
+// prefix parity of the selected z_k bits: within a word by a shift-XOR scan,
+// across the wave's words by a ballot of the word parities, across passes by
+// a carried bit.  The column adds (2 cross_k + Y_k) mod 4 to acc[0]; bit 1 of
+// the total is then (sum cross_k + floor(sum Y_k / 2)) mod 2.
+__global__ __launch_bounds__(64) void k_cliff_exact_terms(const uint32_t* __restrict__ tab, int n, int64_t ws, int q,
+                                                          uint32_t* __restrict__ acc) {
+  const int64_t k = blockIdx.x;
+  const int lane = (int)threadIdx.x;
+  const uint32_t* xc = tab + k * ws;
+  const uint32_t* zc = tab + ((int64_t)n + k) * ws;
+  const uint32_t* xq = tab + (int64_t)q * ws;
+  const int64_t nw = ((int64_t)n + 31) >> 5;
+  const int sh = n & 31;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t carry = 0, cross = 0, ycount = 0;
+  for (int64_t base = 0; base < nw; base += 64) {
+    const int64_t j = base + lane;
+    uint32_t X = 0, Z = 0;
+    if (j < nw) {
+      const uint32_t m = xq[j] & rows_below(j, n);
+      if (m) {
+        const int64_t a = (n + j * 32) >> 5;  // bits n+32j .. n+32j+31 of the columns
+        X = xc[a] >> sh;
+        Z = zc[a] >> sh;
+        if (sh && a + 1 < ws) {
+          X |= xc[a + 1] << (32 - sh);
+          Z |= zc[a + 1] << (32 - sh);
+        }
+        X &= m;
+        Z &= m;
+      }
+    }
+    uint32_t pz = Z;  // inclusive prefix parity within the word
+    pz ^= pz << 1;
+    pz ^= pz << 2;
+    pz ^= pz << 4;
+    pz ^= pz << 8;
+    pz ^= pz << 16;
+    const uint64_t odd = __ballot(__builtin_popcount(Z) & 1);
+    const uint32_t before = carry ^ (uint32_t)(__builtin_popcountll(odd & below) & 1);
+    pz = (pz << 1) ^ (before ? ~0u : 0u);  // exclusive, with every earlier row
+    cross ^= (uint32_t)__builtin_popcount(X & pz) & 1u;
+    ycount += (uint32_t)__builtin_popcount(X & Z);
+    carry ^= (uint32_t)(__builtin_popcountll(odd) & 1);
+  }
+  const uint32_t c = (uint32_t)(__builtin_popcountll(__ballot(cross)) & 1);
+  const uint32_t y = (uint32_t)__builtin_popcountll(__ballot(ycount & 1u)) +
+                     2u * (uint32_t)__builtin_popcountll(__ballot(ycount & 2u));
+  const uint32_t add = (2u * c + y) & 3u;
+  if (lane == 0 && add) atomicAdd(acc, add);
+}
+
+// Exact outcome = (XOR of the selected signs, left in the scratch row's sign
+// by k_cliff_determined) ^ bit 1 of acc[0]: into the scratch row's sign and
+// *out; or, with `form` set, into the constant term of a symbolic outcome.
+__global__ void k_cliff_exact_finish(uint32_t* __restrict__ tab, int n, int64_t ws, const uint32_t* __restrict__ acc,
+                                     int32_t* __restrict__ out, uint8_t* __restrict__ form) {
+  if (threadIdx.x || blockIdx.x) return;
+  const uint32_t c = (acc[0] >> 1) & 1u;
+  if (form) {
+    form[0] ^= (uint8_t)c;
+    return;
+  }
+  const int64_t s = 2 * (int64_t)n;
+  uint32_t* rc = tab + s * ws;
+  const uint32_t o = ((rc[s >> 5] >> (s & 31)) & 1u) ^ c;
+  rc[s >> 5] = (rc[s >> 5] & ~(1u << (s & 31))) | (o << (s & 31));
+  out[0] = (int32_t)o;
+}
+
 // ---- general rowsum (clifford_operations_cpu.py:337-385) ----
 
 // New sign of every pair from the state before the call: rnew[j], and
@@ -1057,6 +1133,29 @@ int qj_clifford_determined_outcome(uint32_t* tab, int nqubits, int q, int32_t* o
   return check_launch("clifford_determined_outcome launch");
 }
 
+// Phase terms of a determined outcome into a zeroed device word, then the
+// finish on the scratch row (form == nullptr) or a symbolic form.
+static int exact_phase(uint32_t* tab, int nqubits, int q, int32_t* out, uint8_t* form, hipStream_t s,
+                       const char* what) {
+  DeviceScratch* sc = nullptr;
+  int rc = scratch_for_current_device(&sc);
+  if (!rc) rc = scratch_reserve_big(sc, sizeof(uint32_t), s);
+  if (rc) return rc;
+  uint32_t* acc = static_cast<uint32_t*>(sc->big);
+  const int64_t ws = cliff_stride(nqubits);
+  k_fill_words<<<dim3(1), dim3(64), 0, s>>>(acc, 1, 0u);
+  k_cliff_exact_terms<<<dim3((unsigned)nqubits), dim3(64), 0, s>>>(tab, nqubits, ws, q, acc);
+  k_cliff_exact_finish<<<dim3(1), dim3(64), 0, s>>>(tab, nqubits, ws, acc, out, form);
+  return check_launch(what);
+}
+
+int qj_clifford_determined_outcome_exact(uint32_t* tab, int nqubits, int q, int32_t* outcome_out, void* stream) {
+  int rc = qj_clifford_determined_outcome(tab, nqubits, q, outcome_out, stream);
+  if (rc) return rc;
+  return exact_phase(tab, nqubits, q, outcome_out, nullptr, (hipStream_t)stream,
+                     "clifford_determined_outcome_exact launch");
+}
+
 int qj_clifford_rowsum(uint32_t* tab, int nqubits, const int64_t* h, const int64_t* i, int64_t nrows, int determined,
                        void* stream) {
   int rc = check_tab(tab, nqubits, "clifford_rowsum");
@@ -1180,6 +1279,14 @@ int qj_clifford_sym_determined(const uint32_t* tab, int nqubits, const uint32_t*
   k_sym_determined<<<dim3((unsigned)(nvars + 1)), dim3(64), 0, (hipStream_t)stream>>>(tab, signs, nqubits,
                                                                                     cliff_stride(nqubits), q, form);
   return check_launch("clifford_sym_determined launch");
+}
+
+int qj_clifford_sym_determined_exact(const uint32_t* tab, int nqubits, const uint32_t* signs, int nvars, int q,
+                                     uint8_t* form, void* stream) {
+  int rc = qj_clifford_sym_determined(tab, nqubits, signs, nvars, q, form, stream);
+  if (rc) return rc;
+  return exact_phase(const_cast<uint32_t*>(tab), nqubits, q, nullptr, form, (hipStream_t)stream,
+                     "clifford_sym_determined_exact launch");
 }
 
 int qj_gf2_samples(const uint64_t* u, const uint64_t* dt, const uint8_t* s0, int64_t nshots, int64_t m, int64_t kw,

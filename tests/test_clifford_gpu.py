@@ -293,6 +293,70 @@ def test_reference_determined_rule_case():
     assert int(outcome) == 1  # the reference's rule (the state's certain outcome is 0)
 
 
+@pytest.mark.parametrize("n", [1, 2, 31, 33, 100, 700, 2100])
+def test_exact_determined_outcome_on_random_matrices(n):
+    """The exact rule bit for bit against the oracle on random 0/1 matrices
+    (any selection, so long products and both outcomes occur); n = 2100
+    spans more than one 64-word pass of a column."""
+    rng = np.random.default_rng(n + 50)
+    outcomes = set()
+    for trial in range(6 if n < 2100 else 3):
+        sm = _dense(rng, n)
+        q = int(rng.integers(n))
+        tab = ch._clifford_pre_execution_reshape(sm)
+        _, outcome = ch._determined_outcome(tab, q, n, exact=True)
+        want, wout = cr.determined_outcome_exact(sm.copy(), q, n)
+        assert int(outcome) == wout, f"trial {trial} q={q}"
+        np.testing.assert_array_equal(_to_host(tab, n), want, err_msg=f"trial {trial} q={q}")
+        outcomes.add(wout)
+    if n >= 31:
+        assert outcomes == {0, 1}
+
+
+def test_exact_rule_on_the_reference_case():
+    n, gates, q = REFERENCE_DETERMINED_CASE
+    tab = ch.zero_state(n)
+    ch.apply_circuit(tab, gates, n)
+    _, outcome = ch._determined_outcome(tab, q, n, exact=True)
+    assert int(outcome) == 0  # the state's certain outcome
+    tab = ch.zero_state(n)
+    ch.apply_circuit(tab, gates, n)
+    assert ch.M(tab, [q], n, exact=True) == [0]
+    assert ch.sample_shots(tab, [q], n, 8, exact=True).cpu().numpy().tolist() == [[0]] * 8
+
+
+@pytest.mark.parametrize("n,nmeas", [(12, 12), (90, 40)])
+def test_exact_measurement_loop_and_sampling(n, nmeas):
+    """M(exact=True) against the oracle's loop with the exact rule, and
+    sample_shots(exact=True) against M(exact=True) shot by shot."""
+    rng = np.random.default_rng(n + 1)
+    gates = random_circuit(rng, n, 10 * n)
+    sm = _oracle_run(cr.zero_tableau(n), gates, n)
+    tab = ch.zero_state(n)
+    ch.apply_circuit(tab, gates, n)
+    qubits = [int(v) for v in rng.permutation(n)[:nmeas]]
+    before = tab.clone()
+    np.random.seed(n)
+    got = ch.sample_shots(tab, qubits, n, 20, exact=True).cpu().numpy()
+    assert torch.equal(tab, before)
+    np.random.seed(n)
+    want = np.array([ch.M(tab, qubits, n, exact=True) for _ in range(20)], dtype=np.uint8)
+    np.testing.assert_array_equal(got, want)
+    np.random.seed(n)
+    sample = ch.M(tab, qubits, n, collapse=True, exact=True)
+    np.random.seed(n)
+    osm, osample = sm.copy(), []
+    for q in qubits:
+        p = cr.first_stabilizer(osm, q, n)
+        if p >= 0:
+            osm, out = cr.random_outcome(osm, p, q, n, int(np.random.randint(2, size=None, dtype=np.uint)))
+        else:
+            osm, out = cr.determined_outcome_exact(osm, q, n)
+        osample.append(out)
+    assert sample == osample
+    np.testing.assert_array_equal(_to_host(tab, n), osm)
+
+
 def test_repeated_measurement_at_size():
     """n = 4096: measuring every qubit twice gives the same outcomes."""
     n = 4096
