@@ -756,12 +756,30 @@ def run_clifford(sizes=((100, 100000), (1000, 100000), (10000, 20000)), reps=3, 
         qubits = [int(v) for v in rng.choice(n, size=min(nmeas, n), replace=False)]
         np.random.seed(seed)
         ch.M(tab, qubits[:2], n, collapse=True)
+        tab_host_loop = tab.clone()
         torch.cuda.synchronize()
+        state = np.random.get_state()
         t0 = time.perf_counter()
-        sample = ch.M(tab, qubits, n, collapse=True)
+        sample = ch.M(tab, qubits, n, collapse=True)  # one call: the loop enqueued on the device
         torch.cuda.synchronize()
         rec["measure_seconds_per_qubit"] = (time.perf_counter() - t0) / len(qubits)
         rec["measured_ones"] = int(sum(sample))
+        # the same measurements through the per-qubit entry points (two host
+        # round trips per qubit, as qibo's loop over the outcome functions)
+        np.random.set_state(state)
+        t0 = time.perf_counter()
+        sample_host_loop = []
+        for q in qubits:
+            p = ch.first_stabilizer(tab_host_loop, q, n)
+            if p >= 0:
+                _, o = ch._random_outcome(tab_host_loop, [p], q, n)
+            else:
+                _, o = ch._determined_outcome(tab_host_loop, q, n)
+            sample_host_loop.append(int(o))
+        torch.cuda.synchronize()
+        rec["measure_seconds_per_qubit_host_loop"] = (time.perf_counter() - t0) / len(qubits)
+        rec["measure_loops_agree"] = bool(sample_host_loop == sample and torch.equal(tab_host_loop, tab))
+        del tab_host_loop
         # many shots of 32 qubits not measured above: affine-map sampling vs
         # the shot-by-shot loop
         measured = set(qubits)

@@ -460,6 +460,18 @@ int qj_clifford_determined_outcome(uint32_t* tab, int nqubits, int q, int32_t* o
  * can differ from it.  No reference counterpart (an opt-in beside
  * clifford_operations_gpu.py:628-640). */
 int qj_clifford_determined_outcome_exact(uint32_t* tab, int nqubits, int q, int32_t* outcome_out, void* stream);
+/* qibo's Clifford measurement loop (the loop of _random_outcome /
+ * _determined_outcome, clifford_operations_gpu.py:606-640, over `qubits`)
+ * enqueued in one call with no host round trip: for each qubits[i] (host
+ * int32, m entries) the first anticommuting stabilizer is found on the
+ * device; a random outcome takes draws[t] (device uint8, m entries, t = the
+ * random outcomes taken before it), a determined one the reference's XOR
+ * rule (exact != 0: the rule of qj_clifford_determined_outcome_exact).
+ * sample_out (device int32, m) gets the outcomes, *nrandom_out (device
+ * int32) the number of random ones.  Same tableau and outcomes as the
+ * per-qubit entry points. */
+int qj_clifford_measure(uint32_t* tab, int nqubits, const int32_t* qubits, int m, const uint8_t* draws, int exact,
+                        int32_t* sample_out, int32_t* nrandom_out, void* stream);
 /* rowsum of clifford_operations_cpu.py:337-385 on host arrays h, i of
  * nrows row indices: row h[j] <- P_{i[j]} P_{h[j]} (x, z XORed; r = (2 r_h +
  * 2 r_i + sum of the exponents of i) mod 4 != 0), every product from the

@@ -76,6 +76,7 @@ SIGNATURES = {
     "qj_clifford_random_outcome": [_vp, _i32, _i32, _i32, _i32, _vp],
     "qj_clifford_determined_outcome": [_vp, _i32, _i32, _vp, _vp],
     "qj_clifford_determined_outcome_exact": [_vp, _i32, _i32, _vp, _vp],
+    "qj_clifford_measure": [_vp, _i32, _vp, _i32, _vp, _i32, _vp, _vp, _vp],
     "qj_clifford_rowsum": [_vp, _i32, _vp, _vp, _i64, _i32, _vp],
     "qj_clifford_sym_init": [_vp, _i32, _vp, _i32, _vp],
     "qj_clifford_sym_random": [_vp, _i32, _vp, _i32, _i32, _i32, _i32, _vp],

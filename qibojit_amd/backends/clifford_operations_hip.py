@@ -275,17 +275,30 @@ def _init_state_for_measurements(state, nqubits, collapse):
 def M(state, qubits, nqubits, collapse=False, exact=False):
     """Measures ``qubits`` one after another (the measurement loop of qibo's
     Clifford backend, which calls this module's outcome functions);
-    ``exact`` selects the determined-outcome rule of ``_determined_outcome``."""
-    sample = []
+    ``exact`` selects the determined-outcome rule of ``_determined_outcome``.
+
+    The loop runs on the device in one call (``qj_clifford_measure``) and
+    reads back once.  The random bits come from NumPy's global generator as
+    in the loop of ``_random_outcome`` calls: len(qubits) bits are drawn up
+    front, and the generator is then rewound and advanced by the number of
+    random outcomes actually taken (a vector draw of randint(2) consumes the
+    stream as the same number of scalar draws)."""
     state = _init_state_for_measurements(state, nqubits, collapse)
-    for q in qubits:
-        p = first_stabilizer(state, q, nqubits)
-        if p >= 0:
-            state, outcome = _random_outcome(state, [p], q, nqubits)
-        else:
-            state, outcome = _determined_outcome(state, q, nqubits, exact)
-        sample.append(int(outcome))
-    return sample
+    qs = np.asarray([int(q) for q in qubits], dtype=np.int32)
+    m = len(qs)
+    if m == 0:
+        return []
+    saved = np.random.get_state()
+    draws = np.random.randint(2, size=m, dtype=np.uint).astype(np.uint8)
+    dev = state.device
+    draws_d = torch.from_numpy(draws).to(dev)
+    out = torch.empty(m + 1, dtype=torch.int32, device=dev)
+    _native.call("qj_clifford_measure", _ptr(state), int(nqubits), qs.ctypes.data_as(ctypes.c_void_p), m,
+                 _ptr(draws_d), int(bool(exact)), _ptr(out), _ptr(out[m:]), _stream(state))
+    host = out.cpu().numpy()
+    np.random.set_state(saved)
+    np.random.randint(2, size=int(host[m]), dtype=np.uint)
+    return [int(v) for v in host[:m]]
 
 
 def measure_with_outcomes(state, qubits, nqubits, ones, exact=False):

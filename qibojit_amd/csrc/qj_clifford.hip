@@ -331,6 +331,14 @@ __global__ __launch_bounds__(64) void k_cliff_circuit(uint32_t* __restrict__ tab
 }
 
 // ---- measurement ----
+//
+// The kernels of one measured qubit take an optional control block `ctl`
+// (device int32: [0] the first stabilizer p - n or -1, [1] the random
+// outcomes taken so far, [2] the determined outcome).  With ctl null they use
+// their scalar arguments; with ctl set, p comes from ctl[0] and each kernel
+// returns at once when its branch (random: p >= 0, determined: p < 0) is not
+// the one taken, so a whole measurement loop is enqueued without a host
+// round trip.
 
 // First stabilizer row (n <= row < 2n) with x_q = 1, as row - n, or -1.
 __global__ __launch_bounds__(256) void k_cliff_first_stab(const uint32_t* __restrict__ tab, int n, int64_t ws, int q,
@@ -355,7 +363,8 @@ __global__ __launch_bounds__(256) void k_cliff_first_stab(const uint32_t* __rest
 
 // hm[w] = x_q bits of rows < 2n (the rows a random outcome multiplies).
 __global__ void k_cliff_hmask(const uint32_t* __restrict__ tab, int64_t ws, int64_t W, int n, int q,
-                              uint32_t* __restrict__ hm) {
+                              uint32_t* __restrict__ hm, const int32_t* __restrict__ ctl) {
+  if (ctl && ctl[0] < 0) return;
   const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (w < W) hm[w] = tab[(int64_t)q * ws + w] & rows_below(w, 2 * (int64_t)n);
 }
@@ -366,9 +375,14 @@ __global__ void k_cliff_hmask(const uint32_t* __restrict__ tab, int64_t ws, int6
 // part[(chunk * W + w) * 2 + {0, 1}].  Row p is not rewritten, so its bits
 // read by every thread stay those of the state before the call.
 __global__ void k_cliff_rowsum_sliced(uint32_t* __restrict__ tab, int n, int64_t ws, int64_t W, int64_t p,
-                                      const uint32_t* __restrict__ hm, int kchunk, uint32_t* __restrict__ part) {
+                                      const uint32_t* __restrict__ hm, int kchunk, uint32_t* __restrict__ part,
+                                      const int32_t* __restrict__ ctl) {
   const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (w >= W) return;
+  if (ctl) {
+    if (ctl[0] < 0) return;
+    p = (int64_t)n + ctl[0];
+  }
   uint32_t h = hm[w];
   if (w == (p >> 5)) h &= ~(1u << (p & 31));
   uint32_t c0 = 0, c1 = 0;
@@ -406,9 +420,14 @@ __global__ void k_cliff_rowsum_sliced(uint32_t* __restrict__ tab, int n, int64_t
 
 // r_h = (2 r_h + 2 r_p + sum) mod 4 != 0 for the rows of hm (minus p).
 __global__ void k_cliff_rowsum_sign(uint32_t* __restrict__ tab, int n, int64_t ws, int64_t W, int64_t p,
-                                    const uint32_t* __restrict__ hm, int nchunks, const uint32_t* __restrict__ part) {
+                                    const uint32_t* __restrict__ hm, int nchunks, const uint32_t* __restrict__ part,
+                                    const int32_t* __restrict__ ctl) {
   const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (w >= W) return;
+  if (ctl) {
+    if (ctl[0] < 0) return;
+    p = (int64_t)n + ctl[0];
+  }
   uint32_t h = hm[w];
   if (w == (p >> 5)) h &= ~(1u << (p & 31));
   if (!h) return;
@@ -427,11 +446,17 @@ __global__ void k_cliff_rowsum_sign(uint32_t* __restrict__ tab, int n, int64_t w
 }
 
 // Row p - n <- row p; row p <- (r = outcome, z_q = 1, zero elsewhere).
-// Thread per column.
-__global__ void k_cliff_set_rows(uint32_t* __restrict__ tab, int n, int64_t ws, int64_t p, int q, int outcome) {
+// Thread per column.  With ctl, the outcome is draws[ctl[1]].
+__global__ void k_cliff_set_rows(uint32_t* __restrict__ tab, int n, int64_t ws, int64_t p, int q, int outcome,
+                                 const int32_t* __restrict__ ctl, const uint8_t* __restrict__ draws) {
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t C = 2 * (int64_t)n + 1;
   if (c >= C) return;
+  if (ctl) {
+    if (ctl[0] < 0) return;
+    p = (int64_t)n + ctl[0];
+    outcome = draws[ctl[1]] & 1;
+  }
   uint32_t* col = tab + c * ws;
   const int64_t d = p - n;
   const uint32_t b = bit_of(col, p);
@@ -446,7 +471,8 @@ __global__ void k_cliff_set_rows(uint32_t* __restrict__ tab, int n, int64_t ws, 
 // the XOR of their signs.  Workgroup (one wave) per column; the scratch bit
 // of column c is written by its own wave only.
 __global__ __launch_bounds__(64) void k_cliff_determined(uint32_t* __restrict__ tab, int n, int64_t ws, int q,
-                                                         int32_t* __restrict__ out) {
+                                                         int32_t* out, const int32_t* ctl) {
+  if (ctl && ctl[0] >= 0) return;
   const int64_t c = blockIdx.x;
   uint32_t* col = tab + c * ws;
   const uint32_t* xq = tab + (int64_t)q * ws;
@@ -481,7 +507,8 @@ __global__ __launch_bounds__(64) void k_cliff_determined(uint32_t* __restrict__ 
 // a carried bit.  The column adds (2 cross_k + Y_k) mod 4 to acc[0]; bit 1 of
 // the total is then (sum cross_k + floor(sum Y_k / 2)) mod 2.
 __global__ __launch_bounds__(64) void k_cliff_exact_terms(const uint32_t* __restrict__ tab, int n, int64_t ws, int q,
-                                                          uint32_t* __restrict__ acc) {
+                                                          uint32_t* __restrict__ acc, const int32_t* __restrict__ ctl) {
+  if (ctl && ctl[0] >= 0) return;
   const int64_t k = blockIdx.x;
   const int lane = (int)threadIdx.x;
   const uint32_t* xc = tab + k * ws;
@@ -532,8 +559,9 @@ __global__ __launch_bounds__(64) void k_cliff_exact_terms(const uint32_t* __rest
 // by k_cliff_determined) ^ bit 1 of acc[0]: into the scratch row's sign and
 // *out; or, with `form` set, into the constant term of a symbolic outcome.
 __global__ void k_cliff_exact_finish(uint32_t* __restrict__ tab, int n, int64_t ws, const uint32_t* __restrict__ acc,
-                                     int32_t* __restrict__ out, uint8_t* __restrict__ form) {
+                                     int32_t* out, uint8_t* __restrict__ form, const int32_t* ctl) {
   if (threadIdx.x || blockIdx.x) return;
+  if (ctl && ctl[0] >= 0) return;
   const uint32_t c = (acc[0] >> 1) & 1u;
   if (form) {
     form[0] ^= (uint8_t)c;
@@ -544,6 +572,22 @@ __global__ void k_cliff_exact_finish(uint32_t* __restrict__ tab, int n, int64_t 
   const uint32_t o = ((rc[s >> 5] >> (s & 31)) & 1u) ^ c;
   rc[s >> 5] = (rc[s >> 5] & ~(1u << (s & 31))) | (o << (s & 31));
   out[0] = (int32_t)o;
+}
+
+// End of one measured qubit of a device loop: sample[i] = the random outcome
+// draws[ctl[1]] (then ctl[1] + 1) or the determined outcome ctl[2];
+// *nrandom = the random outcomes taken so far.
+__global__ void k_cliff_meas_finish(int32_t* __restrict__ ctl, const uint8_t* __restrict__ draws,
+                                    int32_t* __restrict__ sample, int i, int32_t* __restrict__ nrandom) {
+  if (threadIdx.x || blockIdx.x) return;
+  int32_t count = ctl[1];
+  if (ctl[0] >= 0) {
+    sample[i] = draws[count] & 1;
+    ctl[1] = ++count;
+  } else {
+    sample[i] = ctl[2];
+  }
+  nrandom[0] = count;
 }
 
 // ---- general rowsum (clifford_operations_cpu.py:337-385) ----
@@ -1112,13 +1156,14 @@ int qj_clifford_random_outcome(uint32_t* tab, int nqubits, int p, int q, int out
   if (rc) return rc;
   uint32_t* hm = static_cast<uint32_t*>(sc->big);
   uint32_t* part = hm + W;
-  k_cliff_hmask<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(tab, ws, W, nqubits, q, hm);
+  k_cliff_hmask<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(tab, ws, W, nqubits, q, hm, nullptr);
   k_cliff_rowsum_sliced<<<dim3(blocks_for(W, kBlock), (unsigned)nchunks), dim3(kBlock), 0, s>>>(
-      tab, nqubits, ws, W, prow, hm, kchunk, part);
+      tab, nqubits, ws, W, prow, hm, kchunk, part, nullptr);
   k_cliff_rowsum_sign<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, W, prow, hm,
-                                                                           (int)nchunks, part);
+                                                                           (int)nchunks, part, nullptr);
   const int64_t C = 2 * (int64_t)nqubits + 1;
-  k_cliff_set_rows<<<dim3(blocks_for(C, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, prow, q, outcome);
+  k_cliff_set_rows<<<dim3(blocks_for(C, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, prow, q, outcome, nullptr,
+                                                                        nullptr);
   return check_launch("clifford_random_outcome launch");
 }
 
@@ -1129,7 +1174,7 @@ int qj_clifford_determined_outcome(uint32_t* tab, int nqubits, int q, int32_t* o
   if (q < 0 || q >= nqubits) return set_error(QJ_EVALUE, "clifford_determined_outcome: qubit %d out of range", q);
   const int64_t C = 2 * (int64_t)nqubits + 1;
   k_cliff_determined<<<dim3((unsigned)C), dim3(64), 0, (hipStream_t)stream>>>(tab, nqubits, cliff_stride(nqubits), q,
-                                                                              outcome_out);
+                                                                              outcome_out, nullptr);
   return check_launch("clifford_determined_outcome launch");
 }
 
@@ -1144,8 +1189,8 @@ static int exact_phase(uint32_t* tab, int nqubits, int q, int32_t* out, uint8_t*
   uint32_t* acc = static_cast<uint32_t*>(sc->big);
   const int64_t ws = cliff_stride(nqubits);
   k_fill_words<<<dim3(1), dim3(64), 0, s>>>(acc, 1, 0u);
-  k_cliff_exact_terms<<<dim3((unsigned)nqubits), dim3(64), 0, s>>>(tab, nqubits, ws, q, acc);
-  k_cliff_exact_finish<<<dim3(1), dim3(64), 0, s>>>(tab, nqubits, ws, acc, out, form);
+  k_cliff_exact_terms<<<dim3((unsigned)nqubits), dim3(64), 0, s>>>(tab, nqubits, ws, q, acc, nullptr);
+  k_cliff_exact_finish<<<dim3(1), dim3(64), 0, s>>>(tab, nqubits, ws, acc, out, form, nullptr);
   return check_launch(what);
 }
 
@@ -1154,6 +1199,52 @@ int qj_clifford_determined_outcome_exact(uint32_t* tab, int nqubits, int q, int3
   if (rc) return rc;
   return exact_phase(tab, nqubits, q, outcome_out, nullptr, (hipStream_t)stream,
                      "clifford_determined_outcome_exact launch");
+}
+
+int qj_clifford_measure(uint32_t* tab, int nqubits, const int32_t* qubits, int m, const uint8_t* draws, int exact,
+                        int32_t* sample_out, int32_t* nrandom_out, void* stream) {
+  int rc = check_tab(tab, nqubits, "clifford_measure");
+  if (rc) return rc;
+  if (m < 0) return set_error(QJ_EVALUE, "clifford_measure: negative qubit count");
+  if (m == 0) return QJ_OK;
+  if (!qubits || !draws || !sample_out || !nrandom_out) return set_error(QJ_EVALUE, "clifford_measure: null pointer");
+  for (int i = 0; i < m; ++i) {
+    if (qubits[i] < 0 || qubits[i] >= nqubits)
+      return set_error(QJ_EVALUE, "clifford_measure: qubit %d out of range at %d", qubits[i], i);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t W = cliff_words(nqubits), ws = cliff_stride(nqubits);
+  const int64_t C = 2 * (int64_t)nqubits + 1;
+  int nchunks = 0, kchunk = 0;
+  column_chunks(nqubits, W, &nchunks, &kchunk);
+  DeviceScratch* sc = nullptr;
+  rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  rc = scratch_reserve_big(sc, (size_t)(W + 2 * nchunks * W + 8) * sizeof(uint32_t), s);
+  if (rc) return rc;
+  uint32_t* hm = static_cast<uint32_t*>(sc->big);
+  uint32_t* part = hm + W;
+  int32_t* ctl = reinterpret_cast<int32_t*>(part + 2 * nchunks * W);
+  uint32_t* acc = reinterpret_cast<uint32_t*>(ctl + 4);
+  k_fill_words<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<uint32_t*>(ctl), 4, 0u);
+  for (int i = 0; i < m; ++i) {
+    const int q = qubits[i];
+    k_cliff_first_stab<<<dim3(1), dim3(256), 0, s>>>(tab, nqubits, ws, q, ctl);
+    k_cliff_hmask<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(tab, ws, W, nqubits, q, hm, ctl);
+    k_cliff_rowsum_sliced<<<dim3(blocks_for(W, kBlock), (unsigned)nchunks), dim3(kBlock), 0, s>>>(
+        tab, nqubits, ws, W, 0, hm, kchunk, part, ctl);
+    k_cliff_rowsum_sign<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, W, 0, hm,
+                                                                             (int)nchunks, part, ctl);
+    k_cliff_set_rows<<<dim3(blocks_for(C, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, 0, q, 0, ctl, draws);
+    k_cliff_determined<<<dim3((unsigned)C), dim3(64), 0, s>>>(tab, nqubits, ws, q, ctl + 2, ctl);
+    if (exact) {
+      k_fill_words<<<dim3(1), dim3(64), 0, s>>>(acc, 1, 0u);
+      k_cliff_exact_terms<<<dim3((unsigned)nqubits), dim3(64), 0, s>>>(tab, nqubits, ws, q, acc, ctl);
+      k_cliff_exact_finish<<<dim3(1), dim3(64), 0, s>>>(tab, nqubits, ws, acc, ctl + 2, nullptr, ctl);
+    }
+    k_cliff_meas_finish<<<dim3(1), dim3(64), 0, s>>>(ctl, draws, sample_out, i, nrandom_out);
+  }
+  return check_launch("clifford_measure launch");
 }
 
 int qj_clifford_rowsum(uint32_t* tab, int nqubits, const int64_t* h, const int64_t* i, int64_t nrows, int determined,
@@ -1256,14 +1347,15 @@ int qj_clifford_sym_random(uint32_t* tab, int nqubits, uint32_t* signs, int nvar
   uint32_t* hm = static_cast<uint32_t*>(sc->big);
   uint32_t* cc = hm + W;
   uint32_t* part = cc + 2 * W;
-  k_cliff_hmask<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(tab, ws, W, nqubits, q, hm);
+  k_cliff_hmask<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(tab, ws, W, nqubits, q, hm, nullptr);
   k_cliff_rowsum_sliced<<<dim3(blocks_for(W, kBlock), (unsigned)nchunks), dim3(kBlock), 0, s>>>(
-      tab, nqubits, ws, W, prow, hm, kchunk, part);
+      tab, nqubits, ws, W, prow, hm, kchunk, part, nullptr);
   k_sym_counters<<<dim3(blocks_for(W, kBlock)), dim3(kBlock), 0, s>>>(W, nchunks, part, cc);
   k_sym_rowsum_signs<<<dim3(blocks_for(W, kBlock), (unsigned)(nvars + 1)), dim3(kBlock), 0, s>>>(signs, ws, W, prow,
                                                                                                   hm, cc);
   const int64_t C = 2 * (int64_t)nqubits + 1;
-  k_cliff_set_rows<<<dim3(blocks_for(C, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, prow, q, 0);
+  k_cliff_set_rows<<<dim3(blocks_for(C, kBlock)), dim3(kBlock), 0, s>>>(tab, nqubits, ws, prow, q, 0, nullptr,
+                                                                        nullptr);
   k_sym_set_rows<<<dim3(blocks_for(nvars + 1, kBlock)), dim3(kBlock), 0, s>>>(signs, ws, nvars + 1, prow,
                                                                              prow - nqubits, var);
   return check_launch("clifford_sym_random launch");

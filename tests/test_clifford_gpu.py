@@ -226,6 +226,51 @@ def test_measurement_loop(n):
     np.testing.assert_array_equal(_to_host(tab, n), want_sm)
 
 
+def _per_qubit_loop(tab, qubits, n, exact=False):
+    sample, nrandom = [], 0
+    for q in qubits:
+        p = ch.first_stabilizer(tab, q, n)
+        if p >= 0:
+            _, out = ch._random_outcome(tab, [p], q, n)
+            nrandom += 1
+        else:
+            _, out = ch._determined_outcome(tab, q, n, exact)
+        sample.append(int(out))
+    return sample, nrandom
+
+
+@pytest.mark.parametrize("exact", [False, True])
+@pytest.mark.parametrize("n,nmeas,ngates", [(1, 1, 3), (7, 14, 40), (300, 120, 3000), (1000, 200, 20000)])
+def test_device_measurement_loop_equals_per_qubit_calls(n, nmeas, ngates, exact):
+    """M's one-call device loop against the per-qubit entry points: same
+    outcomes, same collapsed tableau, and the generator left where the
+    per-qubit loop's scalar draws leave it (qubits repeat when nmeas > n)."""
+    rng = np.random.default_rng(n + 3)
+    tab = ch.zero_state(n)
+    ch.apply_circuit(tab, _random_gates(rng, n, ngates), n)
+    qubits = [int(v) for v in rng.integers(0, n, size=nmeas)]
+    ref = tab.clone()
+    np.random.seed(n)
+    got = ch.M(tab, qubits, n, collapse=True, exact=exact)
+    after = np.random.randint(2**31)
+    np.random.seed(n)
+    want, nrandom = _per_qubit_loop(ref, qubits, n, exact)
+    assert got == want
+    assert after == np.random.randint(2**31)
+    assert torch.equal(tab, ref)
+    if n > 1:
+        assert 0 < nrandom < nmeas  # both branches taken
+
+
+def test_device_measurement_loop_with_no_qubits():
+    tab = ch.zero_state(4)
+    before = tab.clone()
+    np.random.seed(1)
+    assert ch.M(tab, [], 4, collapse=True) == []
+    assert np.random.randint(2**31) == np.random.RandomState(1).randint(2**31)
+    assert torch.equal(tab, before)
+
+
 def test_non_collapsing_measurement_keeps_the_state():
     n = 10
     rng = np.random.default_rng(2)
