@@ -590,6 +590,192 @@ __global__ void k_cliff_meas_finish(int32_t* __restrict__ ctl, const uint8_t* __
   nrandom[0] = count;
 }
 
+// The whole measurement loop of a small tableau (n <= kMeasureSmallMax) in one
+// workgroup of 1024 threads, the phases of the multi-kernel loop separated by
+// barriers: first stabilizer (LDS atomicMin); random outcome as the sliced
+// rowsum (thread = (word, column chunk), counters combined in LDS), sign
+// update, row moves; determined outcome (wave per column, ballot parity) and
+// its exact phase (wave per qubit column, LDS sum).  Every thread reaches
+// every barrier; the branch taken is uniform (read from LDS).
+constexpr int kMeasureSmallMax = 256;
+constexpr int kMeasureThreads = 1024;
+
+__global__ __launch_bounds__(1024) void k_cliff_measure_small(uint32_t* __restrict__ tab, int n, int64_t ws,
+                                                              const int32_t* __restrict__ qubits, int m,
+                                                              const uint8_t* __restrict__ draws, int exact,
+                                                              int32_t* __restrict__ sample,
+                                                              int32_t* __restrict__ nrandom) {
+  __shared__ int s_best;
+  __shared__ uint32_t s_acc;
+  __shared__ uint32_t s_part[2 * kMeasureThreads];
+  __shared__ uint32_t s_hm[(2 * kMeasureSmallMax + 1 + 31) / 32];
+  const int tid = (int)threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6, nwaves = kMeasureThreads >> 6;
+  const int64_t C = 2 * (int64_t)n + 1;
+  const int W = (int)((C + 31) >> 5);
+  const int nchunks = kMeasureThreads / W;  // >= 60 for n <= 256
+  const int kchunk = (n + nchunks - 1) / nchunks;
+  const int nw = (n + 31) >> 5;
+  const int sh = n & 31;
+  uint32_t* rcol = tab + 2 * (int64_t)n * ws;
+  int count = 0;
+  for (int i = 0; i < m; ++i) {
+    const int q = qubits[i];
+    const uint32_t* xq = tab + (int64_t)q * ws;
+    if (tid == 0) {
+      s_best = 0x7fffffff;
+      s_acc = 0;
+    }
+    __syncthreads();
+    // first stabilizer row with x_q = 1; the x_q bits of rows < 2n kept in
+    // LDS (the rowsum below rewrites column q)
+    if (tid < W) {
+      const uint32_t xw = xq[tid] & rows_below(tid, 2 * (int64_t)n);
+      s_hm[tid] = xw;
+      const uint32_t mk = xw & ~rows_below(tid, n);
+      if (mk) atomicMin(&s_best, tid * 32 + __builtin_ctz(mk));
+    }
+    __syncthreads();
+    const int best = s_best;
+    int outcome;
+    if (best != 0x7fffffff) {
+      const int64_t p = best;  // row n + (p - n)
+      const int w = tid % W, chunk = tid / W;
+      uint32_t c0 = 0, c1 = 0;
+      if (chunk < nchunks) {
+        uint32_t h = s_hm[w];
+        if (w == (p >> 5)) h &= ~(1u << (p & 31));
+        const int k0 = chunk * kchunk, k1 = min(n, k0 + kchunk);
+        if (h) {
+          for (int k = k0; k < k1; ++k) {
+            uint32_t* xc = tab + (int64_t)k * ws;
+            uint32_t* zc = tab + ((int64_t)n + k) * ws;
+            const uint32_t xp = bit_of(xc, p), zp = bit_of(zc, p);
+            if (!(xp | zp)) continue;
+            const uint32_t X = xc[w], Z = zc[w];
+            uint32_t P, M;
+            if (xp && !zp) {
+              P = X & Z;
+              M = ~X & Z;
+            } else if (!xp) {
+              P = X & ~Z;
+              M = X & Z;
+            } else {
+              P = ~X & Z;
+              M = X & ~Z;
+            }
+            c1 ^= c0 & P;
+            c0 ^= P;
+            c1 ^= ~c0 & M;
+            c0 ^= M;
+            if (xp) xc[w] = X ^ h;
+            if (zp) zc[w] = Z ^ h;
+          }
+        }
+      }
+      s_part[2 * tid] = c0;
+      s_part[2 * tid + 1] = c1;
+      __syncthreads();
+      if (tid < W) {
+        uint32_t h = s_hm[tid];
+        if (tid == (p >> 5)) h &= ~(1u << (p & 31));
+        uint32_t a0 = 0, a1 = 0;
+        for (int y = 0; y < nchunks; ++y) {
+          const uint32_t b0 = s_part[2 * (y * W + tid)], b1 = s_part[2 * (y * W + tid) + 1];
+          a1 ^= b1 ^ (a0 & b0);
+          a0 ^= b0;
+        }
+        const uint32_t r = rcol[tid];
+        const uint32_t rp = bit_of(rcol, p) ? ~0u : 0u;
+        const uint32_t rnew = a0 | (a1 ^ r ^ rp);
+        if (h) rcol[tid] = (r & ~h) | (rnew & h);
+      }
+      __syncthreads();
+      outcome = draws[count] & 1;
+      for (int64_t c = tid; c < C; c += kMeasureThreads) {
+        uint32_t* col = tab + c * ws;
+        const int64_t d = p - n;
+        const uint32_t b = bit_of(col, p);
+        col[d >> 5] = (col[d >> 5] & ~(1u << (d & 31))) | (b << (d & 31));
+        const uint32_t nb = c == (int64_t)n + q ? 1u : (c == C - 1 ? (uint32_t)outcome : 0u);
+        col[p >> 5] = (col[p >> 5] & ~(1u << (p & 31))) | (nb << (p & 31));
+      }
+      ++count;
+    } else {
+      // scratch row <- XOR of the selected stabilizer rows, wave per column
+      for (int64_t c = wave; c < C; c += nwaves) {
+        uint32_t* col = tab + c * ws;
+        uint32_t acc = 0;
+        if (lane < nw) {
+          const uint32_t mk = xq[lane] & rows_below(lane, n);
+          if (mk) {
+            const int64_t a = ((int64_t)n + lane * 32) >> 5;
+            uint32_t v = col[a] >> sh;
+            if (sh && a + 1 < ws) v |= col[a + 1] << (32 - sh);
+            acc = v & mk;
+          }
+        }
+        const uint32_t parity = (uint32_t)(__builtin_popcountll(__ballot(__builtin_popcount(acc) & 1)) & 1);
+        if (lane == 0) {
+          const int64_t s2 = 2 * (int64_t)n;
+          col[s2 >> 5] = (col[s2 >> 5] & ~(1u << (s2 & 31))) | (parity << (s2 & 31));
+        }
+      }
+      if (exact) {
+        __syncthreads();
+        for (int k = wave; k < n; k += nwaves) {
+          const uint32_t* xc = tab + (int64_t)k * ws;
+          const uint32_t* zc = tab + ((int64_t)n + k) * ws;
+          uint32_t X = 0, Z = 0;
+          if (lane < nw) {
+            const uint32_t mk = xq[lane] & rows_below(lane, n);
+            if (mk) {
+              const int64_t a = ((int64_t)n + lane * 32) >> 5;
+              X = xc[a] >> sh;
+              Z = zc[a] >> sh;
+              if (sh && a + 1 < ws) {
+                X |= xc[a + 1] << (32 - sh);
+                Z |= zc[a + 1] << (32 - sh);
+              }
+              X &= mk;
+              Z &= mk;
+            }
+          }
+          uint32_t pz = Z;
+          pz ^= pz << 1;
+          pz ^= pz << 2;
+          pz ^= pz << 4;
+          pz ^= pz << 8;
+          pz ^= pz << 16;
+          const uint64_t odd = __ballot(__builtin_popcount(Z) & 1);
+          const uint32_t before = (uint32_t)(__builtin_popcountll(odd & ((1ull << lane) - 1ull)) & 1);
+          pz = (pz << 1) ^ (before ? ~0u : 0u);
+          const uint32_t cr = (uint32_t)__builtin_popcount(X & pz) & 1u;
+          const uint32_t yc = (uint32_t)__builtin_popcount(X & Z);
+          const uint32_t cb = (uint32_t)(__builtin_popcountll(__ballot(cr)) & 1);
+          const uint32_t y = (uint32_t)__builtin_popcountll(__ballot(yc & 1u)) +
+                             2u * (uint32_t)__builtin_popcountll(__ballot(yc & 2u));
+          const uint32_t add = (2u * cb + y) & 3u;
+          if (lane == 0 && add) atomicAdd(&s_acc, add);
+        }
+      }
+      __syncthreads();
+      const int64_t s2 = 2 * (int64_t)n;
+      outcome = (int)(((rcol[s2 >> 5] >> (s2 & 31)) & 1u) ^ (exact ? ((s_acc >> 1) & 1u) : 0u));
+    }
+    __syncthreads();  // every thread has read the scratch sign
+    if (tid == 0) {
+      sample[i] = outcome;
+      if (best == 0x7fffffff && exact) {
+        const int64_t s2 = 2 * (int64_t)n;
+        rcol[s2 >> 5] = (rcol[s2 >> 5] & ~(1u << (s2 & 31))) | ((uint32_t)outcome << (s2 & 31));
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) nrandom[0] = count;
+}
+
 // ---- general rowsum (clifford_operations_cpu.py:337-385) ----
 
 // New sign of every pair from the state before the call: rnew[j], and
@@ -1215,6 +1401,18 @@ int qj_clifford_measure(uint32_t* tab, int nqubits, const int32_t* qubits, int m
   hipStream_t s = (hipStream_t)stream;
   const int64_t W = cliff_words(nqubits), ws = cliff_stride(nqubits);
   const int64_t C = 2 * (int64_t)nqubits + 1;
+  const char* mode = std::getenv("QJ_CLIFFORD_MEASURE");
+  if (nqubits <= kMeasureSmallMax && !(mode && std::strcmp(mode, "kernels") == 0)) {
+    void* dq = nullptr;
+    int ticket = -1;
+    rc = ring_upload(qubits, (size_t)m * sizeof(int32_t), s, &dq, &ticket);
+    if (rc) return rc;
+    k_cliff_measure_small<<<dim3(1), dim3(kMeasureThreads), 0, s>>>(
+        tab, nqubits, ws, static_cast<const int32_t*>(dq), m, draws, exact, sample_out, nrandom_out);
+    rc = check_launch("clifford_measure launch");
+    ring_release(ticket, s);
+    return rc;
+  }
   int nchunks = 0, kchunk = 0;
   column_chunks(nqubits, W, &nchunks, &kchunk);
   DeviceScratch* sc = nullptr;
