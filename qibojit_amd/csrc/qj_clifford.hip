@@ -590,14 +590,15 @@ __global__ void k_cliff_meas_finish(int32_t* __restrict__ ctl, const uint8_t* __
   nrandom[0] = count;
 }
 
-// The whole measurement loop of a small tableau (n <= kMeasureSmallMax) in one
+// The whole measurement loop of a small tableau (n <= kMeasureSmallCap) in one
 // workgroup of 1024 threads, the phases of the multi-kernel loop separated by
 // barriers: first stabilizer (LDS atomicMin); random outcome as the sliced
 // rowsum (thread = (word, column chunk), counters combined in LDS), sign
 // update, row moves; determined outcome (wave per column, ballot parity) and
 // its exact phase (wave per qubit column, LDS sum).  Every thread reaches
 // every barrier; the branch taken is uniform (read from LDS).
-constexpr int kMeasureSmallMax = 256;
+constexpr int kMeasureSmallMax = 256;   // default: one workgroup up to here
+constexpr int kMeasureSmallCap = 1024;  // largest n the kernel takes
 constexpr int kMeasureThreads = 1024;
 
 __global__ __launch_bounds__(1024) void k_cliff_measure_small(uint32_t* __restrict__ tab, int n, int64_t ws,
@@ -608,12 +609,12 @@ __global__ __launch_bounds__(1024) void k_cliff_measure_small(uint32_t* __restri
   __shared__ int s_best;
   __shared__ uint32_t s_acc;
   __shared__ uint32_t s_part[2 * kMeasureThreads];
-  __shared__ uint32_t s_hm[(2 * kMeasureSmallMax + 1 + 31) / 32];
+  __shared__ uint32_t s_hm[(2 * kMeasureSmallCap + 1 + 31) / 32];
   const int tid = (int)threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6, nwaves = kMeasureThreads >> 6;
   const int64_t C = 2 * (int64_t)n + 1;
   const int W = (int)((C + 31) >> 5);
-  const int nchunks = kMeasureThreads / W;  // >= 60 for n <= 256
+  const int nchunks = kMeasureThreads / W;  // >= 15 for n <= 1024
   const int kchunk = (n + nchunks - 1) / nchunks;
   const int nw = (n + 31) >> 5;
   const int sh = n & 31;
@@ -1402,7 +1403,9 @@ int qj_clifford_measure(uint32_t* tab, int nqubits, const int32_t* qubits, int m
   const int64_t W = cliff_words(nqubits), ws = cliff_stride(nqubits);
   const int64_t C = 2 * (int64_t)nqubits + 1;
   const char* mode = std::getenv("QJ_CLIFFORD_MEASURE");
-  if (nqubits <= kMeasureSmallMax && !(mode && std::strcmp(mode, "kernels") == 0)) {
+  const bool force_kernels = mode && std::strcmp(mode, "kernels") == 0;
+  const bool force_small = mode && std::strcmp(mode, "small") == 0;
+  if (!force_kernels && (nqubits <= kMeasureSmallMax || (force_small && nqubits <= kMeasureSmallCap))) {
     void* dq = nullptr;
     int ticket = -1;
     rc = ring_upload(qubits, (size_t)m * sizeof(int32_t), s, &dq, &ticket);

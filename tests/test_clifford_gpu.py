@@ -239,7 +239,7 @@ def _per_qubit_loop(tab, qubits, n, exact=False):
     return sample, nrandom
 
 
-@pytest.mark.parametrize("path", ["auto", "kernels"])
+@pytest.mark.parametrize("path", ["auto", "kernels", "small"])
 @pytest.mark.parametrize("exact", [False, True])
 @pytest.mark.parametrize("n,nmeas,ngates", [(1, 1, 3), (7, 14, 40), (33, 40, 400), (256, 100, 3000),
                                             (300, 120, 3000), (1000, 200, 20000)])
@@ -248,11 +248,15 @@ def test_device_measurement_loop_equals_per_qubit_calls(n, nmeas, ngates, exact,
     outcomes, same collapsed tableau, and the generator left where the
     per-qubit loop's scalar draws leave it (qubits repeat when nmeas > n).
     Up to 256 qubits the loop runs in one workgroup unless the per-step
-    kernels are forced."""
+    kernels are forced; the one-workgroup loop can be forced up to 1024."""
     if path == "kernels":
         if n > 256:
-            pytest.skip("the per-step kernels are the only path above 256 qubits")
+            pytest.skip("the per-step kernels are the default above 256 qubits")
         monkeypatch.setenv("QJ_CLIFFORD_MEASURE", "kernels")
+    if path == "small":
+        if n <= 256:
+            pytest.skip("the one-workgroup loop is the default up to 256 qubits")
+        monkeypatch.setenv("QJ_CLIFFORD_MEASURE", "small")
     rng = np.random.default_rng(n + 3)
     tab = ch.zero_state(n)
     ch.apply_circuit(tab, _random_gates(rng, n, ngates), n)
