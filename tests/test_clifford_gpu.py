@@ -78,11 +78,13 @@ def test_single_gate(name, n):
         np.testing.assert_array_equal(_to_host(tab, n), cr.apply(sm.copy(), name, qs, n), err_msg=f"{name} {qs}")
 
 
-@pytest.mark.parametrize("kernel", ["lds", "global"])
+@pytest.mark.parametrize("kernel", ["lds1", "lds4", "lds8", "global"])
 @pytest.mark.parametrize("n,ngates", [(1, 50), (3, 400), (17, 600), (100, 1500), (1000, 600)])
 def test_circuit_matches_gate_by_gate(n, ngates, kernel, monkeypatch):
     if kernel == "global":
         monkeypatch.setenv("QJ_CLIFFORD_KERNEL", "global")
+    if kernel.startswith("lds"):
+        monkeypatch.setenv("QJ_CLIFFORD_WAVES", kernel[3:])
     rng = np.random.default_rng(1000 * n + ngates)
     sm = _dense(rng, n)
     gates = _random_gates(rng, n, ngates)
@@ -91,7 +93,9 @@ def test_circuit_matches_gate_by_gate(n, ngates, kernel, monkeypatch):
     np.testing.assert_array_equal(_to_host(tab, n), _oracle_run(sm.copy(), gates, n))
 
 
-def test_compiled_schedule_applied_twice():
+@pytest.mark.parametrize("waves", ["1", "4", "8"])
+def test_compiled_schedule_applied_twice(waves, monkeypatch):
+    monkeypatch.setenv("QJ_CLIFFORD_WAVES", waves)
     rng = np.random.default_rng(44)
     n = 60
     gates = _random_gates(rng, n, 2000)
@@ -107,8 +111,10 @@ def test_compiled_schedule_applied_twice():
         ch.apply_schedule(ch.zero_state(n + 1), sched, n + 1)
 
 
-def test_circuit_of_dependent_chain():
+@pytest.mark.parametrize("waves", ["1", "4", "8"])
+def test_circuit_of_dependent_chain(waves, monkeypatch):
     """Gates that all share a qubit: one gate per group, order kept."""
+    monkeypatch.setenv("QJ_CLIFFORD_WAVES", waves)
     rng = np.random.default_rng(3)
     n = 6
     gates = []
