@@ -1242,7 +1242,7 @@ __device__ __forceinline__ uint32_t run_waves_sized(uint32_t* col0, int64_t zoff
   return run_group_waves<KIND, kCliffGroup / NW, NW>(col0, zoff, cs, qmax, sv, gd, v);
 }
 
-template <int NW>
+template <int NW, bool LDS>
 __global__ __launch_bounds__(64 * NW) void k_cliff_circuit_waves(uint32_t* __restrict__ tab, int n, int64_t ws,
                                                                  int64_t W, int T, const uint32_t* __restrict__ desc,
                                                                  int64_t ngroups) {
@@ -1257,13 +1257,20 @@ __global__ __launch_bounds__(64 * NW) void k_cliff_circuit_waves(uint32_t* __res
   const int my = lane < nact ? lane : nact - 1;
   const int64_t C = 2 * (int64_t)n + 1;
   const int64_t total = C * T;
-  for (int64_t e = tid; e < total; e += 64 * NW) {
-    const int64_t c = e / T;
-    const int tt = (int)(e - c * T);
-    slice[e] = tt < nact ? tab[c * ws + w0 + tt] : 0u;
+  uint32_t* col0;
+  int64_t cs;
+  if (LDS) {
+    for (int64_t e = tid; e < total; e += 64 * NW) {
+      const int64_t c = e / T;
+      const int tt = (int)(e - c * T);
+      slice[e] = tt < nact ? tab[c * ws + w0 + tt] : 0u;
+    }
+    col0 = slice + my;
+    cs = T;
+  } else {
+    col0 = tab + w0 + my;
+    cs = ws;
   }
-  uint32_t* col0 = slice + my;
-  const int64_t cs = T;
   const int64_t total_words = ngroups * kDescWords;
   const int64_t nchunks = (ngroups + kDescChunk - 1) / kDescChunk;
   for (int64_t e = tid; e < kDescChunkWords; e += 64 * NW) ring[e] = e < total_words ? desc[e] : 0u;
@@ -1329,11 +1336,13 @@ __global__ __launch_bounds__(64 * NW) void k_cliff_circuit_waves(uint32_t* __res
     for (int u = 0; u < NW; ++u) r ^= ring[u * 64 + lane];
     col0[roff] = r;
   }
-  __syncthreads();
-  for (int64_t e = tid; e < total; e += 64 * NW) {
-    const int64_t c = e / T;
-    const int tt = (int)(e - c * T);
-    if (tt < nact) tab[c * ws + w0 + tt] = slice[e];
+  if (LDS) {
+    __syncthreads();
+    for (int64_t e = tid; e < total; e += 64 * NW) {
+      const int64_t c = e / T;
+      const int tt = (int)(e - c * T);
+      if (tt < nact) tab[c * ws + w0 + tt] = slice[e];
+    }
   }
 }
 
@@ -1368,16 +1377,16 @@ void launch_circuit(uint32_t* tab, int nqubits, const uint32_t* ddesc, int64_t n
   if (use_lds && nwaves == 4) {
     const size_t lds = kDescRingBytes + (size_t)T * C * 4;
     if (lds > 65536)
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cliff_circuit_waves<4>),
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cliff_circuit_waves<4, true>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    k_cliff_circuit_waves<4><<<dim3((unsigned)nwg), dim3(256), lds, s>>>(tab, nqubits, ws, W, (int)T, ddesc,
+    k_cliff_circuit_waves<4, true><<<dim3((unsigned)nwg), dim3(256), lds, s>>>(tab, nqubits, ws, W, (int)T, ddesc,
                                                                          ngroups);
   } else if (use_lds && nwaves == 8) {
     const size_t lds = kDescRingBytes + (size_t)T * C * 4;
     if (lds > 65536)
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cliff_circuit_waves<8>),
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cliff_circuit_waves<8, true>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    k_cliff_circuit_waves<8><<<dim3((unsigned)nwg), dim3(512), lds, s>>>(tab, nqubits, ws, W, (int)T, ddesc,
+    k_cliff_circuit_waves<8, true><<<dim3((unsigned)nwg), dim3(512), lds, s>>>(tab, nqubits, ws, W, (int)T, ddesc,
                                                                          ngroups);
   } else if (use_lds) {
     const size_t lds = kDescRingBytes + (size_t)T * C * 4;
@@ -1387,8 +1396,15 @@ void launch_circuit(uint32_t* tab, int nqubits, const uint32_t* ddesc, int64_t n
     k_cliff_circuit<true><<<dim3((unsigned)nwg), dim3(64), lds, s>>>(tab, nqubits, ws, W, (int)T, ddesc, ngroups);
   } else {
     const int64_t gwg = (W + 63) / 64;
-    k_cliff_circuit<false><<<dim3((unsigned)gwg), dim3(64), kDescRingBytes, s>>>(tab, nqubits, ws, W, 64, ddesc,
-                                                                                ngroups);
+    if (nwaves == 8)
+      k_cliff_circuit_waves<8, false><<<dim3((unsigned)gwg), dim3(512), kDescRingBytes, s>>>(tab, nqubits, ws, W, 64,
+                                                                                           ddesc, ngroups);
+    else if (nwaves == 4)
+      k_cliff_circuit_waves<4, false><<<dim3((unsigned)gwg), dim3(256), kDescRingBytes, s>>>(tab, nqubits, ws, W, 64,
+                                                                                           ddesc, ngroups);
+    else
+      k_cliff_circuit<false><<<dim3((unsigned)gwg), dim3(64), kDescRingBytes, s>>>(tab, nqubits, ws, W, 64, ddesc,
+                                                                                  ngroups);
   }
 }
 

@@ -78,13 +78,14 @@ def test_single_gate(name, n):
         np.testing.assert_array_equal(_to_host(tab, n), cr.apply(sm.copy(), name, qs, n), err_msg=f"{name} {qs}")
 
 
-@pytest.mark.parametrize("kernel", ["lds1", "lds4", "lds8", "global"])
+@pytest.mark.parametrize("kernel", ["lds1", "lds4", "lds8", "global1", "global4", "global8"])
 @pytest.mark.parametrize("n,ngates", [(1, 50), (3, 400), (17, 600), (100, 1500), (1000, 600)])
 def test_circuit_matches_gate_by_gate(n, ngates, kernel, monkeypatch):
-    if kernel == "global":
+    """Every gate-list variant: columns in an LDS slice or in L2, with 1, 4
+    or 8 waves per workgroup sharing each group's slots."""
+    if kernel.startswith("global"):
         monkeypatch.setenv("QJ_CLIFFORD_KERNEL", "global")
-    if kernel.startswith("lds"):
-        monkeypatch.setenv("QJ_CLIFFORD_WAVES", kernel[3:])
+    monkeypatch.setenv("QJ_CLIFFORD_WAVES", kernel[-1])
     rng = np.random.default_rng(1000 * n + ngates)
     sm = _dense(rng, n)
     gates = _random_gates(rng, n, ngates)
