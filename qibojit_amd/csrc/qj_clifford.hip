@@ -1238,6 +1238,7 @@ __device__ __forceinline__ uint32_t run_waves_sized(uint32_t* col0, int64_t zoff
                                                     const uint32_t* gd, int v) {
   if (sv <= 0) return 0u;
   if (sv == 1) return run_group_waves<KIND, 1, NW>(col0, zoff, cs, qmax, sv, gd, v);
+  if (NW >= 16) return 0u;  // one slot per wave at most
   if (sv == 2 || NW >= 8) return run_group_waves<KIND, 2, NW>(col0, zoff, cs, qmax, sv, gd, v);
   return run_group_waves<KIND, kCliffGroup / NW, NW>(col0, zoff, cs, qmax, sv, gd, v);
 }
@@ -1373,7 +1374,10 @@ void launch_circuit(uint32_t* tab, int nqubits, const uint32_t* ddesc, int64_t n
   const char* waves = std::getenv("QJ_CLIFFORD_WAVES");
   // waves per workgroup of the LDS variant (8 measured fastest: 56 M gates/s
   // at 1000 qubits against 42 M with 4 and 20 M with 1)
-  const int nwaves = !waves ? 8 : (std::strcmp(waves, "4") == 0 ? 4 : (std::strcmp(waves, "1") == 0 ? 1 : 8));
+  int nwaves = 8;
+  if (waves && std::strcmp(waves, "1") == 0) nwaves = 1;
+  if (waves && std::strcmp(waves, "4") == 0) nwaves = 4;
+  if (waves && std::strcmp(waves, "16") == 0) nwaves = 16;
   if (use_lds && nwaves == 4) {
     const size_t lds = kDescRingBytes + (size_t)T * C * 4;
     if (lds > 65536)
@@ -1381,6 +1385,13 @@ void launch_circuit(uint32_t* tab, int nqubits, const uint32_t* ddesc, int64_t n
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     k_cliff_circuit_waves<4, true><<<dim3((unsigned)nwg), dim3(256), lds, s>>>(tab, nqubits, ws, W, (int)T, ddesc,
                                                                          ngroups);
+  } else if (use_lds && nwaves == 16) {
+    const size_t lds = kDescRingBytes + (size_t)T * C * 4;
+    if (lds > 65536)
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cliff_circuit_waves<16, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_cliff_circuit_waves<16, true><<<dim3((unsigned)nwg), dim3(1024), lds, s>>>(tab, nqubits, ws, W, (int)T, ddesc,
+                                                                             ngroups);
   } else if (use_lds && nwaves == 8) {
     const size_t lds = kDescRingBytes + (size_t)T * C * 4;
     if (lds > 65536)
@@ -1396,7 +1407,10 @@ void launch_circuit(uint32_t* tab, int nqubits, const uint32_t* ddesc, int64_t n
     k_cliff_circuit<true><<<dim3((unsigned)nwg), dim3(64), lds, s>>>(tab, nqubits, ws, W, (int)T, ddesc, ngroups);
   } else {
     const int64_t gwg = (W + 63) / 64;
-    if (nwaves == 8)
+    if (nwaves == 16)
+      k_cliff_circuit_waves<16, false><<<dim3((unsigned)gwg), dim3(1024), kDescRingBytes, s>>>(tab, nqubits, ws, W,
+                                                                                             64, ddesc, ngroups);
+    else if (nwaves == 8)
       k_cliff_circuit_waves<8, false><<<dim3((unsigned)gwg), dim3(512), kDescRingBytes, s>>>(tab, nqubits, ws, W, 64,
                                                                                            ddesc, ngroups);
     else if (nwaves == 4)

@@ -78,14 +78,14 @@ def test_single_gate(name, n):
         np.testing.assert_array_equal(_to_host(tab, n), cr.apply(sm.copy(), name, qs, n), err_msg=f"{name} {qs}")
 
 
-@pytest.mark.parametrize("kernel", ["lds1", "lds4", "lds8", "global1", "global4", "global8"])
+@pytest.mark.parametrize("kernel", ["lds1", "lds4", "lds8", "lds16", "global1", "global4", "global8", "global16"])
 @pytest.mark.parametrize("n,ngates", [(1, 50), (3, 400), (17, 600), (100, 1500), (1000, 600)])
 def test_circuit_matches_gate_by_gate(n, ngates, kernel, monkeypatch):
-    """Every gate-list variant: columns in an LDS slice or in L2, with 1, 4
-    or 8 waves per workgroup sharing each group's slots."""
+    """Every gate-list variant: columns in an LDS slice or in L2, with 1, 4,
+    8 or 16 waves per workgroup sharing each group's slots."""
     if kernel.startswith("global"):
         monkeypatch.setenv("QJ_CLIFFORD_KERNEL", "global")
-    monkeypatch.setenv("QJ_CLIFFORD_WAVES", kernel[-1])
+    monkeypatch.setenv("QJ_CLIFFORD_WAVES", kernel.lstrip("ldsglobal"))
     rng = np.random.default_rng(1000 * n + ngates)
     sm = _dense(rng, n)
     gates = _random_gates(rng, n, ngates)
@@ -94,7 +94,7 @@ def test_circuit_matches_gate_by_gate(n, ngates, kernel, monkeypatch):
     np.testing.assert_array_equal(_to_host(tab, n), _oracle_run(sm.copy(), gates, n))
 
 
-@pytest.mark.parametrize("waves", ["1", "4", "8"])
+@pytest.mark.parametrize("waves", ["1", "4", "8", "16"])
 def test_compiled_schedule_applied_twice(waves, monkeypatch):
     monkeypatch.setenv("QJ_CLIFFORD_WAVES", waves)
     rng = np.random.default_rng(44)
@@ -112,7 +112,7 @@ def test_compiled_schedule_applied_twice(waves, monkeypatch):
         ch.apply_schedule(ch.zero_state(n + 1), sched, n + 1)
 
 
-@pytest.mark.parametrize("waves", ["1", "4", "8"])
+@pytest.mark.parametrize("waves", ["1", "4", "8", "16"])
 def test_circuit_of_dependent_chain(waves, monkeypatch):
     """Gates that all share a qubit: one gate per group, order kept."""
     monkeypatch.setenv("QJ_CLIFFORD_WAVES", waves)
