@@ -466,17 +466,20 @@ def run_measurement_ops(backend, n, dtype):
     return out
 
 
-def run_capacity(backend, n=33):
-    """One dense 1-qubit gate on an n-qubit complex128 state (137 GB at n=33)."""
+def run_capacity(backend, n=33, dtype="complex128"):
+    """One dense 1-qubit gate on an n-qubit state held whole on the device
+    (137 GB at 33 qubits complex128 or 34 qubits complex64)."""
     import torch
 
     from qibojit_amd._qibo import gates
 
-    backend.set_dtype("complex128")
+    backend.set_dtype(dtype)
+    amp = 8 if dtype == "complex64" else 16
+    torch.cuda.empty_cache()
     try:
         state = backend.zero_state(n)  # values do not affect the timing
-    except (torch.cuda.OutOfMemoryError, MemoryError, RuntimeError) as exc:
-        return {"nqubits": n, "error": str(exc)[:200]}
+    except MemoryError as exc:
+        return {"nqubits": n, "dtype": dtype, "error": str(exc)[:200]}
     g = gates.H(n // 2)
     state = backend.apply_gate(g, state, n)  # warm
     torch.cuda.synchronize()
@@ -486,10 +489,12 @@ def run_capacity(backend, n=33):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
-    b = 2 * 16 * 2**n
+    b = 2 * amp * 2**n
+    norm = backend.norm_squared(state)
     del state
     torch.cuda.empty_cache()
-    return {"nqubits": n, "state_GB": 16 * 2**n / 1e9, "gate": f"H({n // 2})", "ms": ms, "GBps": b / ms * 1e3 / 1e9}
+    return {"nqubits": n, "dtype": dtype, "state_GB": amp * 2**n / 1e9, "gate": f"H({n // 2})", "ms": ms,
+            "GBps": b / ms * 1e3 / 1e9, "final_norm2": norm}
 
 
 def _shifted_energies(ansatz, base, shifts, chunk=64):
@@ -906,42 +911,57 @@ def main():
     n = args.nqubits
     print(f"# sweep complex128 n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
     main_res = run_sweep(backend, n, "complex128", args.steps, args.warmup)
-    extras = {}
+    # Detail lines first (one JSON object per line); the last stdout line is
+    # the compact headline.
+    emit("per_gate_complex128", [_rounded(r) for r in main_res["rows"]])
+    summary = {}
     if not args.only_sweep:
         print("# sweep complex64", file=sys.stderr, flush=True)
         c64 = run_sweep(backend, n, "complex64", args.steps, args.warmup)
-        extras["sweep_complex64"] = {"GBps": c64["GBps"], "ms_per_step": c64["ms_per_step"],
-                                     "frac": c64["GBps"] / HBM_PEAK_GBS, "rows": c64["rows"]}
+        emit("sweep_complex64", {"GBps": c64["GBps"], "ms_per_step": c64["ms_per_step"],
+                                 "frac": c64["GBps"] / HBM_PEAK_GBS, "rows": [_rounded(r) for r in c64["rows"]]})
+        summary["sweep_c64_GBps"] = round(c64["GBps"], 1)
     if not args.no_extras and not args.only_sweep:
         print("# measurement ops", file=sys.stderr, flush=True)
-        extras["measurement_complex128"] = run_measurement_ops(backend, n, "complex128")
+        emit("measurement_complex128", run_measurement_ops(backend, n, "complex128"))
         print("# qft 32q complex64", file=sys.stderr, flush=True)
-        extras["qft_32q_complex64_fused"] = run_qft(backend, 32, "complex64", fused=True)
-        extras["qft_32q_complex64"] = run_qft(backend, 32, "complex64")
-        print("# capacity 33q complex128", file=sys.stderr, flush=True)
-        extras["capacity_33q_complex128"] = run_capacity(backend, 33)
+        qft = run_qft(backend, 32, "complex64", fused=True)
+        emit("qft_32q_complex64_fused", qft)
+        summary["qft_32q_c64_s"] = round(qft["seconds"], 4)
+        emit("qft_32q_complex64", run_qft(backend, 32, "complex64"))
+        print("# capacity 33q complex128, 34q complex64", file=sys.stderr, flush=True)
+        cap = run_capacity(backend, 33, "complex128")
+        emit("capacity_33q_complex128", cap)
+        cap64 = run_capacity(backend, 34, "complex64")
+        emit("capacity_34q_complex64", cap64)
+        summary["capacity_GBps"] = {"33q_c128": round(cap.get("GBps", float("nan")), 1),
+                                    "34q_c64": round(cap64.get("GBps", float("nan")), 1)}
         print("# precision 32q complex64 vs complex128", file=sys.stderr, flush=True)
-        extras["mfma_kernel_32q_complex64"] = run_mfma_kernel(backend)
-        extras["precision_32q_complex64"] = run_precision(backend)
+        emit("mfma_kernel_32q_complex64", run_mfma_kernel(backend))
+        emit("precision_32q_complex64", run_precision(backend))
         print("# adjoint gradients 24q", file=sys.stderr, flush=True)
-        extras["adjoint_24q_8layers_batch16"] = run_adjoint(backend)
+        adj = run_adjoint(backend)
+        emit("adjoint_24q_8layers_batch16", adj)
+        summary["adjoint_24q_c64_grads_per_s"] = round(adj["batched_complex64"]["grads_per_s"], 1)
+        summary["adjoint_vs_plain_c128"] = round(adj["speedup_batched_c64_vs_plain_c128"], 2)
+        summary["adjoint_c64_max_grad_err_vs_fd"] = adj["max_grad_error"]["batched_c64_vs_fd"]
         print("# 1024 ansatz circuits", file=sys.stderr, flush=True)
-        extras["batch_1024_ansatz_complex64"] = run_batch(backend)
+        bat = run_batch(backend)
+        emit("batch_1024_ansatz_complex64", bat)
         print("# clifford tableaux", file=sys.stderr, flush=True)
-        extras["clifford_tableau"] = run_clifford()
+        emit("clifford_tableau", run_clifford())
 
     name, avg_ms, avg_bytes, achieved, launches = dominant_kernel(main_res["rows"])
     roofline = {
         "bound": "hbm",
-        "achieved": achieved,
+        "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS,
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": PROFILED_TRAFFIC.get((name, n)),
-        "traffic_source": "profiles/sweep_30q_c128_hbm_traffic.md" if (name, n) in PROFILED_TRAFFIC else None,
         "kernel": name,
         "launches_per_step": launches,
-        "avg_launch_ms": avg_ms,
+        "avg_launch_ms": round(avg_ms, 4),
         "algorithmic_bytes_per_launch": avg_bytes,
     }
     cpu = None
@@ -951,12 +971,12 @@ def main():
 
     line = {
         "metric": "eff. HBM GB/s (30q gate sweep); QFT circuit s; adjoint grads/s vs plain @1e-6",
-        "value": main_res["GBps"],
+        "value": round(main_res["GBps"], 2),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": main_res["ms_per_step"],
+        "ms_per_step": round(main_res["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -966,15 +986,23 @@ def main():
             "workload": f"{n}-qubit gate sweep: H, X, Z, RZ, CNOT, SWAP, fSim, CU3, 3q Haar on targets 0, {n // 2}, {n - 1}",
             "nqubits": n,
             "gates_per_step": len(main_res["rows"]),
-            "algorithmic_GB_per_step": main_res["bytes_per_step"] / 1e9,
+            "algorithmic_GB_per_step": round(main_res["bytes_per_step"] / 1e9, 3),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "per_gate": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in main_res["rows"]],
-        "extras": extras,
+        "summary": summary,
         "device": torch.cuda.get_device_name(0),
     }
-    print(json.dumps(line), flush=True)
+    print(json.dumps(line, separators=(",", ":")), flush=True)
+
+
+def _rounded(row):
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in row.items()}
+
+
+def emit(name, payload):
+    """One detail record per stdout line, ahead of the headline."""
+    print(json.dumps({"detail": name, "result": payload}, default=str, separators=(",", ":")), flush=True)
 
 
 if __name__ == "__main__":

@@ -2,7 +2,7 @@
 
 The host side mirrors qibojit's NumbaBackend/CupyBackend dispatch
 (src/qibojit/backends/cpu.py:369-635, gpu.py:584-1076): a gate becomes a
-kernel family (GATE_OPS), a kernel-facing matrix (CustomMatrices), a sorted
+kernel family (GATE_OPS), a kernel-facing matrix (KernelMatrices), a sorted
 "qubits tensor" of bit positions and a target count, and then one call into
 ``libqjhip.so``.  Unlike CupyBackend, gate matrices and qubit tensors stay on
 the host (they travel as kernel arguments or through the library's staging
@@ -28,14 +28,14 @@ from qibojit_amd._native import ONE_QUBIT_KINDS, TWO_QUBIT_KINDS, call, i32_arra
 from qibojit_amd._qibo import (
     SHOT_METROPOLIS_THRESHOLD,
     Backend,
+    CircuitResult,
     FusedGate,
     NumpyMatrices,
-    ParametrizedGate,
     QuantumState,
     raise_error,
 )
 from qibojit_amd import tiles
-from qibojit_amd.backends.matrices import CustomMatrices
+from qibojit_amd.backends.matrices import KernelMatrices
 from qibojit_amd.fusion import GATE_OPS, fused_matrix
 
 
@@ -67,6 +67,20 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _device_empty(numel, tdtype, device):
+    """Uninitialised device tensor; an allocation the device cannot hold
+    raises MemoryError (numpy's error for the same case on NumbaBackend),
+    checked against the device's capacity before any allocation is tried."""
+    nbytes = int(numel) * (torch.empty((), dtype=tdtype).element_size())
+    capacity = torch.cuda.get_device_properties(device).total_memory
+    if nbytes > capacity:
+        raise MemoryError(f"{nbytes / 2**30:.1f} GiB do not fit the {capacity / 2**30:.1f} GiB of {device}.")
+    try:
+        return torch.empty(int(numel), dtype=tdtype, device=device)
+    except torch.cuda.OutOfMemoryError as exc:
+        raise MemoryError(f"device allocation of {nbytes / 2**30:.1f} GiB failed: {exc}") from exc
+
+
 class HipBackend(Backend):
     """qibo Backend whose state-vector operations run as gfx950 HIP kernels."""
 
@@ -90,7 +104,7 @@ class HipBackend(Backend):
         self.device = "/GPU:0"
         self._torch_device = torch.device("cuda", 0)
         self.matrices = NumpyMatrices(self.dtype)
-        self.custom_matrices = CustomMatrices(self.dtype)
+        self.custom_matrices = KernelMatrices(self.dtype)
         self.tensor_types = (np.ndarray, torch.Tensor)
         self.supports_multigpu = False
         self.oom_error = (MemoryError, torch.cuda.OutOfMemoryError)
@@ -104,7 +118,7 @@ class HipBackend(Backend):
         self.fusion = True
         # uncontrolled 4/5-target complex64 blocks run on FP32 matrix cores
         self.mfma = True
-        self._custom_matrices_c128 = CustomMatrices("complex128")
+        self._custom_matrices_c128 = KernelMatrices("complex128")
 
     # ------------------------------------------------------------ config --
     def set_device(self, device):
@@ -123,7 +137,7 @@ class HipBackend(Backend):
         if dtype != self.dtype:
             super().set_dtype(dtype)
             self.matrices = NumpyMatrices(self.dtype)
-            self.custom_matrices = CustomMatrices(self.dtype)
+            self.custom_matrices = KernelMatrices(self.dtype)
 
     def set_seed(self, seed):
         super().set_seed(seed)
@@ -139,16 +153,19 @@ class HipBackend(Backend):
         if dtype is None:
             dtype = self.dtype
         tdt = _TORCH_DTYPES[_dtype_name(dtype)]
-        if isinstance(array, torch.Tensor):
-            out = array.to(device=self._torch_device, dtype=tdt)
-            if copy and out.data_ptr() == array.data_ptr():
-                out = out.clone()
-            return out.contiguous()
-        arr = np.asarray(array)
-        if arr.dtype.kind == "c" and tdt in (torch.float32, torch.float64):
-            arr = arr.real
-        arr = np.ascontiguousarray(arr, dtype=_dtype_name(tdt))
-        return torch.from_numpy(arr).to(self._torch_device)
+        try:
+            if isinstance(array, torch.Tensor):
+                out = array.to(device=self._torch_device, dtype=tdt)
+                if copy and out.data_ptr() == array.data_ptr():
+                    out = out.clone()
+                return out.contiguous()
+            arr = np.asarray(array)
+            if arr.dtype.kind == "c" and tdt in (torch.float32, torch.float64):
+                arr = arr.real
+            arr = np.ascontiguousarray(arr, dtype=_dtype_name(tdt))
+            return torch.from_numpy(arr).to(self._torch_device)
+        except torch.cuda.OutOfMemoryError as exc:
+            raise MemoryError(f"cast: device allocation failed: {exc}") from exc
 
     def to_numpy(self, array):
         if isinstance(array, torch.Tensor):
@@ -193,7 +210,7 @@ class HipBackend(Backend):
         tdt = _TORCH_DTYPES[_dtype_name(dtype)]
         n = 1 << nqubits
         numel = n * n if density_matrix else n
-        state = torch.empty(numel, dtype=tdt, device=self._torch_device)
+        state = _device_empty(numel, tdt, self._torch_device)
         call("qj_initial_state", _ptr(state), _NATIVE_CODES[tdt], numel, self._stream())
         return state.reshape(n, n) if density_matrix else state
 
@@ -204,7 +221,7 @@ class HipBackend(Backend):
         tdt = _TORCH_DTYPES[_dtype_name(dtype)]
         n = 1 << nqubits
         numel = n * n if density_matrix else n
-        state = torch.empty(numel, dtype=tdt, device=self._torch_device)
+        state = _device_empty(numel, tdt, self._torch_device)
         value = 1.0 / n if density_matrix else 1.0 / math.sqrt(n)
         call("qj_fill_state", _ptr(state), _NATIVE_CODES[tdt], numel, value, 0.0, self._stream())
         return state.reshape(n, n) if density_matrix else state
@@ -341,22 +358,12 @@ class HipBackend(Backend):
     def _as_custom_matrix(self, gate, matrices=None):
         """Kernel-facing encoding of ``gate`` (cpu.py:519-539), from
         ``matrices`` (default: the backend's, in its dtype)."""
-        name = gate.__class__.__name__
         if matrices is None:
             matrices = self.custom_matrices
         if isinstance(gate, FusedGate):
             return fused_matrix(lambda g: self._as_custom_matrix(g, matrices), gate)
-        _matrix = getattr(matrices, name)
-        if name == "FanOut":
-            return _matrix(*gate.init_args)
-        if isinstance(gate, ParametrizedGate):
-            if name == "GeneralizedRBS":
-                theta = gate.init_kwargs["theta"]
-                phi = gate.init_kwargs["phi"]
-                return _matrix(gate.init_args[0], gate.init_args[1], theta, phi)
-            params = [self.to_numpy(p) if isinstance(p, torch.Tensor) else p for p in gate.parameters]
-            return _matrix(*params)
-        return _matrix(2 ** len(gate.target_qubits)) if callable(_matrix) else _matrix
+        params = [self.to_numpy(p) if isinstance(p, torch.Tensor) else p for p in getattr(gate, "parameters", ())]
+        return matrices.encode(gate, params)
 
     @staticmethod
     def _create_qubits_tensor(gate, nqubits):
@@ -719,31 +726,62 @@ class HipBackend(Backend):
 
     # ------------------------------------------------------- execution --
     def execute_circuit(self, circuit, initial_state=None, nshots=1000):
-        """Apply the circuit's queue to |0...0> or ``initial_state``.
+        """Run ``circuit`` from |0...0> or ``initial_state`` (qibo's
+        execute_circuit contract, which NumbaBackend inherits).
 
         With ``self.fusion`` (default), runs of unitary gates go through the
         fused tile executor (a few HBM passes per run instead of one per
         gate); other gates (measurements, channels, wide gates) are applied
-        one by one, as qibo's execute_circuit does.
+        one by one.  A circuit with measurements returns a CircuitResult
+        whose frequencies and samples are drawn on the device for ``nshots``
+        shots (Metropolis chains from SHOT_METROPOLIS_THRESHOLD shots on);
+        one whose measurements collapse the state runs once per shot
+        (``execute_circuit_repeated``).  A state the device cannot hold
+        raises RuntimeError, as qibo's execute_circuit does.
         """
-        del nshots
+        if getattr(circuit, "repeated_execution", False):
+            return self.execute_circuit_repeated(circuit, nshots, initial_state)
         nqubits = circuit.nqubits
         density_matrix = bool(getattr(circuit, "density_matrix", False))
-        if initial_state is None:
-            state = self.zero_state(nqubits, density_matrix=density_matrix)
-        else:
-            if isinstance(initial_state, QuantumState):
-                initial_state = initial_state.state()
-            dtype = _dtype_name(initial_state.dtype) if hasattr(initial_state, "dtype") else self.dtype
-            state = self.cast(initial_state, dtype=dtype, copy=True)
-        if self.fusion:
-            return QuantumState(self.apply_gates_fused(circuit.queue, state, nqubits), self)
-        for gate in circuit.queue:
-            if density_matrix:
-                state = gate.apply_density_matrix(self, state, nqubits)
+        try:
+            if initial_state is None:
+                state = self.zero_state(nqubits, density_matrix=density_matrix)
             else:
-                state = gate.apply(self, state, nqubits)
-        return QuantumState(state, self)
+                if isinstance(initial_state, QuantumState):
+                    initial_state = initial_state.state()
+                dtype = _dtype_name(initial_state.dtype) if hasattr(initial_state, "dtype") else self.dtype
+                state = self.cast(initial_state, dtype=dtype, copy=True)
+            if self.fusion:
+                state = self.apply_gates_fused(circuit.queue, state, nqubits)
+            else:
+                for gate in circuit.queue:
+                    if density_matrix:
+                        state = gate.apply_density_matrix(self, state, nqubits)
+                    else:
+                        state = gate.apply(self, state, nqubits)
+        except MemoryError:
+            raise_error(
+                RuntimeError,
+                f"State does not fit in {self.device} memory. Please switch the execution device to a "
+                "different one using ``qibo.set_device``.",
+            )
+        if circuit.measurements:
+            result = CircuitResult(state, circuit.measurements, backend=self, nshots=nshots)
+        else:
+            result = QuantumState(state, backend=self)
+        circuit._final_state = result  # pylint: disable=protected-access
+        return result
+
+    def samples_to_binary(self, samples, nqubits):
+        """(nshots, nqubits) int32 0/1 host array, first measured qubit first."""
+        samples = np.asarray(self.to_numpy(samples), dtype=np.int64).reshape(-1)
+        shifts = np.arange(nqubits - 1, -1, -1, dtype=np.int64)
+        return ((samples[:, None] >> shifts) & 1).astype(np.int32)
+
+    def samples_to_decimal(self, samples, nqubits):
+        """Outcome integers (int64 host array) of a (nshots, nqubits) 0/1 array."""
+        samples = np.asarray(self.to_numpy(samples), dtype=np.int64).reshape(-1, nqubits)
+        return samples @ np.left_shift(1, np.arange(nqubits - 1, -1, -1, dtype=np.int64))
 
     def tile_geometry(self, nbits, tdtype):
         """(low bits, max high bits) of fused tiles: 64 KiB of LDS per tile,

@@ -1,78 +1,88 @@
-"""Kernel-facing gate encodings (src/qibojit/backends/matrices.py:7-70).
+"""What the gate kernels read for each gate.
 
-They are not always the mathematical matrix of the gate: a controlled gate
-is passed as its target block (the controls select the amplitudes), U1/CU1
-as the scalar phase e^{i theta} that the z_pow kernel multiplies by, fSim and
-GeneralizedfSim as the 5-vector [u00, u01, u10, u11, e^{-i phi}] the fsim
-kernel reads, DEUTSCH as i * RX(2 theta).  Matrices stay on the host: the
-HIP library copies them into kernel arguments or its staging ring.
+The kernels do not always take a gate's full matrix (the encodings of
+src/qibojit/backends/matrices.py:7-70, which the dispatch contract fixes):
+
+* a controlled gate arrives as the block its controls select, since the
+  controls only choose which amplitudes the kernel visits;
+* U1 / CU1 arrive as the scalar e^{i theta} the z_pow kernel multiplies the
+  |1> half by;
+* fSim / GeneralizedfSim arrive as five numbers, the {|01>, |10>} block row
+  by row and then the |11> phase e^{-i phi}, which the fsim kernel reads;
+* DEUTSCH arrives as i RX(2 theta), its block under two controls.
+
+Every other gate arrives as its textbook matrix over its targets.  The
+encodings are host arrays: the library copies them into kernel arguments
+or its staging ring.
 """
-
-from functools import cached_property
 
 import numpy as np
 
 from qibojit_amd._qibo import NumpyMatrices
 
+# controlled gate -> the uncontrolled gate whose matrix is its selected block
+SELECTED_BLOCK = {
+    "CNOT": "X",
+    "TOFFOLI": "X",
+    "CY": "Y",
+    "CZ": "Z",
+    "CCZ": "Z",
+    "CH": "H",
+    "CSX": "SX",
+    "CSXDG": "SXDG",
+    "CRX": "RX",
+    "CRY": "RY",
+    "CRZ": "RZ",
+    "CU1": "U1",
+    "CU2": "U2",
+    "CU3": "U3",
+}
 
-class CustomMatrices(NumpyMatrices):
-    """Matrices used by the custom operators."""
 
-    def CRX(self, theta):
-        return self.RX(theta)
+def _z_pow_scalar(theta, dtype):
+    return np.asarray(np.exp(1j * theta), dtype=dtype)
 
-    def CRY(self, theta):
-        return self.RY(theta)
 
-    def CRZ(self, theta):
-        return self.RZ(theta)
+def _fsim_five(block, phi, dtype):
+    entries = list(np.asarray(block, dtype=np.complex128).reshape(4)) + [np.exp(-1j * phi)]
+    return np.asarray(entries, dtype=dtype)
 
-    def CU2(self, phi, lam):
-        return self.U2(phi, lam)
 
-    def CU3(self, theta, phi, lam):
-        return self.U3(theta, phi, lam)
+def _fsim_block(theta):
+    c, s = np.cos(theta), -1j * np.sin(theta)
+    return [[c, s], [s, c]]
 
-    def U1(self, theta):
-        return np.asarray(np.exp(1j * theta), dtype=self.dtype)
 
-    def CU1(self, theta):
-        return self.U1(theta)
+class KernelMatrices:
+    """Kernel-facing encodings of gates in one precision ("complex64" or
+    "complex128"), built on the textbook matrices of ``NumpyMatrices``."""
 
-    @cached_property
-    def CZ(self):
-        return self.Z
+    def __init__(self, dtype):
+        self.dtype = str(np.dtype(dtype))
+        self.textbook = NumpyMatrices(self.dtype)
 
-    @cached_property
-    def CCZ(self):
-        return self.Z
-
-    @cached_property
-    def CY(self):
-        return self.Y
-
-    @cached_property
-    def CH(self):
-        return self.H
-
-    @cached_property
-    def CSX(self):
-        return self.SX
-
-    @cached_property
-    def CSXDG(self):
-        return self.SXDG
-
-    def DEUTSCH(self, theta):
-        return 1j * self.RX(2 * theta)
-
-    def fSim(self, theta, phi):
-        cost = np.cos(theta) + 0j
-        isint = -1j * np.sin(theta)
-        phase = np.exp(-1j * phi)
-        return np.array([cost, isint, isint, cost, phase], dtype=self.dtype)
-
-    def GeneralizedfSim(self, u, phi):
-        u = np.asarray(u)
-        phase = np.exp(-1j * phi)
-        return np.array([u[0, 0], u[0, 1], u[1, 0], u[1, 1], phase], dtype=self.dtype)
+    def encode(self, gate, parameters=None):
+        """The array the kernels read for ``gate``; ``parameters`` (host
+        numbers) default to ``gate.parameters``."""
+        name = gate.__class__.__name__
+        params = list(getattr(gate, "parameters", ()) if parameters is None else parameters)
+        kind = SELECTED_BLOCK.get(name, name)
+        if kind == "U1":
+            return _z_pow_scalar(params[0], self.dtype)
+        if kind == "fSim":
+            return _fsim_five(_fsim_block(params[0]), params[1], self.dtype)
+        if kind == "GeneralizedfSim":
+            return _fsim_five(params[0], params[1], self.dtype)
+        if kind == "DEUTSCH":
+            return np.asarray(1j * self.textbook.RX(2 * params[0]), dtype=self.dtype)
+        form = getattr(self.textbook, kind)
+        if kind == "FanOut":
+            return form(*gate.init_args)
+        if kind == "GeneralizedRBS":
+            return form(gate.init_args[0], gate.init_args[1], gate.init_kwargs["theta"], gate.init_kwargs["phi"])
+        if not callable(form):
+            return form
+        if params:
+            return form(*params)
+        # parameter-free families sized by their targets (identity, ...)
+        return form(2 ** len(gate.target_qubits))

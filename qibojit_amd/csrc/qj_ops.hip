@@ -758,27 +758,25 @@ __global__ void __launch_bounds__(kBlock) k_cdf_block_scan(const T* __restrict__
     }
 }
 
-// cdf /= cdf[n-1], then each uniform's first index with cdf > u.
-__global__ void __launch_bounds__(kBlock) k_cdf_normalize(double* __restrict__ cdf, int64_t n) {
-  const double total = cdf[n - 1];
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    cdf[i] = cdf[i] / total;
-}
-
+// Each uniform's first index with cdf[i] / cdf[n-1] > u.  The quotient is
+// NumPy's `cdf /= cdf[-1]` element (one correctly rounded FP64 division), taken
+// at the compared index only, so the CDF buffer is never rewritten: an
+// in-place normalisation would race with workgroups that still read cdf[n-1].
 __global__ void __launch_bounds__(kBlock) k_cdf_search(const double* __restrict__ cdf, int64_t n,
                                                        const double* __restrict__ u, int64_t nshots,
                                                        int64_t* __restrict__ samples) {
+  const double total = cdf[n - 1];
   for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < nshots; j += (int64_t)gridDim.x * kBlock) {
     const double x = u[j];
     int64_t lo = 0, hi = n;
     while (lo < hi) {
       const int64_t mid = lo + (hi - lo) / 2;
-      if (cdf[mid] > x)
+      if (__ddiv_rn(cdf[mid], total) > x)
         hi = mid;
       else
         lo = mid + 1;
     }
-    // cdf[n-1] == 1 > u for u in [0, 1): lo < n; kept in range regardless
+    // cdf[n-1] / total == 1 > u for u in [0, 1): lo < n; kept in range regardless
     samples[j] = lo < n ? lo : n - 1;
   }
 }
@@ -820,7 +818,6 @@ static int sample_shots_impl(const T* probs, int64_t nstates, const double* u, i
     k_cdf_block_offsets<<<dim3(1), dim3(64), 0, s>>>(bsum, nblocks);
     k_cdf_block_scan<T><<<dim3((unsigned)nblocks), dim3(kBlock), 0, s>>>(probs, nstates, bsum, cdf);
   }
-  k_cdf_normalize<<<dim3((unsigned)grid_for(nstates, kBlock)), dim3(kBlock), 0, s>>>(cdf, nstates);
   k_cdf_search<<<dim3((unsigned)grid_for(nshots, kBlock)), dim3(kBlock), 0, s>>>(cdf, nstates, u, nshots, samples);
   return check_launch("sample_shots launch");
 }

@@ -63,6 +63,57 @@ class Backend:
     def sample_frequencies(self, probabilities, nshots):
         return self.calculate_frequencies(self.sample_shots(probabilities, nshots))
 
+    def samples_to_binary(self, samples, nqubits):
+        """(nshots, nqubits) 0/1 array of outcome integers, most significant
+        bit (first measured qubit) first."""
+        samples = np.asarray(self.to_numpy(samples), dtype=np.int64).reshape(-1)
+        shifts = np.arange(nqubits - 1, -1, -1, dtype=np.int64)
+        return ((samples[:, None] >> shifts) & 1).astype(np.int32)
+
+    def samples_to_decimal(self, samples, nqubits):
+        """Outcome integers of a (nshots, nqubits) 0/1 array."""
+        samples = np.asarray(self.to_numpy(samples), dtype=np.int64).reshape(-1, nqubits)
+        weights = np.left_shift(1, np.arange(nqubits - 1, -1, -1, dtype=np.int64))
+        return samples @ weights
+
+    def execute_circuit_repeated(self, circuit, nshots, initial_state=None):
+        """Run ``circuit`` once per shot (qibo's repeated execution, used when
+        a measurement collapses the state): every collapsing measurement draws
+        its outcome from the backend's generator, then one shot of every
+        measured qubit is sampled from the final state.  Density matrices
+        return the average final state with the samples."""
+        from qibojit_amd.qibo_compat.models import CircuitResult, MeasurementOutcomes, QuantumState
+
+        nqubits = circuit.nqubits
+        density_matrix = bool(getattr(circuit, "density_matrix", False))
+        results, total = [], None
+        for _ in range(int(nshots)):
+            if initial_state is None:
+                state = self.zero_state(nqubits, density_matrix=density_matrix)
+            else:
+                state = self.cast(initial_state, copy=True)
+            for gate in circuit.queue:
+                if density_matrix:
+                    state = gate.apply_density_matrix(self, state, nqubits)
+                else:
+                    state = gate.apply(self, state, nqubits)
+            if density_matrix:
+                host = np.asarray(self.to_numpy(state))
+                total = host.copy() if total is None else total + host
+            if circuit.measurements:
+                results.append(CircuitResult(state, circuit.measurements, self, nshots=1).samples()[0])
+        samples = np.asarray(results, dtype=np.int32) if results else None
+        if density_matrix:
+            final = self.cast(total / int(nshots))
+            if samples is None:
+                result = QuantumState(final, self)
+            else:
+                result = CircuitResult(final, circuit.measurements, self, samples=samples, nshots=nshots)
+        else:
+            result = MeasurementOutcomes(circuit.measurements, self, samples=samples, nshots=nshots)
+        circuit._final_state = result  # pylint: disable=protected-access
+        return result
+
     def assert_allclose(self, value, target, rtol=1e-7, atol=0.0):
         value = np.asarray(self.to_numpy(value))
         target = np.asarray(self.to_numpy(target))

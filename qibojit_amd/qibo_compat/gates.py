@@ -370,22 +370,30 @@ class FusedGate(Gate):
 
 
 class M(Gate):
-    """Measurement of ``q``; with ``collapse=True`` the state is projected."""
+    """Measurement of ``q``.  With ``collapse=True`` the state is projected
+    on an outcome drawn from the backend's generator (qibo.gates.M.apply:
+    probabilities of the sorted qubits, one shot, collapse); otherwise the
+    gate leaves the state as it is and the circuit's result samples it."""
 
     def __init__(self, *q, collapse=False):
         super().__init__()
         self.name = "measure"
         self.init_args = list(q)
         self.collapse = collapse
+        self.drawn = []  # outcomes of the collapsing applications, in order
         self._set_targets_and_controls(q)
 
     def apply(self, backend, state, nqubits):
         if not self.collapse:
             return state
-        probs = backend.to_numpy(backend.calculate_probabilities(state, self.target_qubits, nqubits))
-        probs = probs / probs.sum()
-        shot = backend._rng.choice(len(probs), p=probs)  # pylint: disable=protected-access
-        return backend.collapse_state(state, sorted(self.target_qubits), shot, nqubits)
+        qubits = sorted(self.target_qubits)
+        probs = backend.calculate_probabilities(state, qubits, nqubits)
+        shot = int(np.asarray(backend.to_numpy(backend.sample_shots(probs, 1))).reshape(-1)[0])
+        self.drawn.append(shot)
+        return backend.collapse_state(state, qubits, shot, nqubits)
+
+    def apply_density_matrix(self, backend, state, nqubits):
+        return self.apply(backend, state, nqubits)
 
 
 class Channel(Gate):

@@ -1,8 +1,13 @@
-"""Circuit container, gate fusion grouping and result object (qibo.models).
+"""Circuit container, gate fusion grouping and result objects (qibo.models,
+qibo.result).
 
 Any fusion grouping is valid: a fused block's dense matrix is the ordered
 product of its gates, and blocks keep the order of gates on every qubit.
 """
+
+from collections import Counter
+
+import numpy as np
 
 from qibojit_amd.qibo_compat import gates as gates_module
 
@@ -32,6 +37,16 @@ class Circuit:
     @property
     def ngates(self):
         return len(self.queue)
+
+    @property
+    def has_collapse(self):
+        return any(getattr(g, "collapse", False) for g in self.measurements)
+
+    @property
+    def repeated_execution(self):
+        """qibo re-runs a circuit once per shot when a measurement collapses
+        the state mid-circuit (qibo.models.Circuit.repeated_execution)."""
+        return self.has_collapse
 
     def fuse(self, max_qubits=2):
         """Return a new circuit whose unitary gates are grouped into FusedGates.
@@ -94,6 +109,81 @@ class QuantumState:
         return self.backend.calculate_probabilities(self._state, qubits, nqubits)
 
     def __array__(self, dtype=None, copy=None):
-        import numpy as np
-
         return np.asarray(self.backend.to_numpy(self._state), dtype=dtype)
+
+
+def frequencies_to_binary(frequencies, nqubits):
+    """Counter keyed by bitstrings, the first measured qubit leftmost."""
+    return Counter({format(int(k), "b").zfill(nqubits): v for k, v in frequencies.items()})
+
+
+class MeasurementOutcomes:
+    """Samples and frequencies of a circuit's measurements (qibo.result).
+
+    The measured qubits are those of every measurement gate, in gate order;
+    outcome k has the first measured qubit as its most significant bit.
+    Frequencies come from ``backend.sample_frequencies(probabilities,
+    nshots)`` unless samples were drawn first; samples from
+    ``backend.sample_shots``."""
+
+    def __init__(self, measurements, backend, probabilities=None, samples=None, nshots=1000):
+        self.measurements = list(measurements)
+        self.backend = backend
+        self.nshots = int(nshots)
+        self.measurement_qubits = tuple(q for m in self.measurements for q in m.target_qubits)
+        if not self.measurement_qubits:
+            raise ValueError("Circuit does not contain measurements.")
+        if probabilities is None and samples is None:
+            raise ValueError("MeasurementOutcomes needs probabilities or samples.")
+        self._probs = probabilities
+        self._samples = None if samples is None else np.asarray(samples, dtype=np.int32)
+        self._frequencies = None
+
+    def samples(self, binary=True):
+        """(nshots, k) 0/1 array, or with ``binary=False`` the outcome integers."""
+        k = len(self.measurement_qubits)
+        if self._samples is None:
+            if self._frequencies is not None:
+                # samples that realise the frequencies already reported
+                values = np.concatenate([np.repeat(x, f) for x, f in sorted(self._frequencies.items())])
+                np.random.shuffle(values)
+            else:
+                values = self.backend.sample_shots(self._probs, self.nshots)
+            self._samples = np.asarray(self.backend.samples_to_binary(values, k), dtype=np.int32)
+        if binary:
+            return self._samples
+        return self.backend.samples_to_decimal(self._samples, k)
+
+    def frequencies(self, binary=True):
+        """Counter of outcomes: bitstrings, or integers with ``binary=False``."""
+        if self._frequencies is None:
+            if self._samples is None:
+                self._frequencies = self.backend.sample_frequencies(self._probs, self.nshots)
+            else:
+                self._frequencies = self.backend.calculate_frequencies(self.samples(binary=False))
+        if binary:
+            return frequencies_to_binary(self._frequencies, len(self.measurement_qubits))
+        return self._frequencies
+
+    def has_samples(self):
+        return self._samples is not None
+
+
+class CircuitResult(QuantumState, MeasurementOutcomes):
+    """Final state plus measurement outcomes (qibo.result.CircuitResult):
+    the marginal probabilities of the measured qubits are computed once,
+    shots are drawn from them on demand."""
+
+    def __init__(self, final_state, measurements, backend=None, samples=None, nshots=1000):
+        QuantumState.__init__(self, final_state, backend)
+        qubits = tuple(q for m in measurements for q in m.target_qubits)
+        if not qubits:
+            raise ValueError("Circuit does not contain measurements.")
+        probs = QuantumState.probabilities(self, qubits) if samples is None else None
+        MeasurementOutcomes.__init__(self, measurements, backend, probabilities=probs, samples=samples,
+                                     nshots=nshots)
+
+    def probabilities(self, qubits=None):
+        if qubits is None:
+            qubits = self.measurement_qubits
+        return QuantumState.probabilities(self, qubits)

@@ -135,3 +135,76 @@ COLLAPSE = [
     (15, [4, 7], [0, 0]),
     (16, [8, 12, 15], [1, 0, 1]),
 ]
+
+
+# ------------------------------------------------------------------------
+# Cases beyond the reference's lists: the kernels the 30-qubit gate sweep
+# runs, at their strides.
+
+
+def sweep_placement(n):
+    """The sweep's gates with its placement, as (label, gate name, targets,
+    controls, parameters): H, X, Z, RZ, CNOT, SWAP, fSim, CU3 and a 3-qubit
+    dense gate on target qubits 0, n // 2 and n - 1 (bit strides 2^(n-1),
+    2^(n-1-n//2), 1); two-qubit gates on (t, t + 1), or (n - 2, n - 1) at
+    t = n - 1; the control of CNOT / CU3 is the neighbouring qubit; the
+    3-qubit gate on (t, t + 1, t + 2), or the three last qubits."""
+    out = []
+    theta = 0.731
+    for t in (0, n // 2, n - 1):
+        p2 = [t, t + 1] if t < n - 1 else [t - 1, t]
+        ctrl = t + 1 if t < n - 1 else t - 1
+        h3 = [t, t + 1, t + 2] if t + 2 < n else [n - 3, n - 2, n - 1]
+        out += [
+            (f"H({t})", "H", [t], [], ()),
+            (f"X({t})", "X", [t], [], ()),
+            (f"Z({t})", "Z", [t], [], ()),
+            (f"RZ({t})", "RZ", [t], [], (theta,)),
+            (f"CNOT({ctrl},{t})", "CNOT", [t], [ctrl], ()),
+            (f"SWAP{tuple(p2)}", "SWAP", p2, [], ()),
+            (f"fSim{tuple(p2)}", "fSim", p2, [], (0.3, 0.4)),
+            (f"CU3({ctrl},{t})", "CU3", [t], [ctrl], (0.1, 0.2, 0.3)),
+            (f"Haar3{tuple(h3)}", "Unitary", h3, [], ()),
+        ]
+    return out
+
+
+# uncontrolled 3-target gates on the three lowest bits (qubits n-3..n-1) in
+# several target orders: the LDS-staged chunk kernel takes them from 11 qubits
+LOW3 = [
+    (11, [8, 9, 10]),
+    (11, [10, 8, 9]),
+    (16, [15, 13, 14]),
+    (16, [13, 14, 15]),
+    (22, [21, 20, 19]),
+    (22, [20, 19, 21]),
+]
+
+# one-target gates whose target bit (n - 1 - q) is >= 6, so the strided
+# kernel runs instead of the in-wave exchange; complex64 moves two
+# amplitudes per lane when bit 0 (qubit n - 1) is free, one otherwise
+HIGH_BIT_ONE = [
+    (12, 0, []),  # bit 11
+    (12, 5, []),  # bit 6
+    (12, 3, [11]),  # bit 8, control on bit 0
+    (12, 2, [11, 7]),  # bit 9, controls on bits 0 and 4
+    (12, 5, [0]),  # bit 6, control on bit 11
+    (12, 4, [8]),  # bit 7, control inside the wave's 6 bits
+    (16, 1, [15, 3]),  # bit 14
+    (22, 0, []),  # bit 21
+    (22, 10, [21]),  # bit 11, control on bit 0
+    (22, 15, [2, 9]),  # bit 6
+]
+
+# two-target gates with at least one target bit >= 6
+HIGH_BIT_TWO = [
+    (12, [0, 5], []),  # bits 11, 6
+    (12, [5, 0], []),
+    (12, [3, 9], []),  # bits 8, 2
+    (12, [11, 2], []),  # bits 0, 9
+    (12, [1, 4], [11]),  # bits 10, 7, control on bit 0
+    (12, [6, 2], [0, 10]),  # bits 5, 9, controls on bits 11, 1
+    (16, [14, 0], [7]),  # bits 1, 15
+    (22, [0, 21], []),  # bits 21, 0
+    (22, [10, 11], [21, 3]),  # bits 11, 10
+]

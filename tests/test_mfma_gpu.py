@@ -39,18 +39,26 @@ def test_mfma_entry_rejects_bad_arguments(backend):
 
     from qibojit_amd import _native
 
+    # real buffers sized for every call below (2^12 amplitudes, a 32x32
+    # matrix), so a call that slipped past validation would stay in bounds
+    backend.set_dtype("complex64")
+    state = backend.zero_state(12)
+    sp = ctypes.c_void_p(state.data_ptr())
+    mat = np.eye(32, dtype=np.complex64)
+    mp = mat.ctypes.data_as(ctypes.c_void_p)
+    s = backend._stream()
     t, keep, nt = _native.i32_array([0, 1, 2, 3, 4])
-    fake = ctypes.c_void_p(0x1000)
     with pytest.raises(ValueError, match="complex64"):
-        _native.call("qj_apply_block_mfma", fake, _native.QJ_COMPLEX128, 12, t, nt, fake, None)
+        _native.call("qj_apply_block_mfma", sp, _native.QJ_COMPLEX128, 11, t, nt, mp, s)
     with pytest.raises(ValueError, match="10..40"):
-        _native.call("qj_apply_block_mfma", fake, _native.QJ_COMPLEX64, 9, t, nt, fake, None)
+        _native.call("qj_apply_block_mfma", sp, _native.QJ_COMPLEX64, 9, t, nt, mp, s)
     t4, keep4, n4 = _native.i32_array([0, 1, 2, 3])
     with pytest.raises(ValueError, match="exactly 5"):
-        _native.call("qj_apply_block_mfma", fake, _native.QJ_COMPLEX64, 12, t4, n4, fake, None)
+        _native.call("qj_apply_block_mfma", sp, _native.QJ_COMPLEX64, 12, t4, n4, mp, s)
     tr, keepr, nr = _native.i32_array([0, 1, 2, 3, 3])
     with pytest.raises(ValueError, match="repeated"):
-        _native.call("qj_apply_block_mfma", fake, _native.QJ_COMPLEX64, 12, tr, nr, fake, None)
+        _native.call("qj_apply_block_mfma", sp, _native.QJ_COMPLEX64, 12, tr, nr, mp, s)
+    backend.set_dtype("complex128")
 
 
 @pytest.mark.parametrize("dtype_", ["complex64", "complex128"])

@@ -233,6 +233,24 @@ def test_sample_shots_match_numpy_choice(backend, nstates, realtype):
     assert backend._rng.randint(0, 2**31) == rs.randint(0, 2**31)
 
 
+@pytest.mark.parametrize("nstates", [1000, 2**17 + 3])
+@pytest.mark.parametrize("scale", [2.0, 0.5])
+@pytest.mark.parametrize("realtype", [np.float64, np.float32])
+def test_sample_shots_unnormalized_probabilities(backend, nstates, scale, realtype):
+    """Probabilities summing to `scale` sample as choice() on p / sum(p).
+    A power-of-two scale changes neither the CDF's rounding nor the quotient
+    cdf[i] / cdf[-1], so the samples equal choice() on p exactly; this runs
+    the search over many workgroups, each of which reads cdf[-1]."""
+    rng = np.random.default_rng(nstates + 17)
+    p = rng.random(nstates) ** 2
+    p = (p / p.sum()).astype(realtype)
+    nshots = 40000
+    backend.set_seed(4321)
+    got = backend.to_numpy(backend.sample_shots((p * realtype(scale)).astype(realtype), nshots))
+    want = np.random.RandomState(4321).choice(nstates, size=nshots, p=p.astype(np.float64))
+    np.testing.assert_array_equal(got, want)
+
+
 def test_sample_shots_stream_continues_like_numpy(backend):
     p = np.array([0.1, 0.2, 0.3, 0.4])
     backend.set_seed(7)

@@ -146,10 +146,10 @@ def test_gate_matrix_identities():
 
 def test_product_matrices_match_oracle():
     """qibo-convention matrices of the product against the oracle's own."""
-    from qibojit_amd.backends.matrices import CustomMatrices
+    from qibojit_amd.backends.matrices import KernelMatrices
     from qibojit_amd.qibo_compat.npmatrices import NumpyMatrices
 
-    nm, cm = NumpyMatrices("complex128"), CustomMatrices("complex128")
+    nm, km = NumpyMatrices("complex128"), KernelMatrices("complex128")
     for name in ("H", "X", "Y", "Z", "S", "SDG", "T", "TDG", "SX", "SXDG"):
         np.testing.assert_allclose(getattr(nm, name), getattr(gm, name), atol=1e-15)
     for name in ("RX", "RY", "RZ"):
@@ -162,13 +162,23 @@ def test_product_matrices_match_oracle():
     np.testing.assert_allclose(nm.fSim(0.3, 0.4), gm.fSim(0.3, 0.4), atol=1e-15)
     np.testing.assert_allclose(nm.TOFFOLI, gm.controlled(gm.X, 2), atol=1e-15)
     np.testing.assert_allclose(nm.DEUTSCH(0.3), gm.DEUTSCH(0.3), atol=1e-15)
-    # kernel-facing encodings (matrices.py:13-70)
-    np.testing.assert_allclose(cm.U1(0.2), np.exp(0.2j), atol=1e-15)
-    np.testing.assert_allclose(cm.CU3(0.1, 0.2, 0.3), gm.U3(0.1, 0.2, 0.3), atol=1e-15)
-    np.testing.assert_allclose(cm.CZ, gm.Z)
-    np.testing.assert_allclose(cm.DEUTSCH(0.3), 1j * gm.RX(0.6), atol=1e-15)
-    f = cm.fSim(0.3, 0.4)
-    np.testing.assert_allclose(f, port.custom_matrix(gates.fSim(0, 1, 0.3, 0.4)), atol=1e-15)
+    # kernel-facing encodings (matrices.py:13-70), against the port's
+    # independent restatement, for every gate the encoder special-cases
+    rng = np.random.default_rng(9)
+    u2 = gm.haar_unitary(2, rng)
+    cases_ = [gates.U1(0, 0.2), gates.CU1(0, 1, 0.2), gates.CU3(0, 1, 0.1, 0.2, 0.3), gates.CZ(0, 1),
+              gates.CNOT(0, 1), gates.TOFFOLI(0, 1, 2), gates.CY(0, 1), gates.CH(0, 1), gates.CSX(0, 1),
+              gates.CSXDG(0, 1), gates.CRX(0, 1, 0.4), gates.CRY(0, 1, 0.5), gates.CRZ(0, 1, 0.6),
+              gates.CU2(0, 1, 0.2, 0.7), gates.CCZ(0, 1, 2), gates.DEUTSCH(0, 1, 2, 0.3),
+              gates.fSim(0, 1, 0.3, 0.4), gates.GeneralizedfSim(0, 1, u2, 0.8), gates.H(0), gates.RY(0, 0.9),
+              gates.U3(0, 0.1, 0.2, 0.3), gates.Unitary(gm.haar_unitary(8, rng), 0, 1, 2)]
+    for g in cases_:
+        got = km.encode(g)
+        assert got.dtype == np.complex128, g.__class__.__name__
+        np.testing.assert_allclose(got, port.custom_matrix(g), atol=1e-15, err_msg=g.__class__.__name__)
+    np.testing.assert_allclose(km.encode(gates.U1(0, 0.2)), np.exp(0.2j), atol=1e-15)
+    np.testing.assert_allclose(km.encode(gates.DEUTSCH(0, 1, 2, 0.3)), 1j * gm.RX(0.6), atol=1e-15)
+    assert KernelMatrices("complex64").encode(gates.fSim(0, 1, 0.3, 0.4)).dtype == np.complex64
 
 
 # -------------------------------------------- port kernels vs closed form --
@@ -356,3 +366,39 @@ def test_density_matrix_collapse_and_probabilities_pinned_by_pure_states():
     cp = er.collapse(psi, qubits, results, normalize=False)
     cf = er.collapse(phi, qubits, results, normalize=False)
     np.testing.assert_allclose(un, 0.3 * np.outer(cp, cp.conj()) + 0.7 * np.outer(cf, cf.conj()), atol=1e-14)
+
+
+@pytest.mark.parametrize("name,targets,controls,params", [
+    ("H", [0], [], ()), ("X", [6], [], ()), ("Z", [3], [], ()), ("RZ", [2], [], (0.73,)),
+    ("CNOT", [4], [5], ()), ("SWAP", [5, 6], [], ()), ("fSim", [1, 2], [], (0.3, 0.4)),
+    ("CU3", [6], [5], (0.1, 0.2, 0.3)), ("Unitary", [4, 5, 6], [], ()), ("Unitary", [6, 0, 3], [2], ()),
+    ("U1", [1], [4, 6], (0.9,)), ("GeneralizedfSim", [3, 0], [], ("haar", 0.35)),
+])
+def test_product_state_closed_form(name, targets, controls, params):
+    """The amplitude formula the full-size GPU checks use equals the einsum
+    closed form on every index of a 7-qubit product state; applying the gate
+    on a neighbouring qubit instead makes it fail."""
+    from tests import product_states as ps
+
+    rng = np.random.default_rng(len(targets) * 10 + targets[0])
+    n = 7
+    if name == "Unitary":
+        g = gates.Unitary(gm.haar_unitary(2 ** len(targets), rng), *targets).controlled_by(*controls)
+    elif name in ("CNOT", "CU3"):
+        g = getattr(gates, name)(controls[0], targets[0], *params)
+    elif name == "GeneralizedfSim":
+        g = gates.GeneralizedfSim(*targets, gm.haar_unitary(2, rng), params[1])
+    else:
+        g = getattr(gates, name)(*targets, *params)
+        if controls:
+            g = g.controlled_by(*controls)
+    a = ps.factors(n, rng)
+    psi = ps.host_product_state(a)
+    want = er.apply_gate(psi, ps.math_matrix(g), g.target_qubits, g.control_qubits, n)
+    idx = np.arange(2**n, dtype=np.int64)
+    np.testing.assert_allclose(ps.expected(a, g, idx), want, atol=1e-14)
+    # planted mistake: the same matrix one qubit over
+    shifted = [(q + 1) % n for q in g.target_qubits]
+    if not set(shifted) & set(g.control_qubits):
+        wrong = er.apply_gate(psi, ps.math_matrix(g), shifted, g.control_qubits, n)
+        assert np.abs(ps.expected(a, g, idx) - wrong).max() > 1e-3
