@@ -343,6 +343,19 @@ int qj_measure_frequencies(int64_t* frequencies, const void* probs,
 int qj_sample_shots(const void* probs, int prob_dtype, int64_t nstates,
                     const double* uniforms, int64_t nshots, int64_t* samples,
                     void* stream);
+/* No counterpart in the reference: qj_sample_shots for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, semantics and results
+ * as qj_sample_shots, but the CDF lives in a fixed per-GPU device array of
+ * 2^20 + 64 doubles (8 MiB, held until the process ends) instead of the
+ * library's growable work buffer, and the call makes no hipMalloc, hipFree
+ * or other allocating call.  At most 2^20 states: larger `nstates` is
+ * rejected with QJ_EVALUE before anything is enqueued.  Every call on one
+ * GPU shares that array, so calls that overlap on one GPU (two streams, two
+ * threads) race; calls ordered on one stream do not. */
+int qj_sample_shots_noalloc(const void* probs, int prob_dtype, int64_t nstates,
+                            const double* uniforms, int64_t nshots, int64_t* samples,
+                            void* stream);
 /* Counts of samples: frequencies[v] += 1 for each samples[j] = v in
  * [0, nstates) (device int64 arrays); samples outside that range add to
  * *out_of_range (device int64) instead.  The device side of

@@ -50,6 +50,7 @@ SIGNATURES = {
     "qj_calculate_probabilities": [_vp, _i32, _i32, _pi32, _i32, _vp, _i32, _vp],
     "qj_calculate_probabilities_dm": [_vp, _i32, _i32, _pi32, _i32, _vp, _i32, _vp],
     "qj_sample_shots": [_vp, _i32, _i64, _vp, _i64, _vp, _vp],
+    "qj_sample_shots_noalloc": [_vp, _i32, _i64, _vp, _i64, _vp, _vp],
     "qj_count_samples": [_vp, _i64, _vp, _i64, _vp, _vp],
     "qj_measure_frequencies": [_vp, _vp, _i32, _i64, _i64, _pu32, _i32, _vp],
     "qj_norm_squared": [_vp, _i32, _u64, _pdbl, _vp],

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "qj_common.h"
@@ -795,21 +796,21 @@ __global__ void __launch_bounds__(kBlock) k_count_samples(const int64_t* __restr
   }
 }
 
+// Doubles of work memory a sample of `nstates` needs: the CDF, then the
+// block sums of the blocked scan.
+static constexpr int64_t cdf_work_doubles(int64_t nstates) {
+  const int64_t per_block = (int64_t)kBlock * kCdfRun;
+  return nstates + (nstates + per_block - 1) / per_block;
+}
+
+// The CDF pipeline and the search on caller-provided work memory `cdf`
+// (cdf_work_doubles(nstates) doubles).  Every element it reads was written by
+// an earlier kernel of the same call, so the memory needs no clearing.
 template <typename T>
-static int sample_shots_impl(const T* probs, int64_t nstates, const double* u, int64_t nshots, int64_t* samples,
-                             hipStream_t s) {
-  if (nstates < 1) return set_error(QJ_EVALUE, "sample_shots: nstates must be >= 1");
-  if (nshots < 0) return set_error(QJ_EVALUE, "sample_shots: nshots must be >= 0");
-  if (nshots == 0) return QJ_OK;
-  DeviceScratch* sc = nullptr;
-  int rc = scratch_for_current_device(&sc);
-  if (rc) return rc;
+static int sample_shots_on(const T* probs, int64_t nstates, const double* u, int64_t nshots, int64_t* samples,
+                           double* cdf, hipStream_t s) {
   const int64_t per_block = (int64_t)kBlock * kCdfRun;
   const int64_t nblocks = (nstates + per_block - 1) / per_block;
-  const size_t need = (size_t)(nstates + nblocks) * sizeof(double);
-  rc = scratch_reserve_big(sc, need, s);
-  if (rc) return rc;
-  double* cdf = static_cast<double*>(sc->big);
   double* bsum = cdf + nstates;
   if (nstates <= kSeqCdfStates) {
     k_cdf_seq<T><<<dim3(1), dim3(64), 0, s>>>(probs, nstates, cdf);
@@ -820,6 +821,69 @@ static int sample_shots_impl(const T* probs, int64_t nstates, const double* u, i
   }
   k_cdf_search<<<dim3((unsigned)grid_for(nshots, kBlock)), dim3(kBlock), 0, s>>>(cdf, nstates, u, nshots, samples);
   return check_launch("sample_shots launch");
+}
+
+template <typename T>
+static int sample_shots_impl(const T* probs, int64_t nstates, const double* u, int64_t nshots, int64_t* samples,
+                             hipStream_t s) {
+  if (nstates < 1) return set_error(QJ_EVALUE, "sample_shots: nstates must be >= 1");
+  if (nshots < 0) return set_error(QJ_EVALUE, "sample_shots: nshots must be >= 0");
+  if (nshots == 0) return QJ_OK;
+  DeviceScratch* sc = nullptr;
+  int rc = scratch_for_current_device(&sc);
+  if (rc) return rc;
+  rc = scratch_reserve_big(sc, (size_t)cdf_work_doubles(nstates) * sizeof(double), s);
+  if (rc) return rc;
+  return sample_shots_on<T>(probs, nstates, u, nshots, samples, static_cast<double*>(sc->big), s);
+}
+
+// ------------------------------------- shot sampling without allocation --
+// Work memory of qj_sample_shots_noalloc: a file-scope device array, one copy
+// per GPU, alive until the process ends.  Host code never uses the name
+// `scratch` as an address (on the host it is only a stand-in for the device
+// symbol); it uses the per-device address looked up once with
+// hipGetSymbolAddress.
+constexpr int64_t kNoAllocMaxStates = int64_t(1) << 20;
+constexpr int64_t kNoAllocScratchDoubles = kNoAllocMaxStates + kNoAllocMaxStates / ((int64_t)kBlock * kCdfRun);
+static_assert(cdf_work_doubles(kNoAllocMaxStates) <= kNoAllocScratchDoubles, "noalloc work buffer too small");
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_noalloc_mutex;
+static double* g_noalloc_ptr[kNoAllocMaxDevices] = {nullptr};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on that device, with that device current).
+static int noalloc_scratch(double** out) {
+  int dev = 0;
+  int rc = hip_status(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return set_error(QJ_EHIP, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_noalloc_ptr[dev]) {
+    double* ptr = nullptr;
+    rc = hip_status(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    if (!ptr) return set_error(QJ_EHIP, "hipGetSymbolAddress(scratch) returned a null address");
+    g_noalloc_ptr[dev] = ptr;
+  }
+  *out = g_noalloc_ptr[dev];
+  return QJ_OK;
+}
+
+template <typename T>
+static int sample_shots_noalloc_impl(const T* probs, int64_t nstates, const double* u, int64_t nshots,
+                                     int64_t* samples, hipStream_t s) {
+  if (nstates < 1) return set_error(QJ_EVALUE, "sample_shots_noalloc: nstates must be >= 1");
+  if (nstates > kNoAllocMaxStates)
+    return set_error(QJ_EVALUE, "sample_shots_noalloc: %lld states exceed the fixed work buffer (at most %lld states)",
+                     (long long)nstates, (long long)kNoAllocMaxStates);
+  if (nshots < 0) return set_error(QJ_EVALUE, "sample_shots_noalloc: nshots must be >= 0");
+  if (nshots == 0) return QJ_OK;
+  double* cdf = nullptr;
+  int rc = noalloc_scratch(&cdf);
+  if (rc) return rc;
+  return sample_shots_on<T>(probs, nstates, u, nshots, samples, cdf, s);
 }
 
 }  // namespace qj
@@ -904,6 +968,18 @@ int qj_sample_shots(const void* probs, int prob_dtype, int64_t nstates, const do
   if (prob_dtype == QJ_FLOAT64)
     return sample_shots_impl<double>((const double*)probs, nstates, uniforms, nshots, samples, s);
   return set_error(QJ_EVALUE, "sample_shots: probs must be float32/float64");
+}
+
+int qj_sample_shots_noalloc(const void* probs, int prob_dtype, int64_t nstates, const double* uniforms,
+                            int64_t nshots, int64_t* samples, void* stream) {
+  if (!probs || (nshots > 0 && (!uniforms || !samples)))
+    return set_error(QJ_EVALUE, "sample_shots_noalloc: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (prob_dtype == QJ_FLOAT32)
+    return sample_shots_noalloc_impl<float>((const float*)probs, nstates, uniforms, nshots, samples, s);
+  if (prob_dtype == QJ_FLOAT64)
+    return sample_shots_noalloc_impl<double>((const double*)probs, nstates, uniforms, nshots, samples, s);
+  return set_error(QJ_EVALUE, "sample_shots_noalloc: probs must be float32/float64");
 }
 
 int qj_count_samples(const int64_t* samples, int64_t nshots, int64_t* frequencies, int64_t nstates,
