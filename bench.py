@@ -51,6 +51,8 @@ def parse():
     p.add_argument("--only-precision", action="store_true", help="complex64 vs complex128 runs and the MFMA kernel")
     p.add_argument("--only-clifford", action="store_true", help="stabilizer-tableau benchmark only")
     p.add_argument("--only-mfma", action="store_true", help="the 5-qubit block kernels alone (for counter runs)")
+    p.add_argument("--only-measure", action="store_true", help="probabilities, sampling and collapse only")
+    p.add_argument("--only-qft", action="store_true", help="32-qubit QFT through the tile executor only")
     return p.parse_args()
 
 
@@ -285,6 +287,7 @@ def run_qft(backend, n, dtype, fused=False):
 
 
 MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
+MFMA_FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix
 
 
 def brickwork(n, depth, seed):
@@ -312,42 +315,58 @@ def _timed(fn):
     return out, time.perf_counter() - t0
 
 
-def run_mfma_kernel(backend, n=32, launches=5, seed=11):
-    """One fused 5-qubit block on matrix cores, timed alone: 2^(n-5) columns
-    of a 32x32 complex GEMM per launch.  Algorithmic bytes per launch: one
-    read and one write of the complex64 state; flops: 8 * 32 * 32 per column."""
+def run_block_kernels(backend, launches=5, seed=11):
+    """Dense 4- and 5-target blocks on the 32-qubit complex64 and 30-qubit
+    complex128 states (34.36 / 17.18 GB), matrix-core 3M kernel against the
+    VALU register kernel, targets on the lowest, middle and highest qubit
+    bits, plus the low block controlled by qubit 0.  Algorithmic bytes per
+    launch: one read and one write of the state (of its 2^-c share when
+    controlled); flops: 8 NR^2 per column (the complex product), of which the
+    3M form executes 6 NR^2 on the matrix cores."""
     import torch
 
     from qibojit_amd._qibo import gates
 
-    backend.set_dtype("complex64")
     rng = np.random.default_rng(seed)
-    z = (rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))) / np.sqrt(2)
-    u, _ = np.linalg.qr(z)
-    state = torch.full((2**n,), 2 ** (-n / 2), dtype=torch.complex64, device="cuda")
     res = {}
-    for label, targets in (("mid", (10, 11, 12, 13, 14)), ("low", (27, 28, 29, 30, 31))):
-        gate = gates.Unitary(u, *targets)
-        for mfma in (True, False):
-            backend.mfma = mfma
-            backend.apply_gate(gate, state, n)  # warm-up
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(launches + 1)]
-            ev[0].record()
-            for k in range(launches):
-                backend.apply_gate(gate, state, n)
-                ev[k + 1].record()
-            torch.cuda.synchronize()
-            ms = float(np.mean([ev[k].elapsed_time(ev[k + 1]) for k in range(launches)]))
-            nbytes = 2 * 8 * 2**n
-            flops = 8 * 32 * 32 * 2 ** (n - 5)
-            res[f"{label}_{'mfma' if mfma else 'valu'}"] = {
-                "targets": list(targets), "ms": ms, "GBps": nbytes / ms / 1e6, "TFLOPs": flops / ms / 1e9,
-                "hbm_frac": nbytes / ms / 1e6 / HBM_PEAK_GBS,
-                "mfma_util": flops / ms / 1e9 / MFMA_FP32_PEAK_TFLOPS if mfma else None,
-            }
-    backend.mfma = True
-    del state
-    torch.cuda.empty_cache()
+    for dtype, n in (("complex64", 32), ("complex128", 30)):
+        backend.set_dtype(dtype)
+        tdt = torch.complex64 if dtype == "complex64" else torch.complex128
+        amp = 8 if dtype == "complex64" else 16
+        peak = MFMA_FP32_PEAK_TFLOPS if dtype == "complex64" else MFMA_FP64_PEAK_TFLOPS
+        state = torch.full((2**n,), 2 ** (-n / 2), dtype=tdt, device="cuda")
+        for k in (4, 5):
+            nr = 2**k
+            z = (rng.standard_normal((nr, nr)) + 1j * rng.standard_normal((nr, nr))) / np.sqrt(2)
+            u, _ = np.linalg.qr(z)
+            placements = (("low", tuple(range(n - k, n)), ()), ("mid", tuple(range(10, 10 + k)), ()),
+                          ("high", tuple(range(k)), ()), ("low_ctrl", tuple(range(n - k, n)), (0,)))
+            for label, targets, controls in placements:
+                gate = gates.Unitary(u, *targets)
+                if controls:
+                    gate = gate.controlled_by(*controls)
+                share = 2.0 ** -len(controls)
+                for mfma in (True, False):
+                    backend.mfma = mfma
+                    backend.apply_gate(gate, state, n)  # warm-up
+                    ev = [torch.cuda.Event(enable_timing=True) for _ in range(launches + 1)]
+                    ev[0].record()
+                    for j in range(launches):
+                        backend.apply_gate(gate, state, n)
+                        ev[j + 1].record()
+                    torch.cuda.synchronize()
+                    ms = float(np.mean([ev[j].elapsed_time(ev[j + 1]) for j in range(launches)]))
+                    nbytes = 2 * amp * 2**n * share
+                    cols = 2 ** (n - k) * share
+                    res[f"{dtype}_k{k}_{label}_{'mfma' if mfma else 'valu'}"] = {
+                        "nqubits": n, "targets": list(targets), "controls": list(controls), "ms": round(ms, 4),
+                        "GBps": round(nbytes / ms / 1e6, 1), "hbm_frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4),
+                        "mfma_util": round(6 * nr * nr * cols / ms / 1e9 / peak, 4) if mfma else None,
+                    }
+        backend.mfma = True
+        del state
+        torch.cuda.empty_cache()
+    backend.set_dtype("complex128")
     return res
 
 
@@ -430,7 +449,27 @@ def run_precision(backend, n=32, depth=40, seed=5):
     return out
 
 
-def run_measurement_ops(backend, n, dtype):
+def _event_ms(fn, reps):
+    """Median milliseconds of fn() over `reps` calls, each bracketed by HIP
+    events on the current stream (the stream the library launches on)."""
+    import torch
+
+    times = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        times.append(a.elapsed_time(b))
+    return float(np.median(times))
+
+
+def run_measurement_ops(backend, n, dtype, reps=5):
+    """Marginal probabilities, 10^6-shot Metropolis sampling and a normalised
+    collapse on qubits {0, 7, n-1}.  Collapse is timed on the collapsed state
+    again and again: every repetition zeroes the same 7/8 of the state and
+    rescales the same kept 1/8, so each moves the same bytes."""
     import torch
 
     backend.set_dtype(dtype)
@@ -441,12 +480,10 @@ def run_measurement_ops(backend, n, dtype):
     qubits = [0, 7, n - 1]
     amp = 8 if dtype == "complex64" else 16
     out = {}
-    torch.cuda.synchronize()
-    t = time.perf_counter()
     probs = backend.calculate_probabilities(state, qubits, n)
+    ms = _event_ms(lambda: backend.calculate_probabilities(state, qubits, n), reps)
+    out["probabilities"] = {"ms": ms, "GBps": amp * 2**n / ms / 1e6}
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t
-    out["probabilities"] = {"ms": dt * 1e3, "GBps": amp * 2**n / dt / 1e9}
     t = time.perf_counter()
     freqs = backend.measure_frequencies_op(
         torch.zeros(8, dtype=torch.int64, device="cuda"), probs, 10**6, 3, 1234, backend.nthreads
@@ -454,13 +491,14 @@ def run_measurement_ops(backend, n, dtype):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t
     out["metropolis_1e6_shots"] = {"ms": dt * 1e3, "chains": backend.nthreads, "total": int(freqs.sum())}
-    t = time.perf_counter()
     backend.collapse_state(state, qubits, 5, n)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t
-    b = amp * 2**n * (7 / 8 + 2 / 8)
-    out["collapse_normalized"] = {"ms": dt * 1e3, "GBps_at_1.125x": amp * 2**n * 1.125 / dt / 1e9,
-                                  "GBps_issued": b / dt / 1e9}
+    ms = _event_ms(lambda: backend.collapse_state(state, qubits, 5, n), reps)
+    # 1.125 x state: 7/8 written as zeros, the kept 1/8 read and rewritten.
+    # Line level (qubit n-1 is bit 0, so a kept 128-byte line holds kept and
+    # dropped amplitudes): 3/4 of the lines written, 1/4 read twice and
+    # written once = 1.5 x state.
+    out["collapse_normalized"] = {"ms": ms, "GBps_at_1.125x": amp * 2**n * 1.125 / ms / 1e6,
+                                  "GBps_line_level_1.5x": amp * 2**n * 1.5 / ms / 1e6}
     del state
     torch.cuda.empty_cache()
     return out
@@ -899,13 +937,21 @@ def main():
         print(json.dumps({"batch_1024": run_batch(backend)}), flush=True)
         return
     if args.only_mfma:
-        print(json.dumps({"mfma_kernel_32q": run_mfma_kernel(backend)}), flush=True)
+        print(json.dumps({"block_kernels": run_block_kernels(backend)}), flush=True)
+        return
+    if args.only_qft:
+        for dt in ("complex64", "complex128"):
+            print(json.dumps({f"qft_32q_{dt}_fused": run_qft(backend, 32, dt, fused=True)}), flush=True)
+        return
+    if args.only_measure:
+        for dt in ("complex128", "complex64"):
+            print(json.dumps({f"measurement_{dt}": run_measurement_ops(backend, args.nqubits, dt)}), flush=True)
         return
     if args.only_clifford:
         print(json.dumps({"clifford": run_clifford()}), flush=True)
         return
     if args.only_precision:
-        print(json.dumps({"mfma_kernel_32q": run_mfma_kernel(backend), "precision_32q": run_precision(backend)},
+        print(json.dumps({"block_kernels": run_block_kernels(backend), "precision_32q": run_precision(backend)},
                          default=str), flush=True)
         return
     n = args.nqubits
@@ -937,7 +983,7 @@ def main():
         summary["capacity_GBps"] = {"33q_c128": round(cap.get("GBps", float("nan")), 1),
                                     "34q_c64": round(cap64.get("GBps", float("nan")), 1)}
         print("# precision 32q complex64 vs complex128", file=sys.stderr, flush=True)
-        emit("mfma_kernel_32q_complex64", run_mfma_kernel(backend))
+        emit("block_kernels_k4_k5", run_block_kernels(backend))
         emit("precision_32q_complex64", run_precision(backend))
         print("# adjoint gradients 24q", file=sys.stderr, flush=True)
         adj = run_adjoint(backend)

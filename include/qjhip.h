@@ -127,15 +127,21 @@ int qj_apply_multi_qubit(void* state, int dtype, int nqubits,
                          const int32_t* qubits, int nqubits_sorted,
                          const void* gate, void* stream);
 
-/* A dense 5-target block (32x32, row-major complex64, no controls) applied
- * on FP32 matrix cores: the state viewed as 32 x 2^(n-5) is multiplied by
- * the block as a real 64x64 GEMM with v_mfma_f32_32x32x2f32.  Same matrix
- * convention as qj_apply_multi_qubit (target_bits[0] is the row index's most
- * significant bit); complex64 only, 10 <= nqubits.  It is the matrix-core
- * form of apply_five_qubit_gate_kernel (gates.py:336-411) for the fused
- * 5-qubit blocks of the fusion executor (cpu.py:535-537). */
+/* A dense 4- or 5-target block (2^k x 2^k, row-major, complex64 or
+ * complex128), optionally controlled, applied on matrix cores: the state
+ * viewed as 2^k x 2^(n-k-c) is multiplied by the block as three real GEMMs
+ * (the 3M form) with v_mfma_f32_16x16x4f32 (complex64) or
+ * v_mfma_f64_16x16x4f64 (complex128).  Same conventions as
+ * qj_apply_multi_qubit: target_bits[0] is the row index's most significant
+ * bit; `qubits` (NULL for no controls) lists every bit position, targets and
+ * controls, ascending, and controls must be 1.  At least 4 bits must be left
+ * free (16-column tiles).  It is the matrix-core form of
+ * apply_four/five_qubit_gate_kernel and their multicontrol variants
+ * (gates.py:265-411) for the fused blocks of the fusion executor
+ * (cpu.py:535-537). */
 int qj_apply_block_mfma(void* state, int dtype, int nqubits,
                         const int32_t* target_bits, int ntargets,
+                        const int32_t* qubits, int nqubits_sorted,
                         const void* gate, void* stream);
 
 /* ---- fused passes (tiles) ----

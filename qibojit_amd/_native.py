@@ -65,7 +65,7 @@ SIGNATURES = {
     "qj_reduce_rows": [_vp, _u64, _u64, _vp, _vp],
     "qj_fill_state": [_vp, _i32, _u64, _dbl, _dbl, _vp],
     "qj_compare_states": [_vp, _i32, _vp, _i32, _u64, _pdbl, _vp],
-    "qj_apply_block_mfma": [_vp, _i32, _i32, _pi32, _i32, _vp, _vp],
+    "qj_apply_block_mfma": [_vp, _i32, _i32, _pi32, _i32, _pi32, _i32, _vp, _vp],
     "qj_ansatz_lds": [_i32, _vp, _i32, _vp, _u64, _dbl, _vp, _i32, _i32, _vp],
     "qj_clifford_pack": [_vp, _i32, _vp, _vp],
     "qj_clifford_unpack": [_vp, _i32, _vp, _vp],

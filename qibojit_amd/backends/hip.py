@@ -420,26 +420,6 @@ class HipBackend(Backend):
         del keep
         return state
 
-    @staticmethod
-    def _spectator(nqubits, targets):
-        """Free qubit that pads a 4-qubit block to the 5-qubit matrix-core
-        kernel.  Blocks whose bits all lie below 9 take the lowest free bit,
-        so the padded block still qualifies for the LDS-staged kernel (a
-        spectator on the top bit makes every column strided: 22.5 ms instead
-        of 13.7 ms per launch at 32 qubits); other blocks take the free bit
-        next to their lowest (else highest) bit, keeping the block compact."""
-        bits = sorted(nqubits - 1 - q for q in targets)
-        free = [b for b in range(nqubits) if b not in bits]
-        if bits[-1] < 9:
-            bit = free[0]
-        elif bits[0] - 1 >= 9 and bits[0] - 1 in free:
-            bit = bits[0] - 1
-        elif bits[-1] + 1 < nqubits:
-            bit = bits[-1] + 1
-        else:
-            bit = min(free, key=lambda b: abs(b - bits[0]))
-        return nqubits - 1 - bit
-
     def _multi_qubit_base(self, state, nqubits, targets, gate, qubits):
         """cpu.py:581-604 -> qj_apply_multi_qubit."""
         state = self._as_state(state)
@@ -452,17 +432,18 @@ class HipBackend(Backend):
             qubits = sorted(nqubits - q - 1 for q in targets)
         mat = self._host_matrix(gate, state.dtype)
         qseq = self._seq(qubits)
-        if self.mfma and state.dtype == torch.complex64 and len(targets) in (4, 5) and len(qseq) == len(targets) \
-                and nqubits >= 10:
-            # uncontrolled 4/5-qubit block: FP32 matrix-core GEMM; a 4-qubit
-            # block gets a spectator qubit (U x I on a free bit)
-            if len(targets) == 4:
-                targets = targets + [self._spectator(nqubits, targets)]
-                mat = np.ascontiguousarray(np.kron(mat, np.eye(2, dtype=mat.dtype)))
-            tptr, tkeep, nt = i32_array([nqubits - t - 1 for t in targets])
+        tbits = [nqubits - t - 1 for t in targets]
+        if self.mfma and nqubits - len(qseq) >= 4 and (len(targets) == 5 or (len(targets) == 4 and max(tbits) < 8)):
+            # dense 5-target block, or 4-target block inside 256-amplitude
+            # runs (controlled or not): three real GEMMs on the matrix cores
+            # (FP32 MFMA for complex64, FP64 for complex128).  Other 4-target
+            # gates stay on the VALU register kernel, which measured as fast
+            # or faster there (6.5 vs 7.0 ms at 30 qubits complex128).
+            tptr, tkeep, nt = i32_array(tbits)
+            qptr, qkeep, nq = i32_array(qseq)
             call("qj_apply_block_mfma", _ptr(state), _NATIVE_CODES[state.dtype], nqubits, tptr, nt,
-                 mat.ctypes.data_as(ctypes.c_void_p), self._stream())
-            del tkeep
+                 qptr if nq > nt else None, nq, mat.ctypes.data_as(ctypes.c_void_p), self._stream())
+            del tkeep, qkeep
             return state
         tbits = [nqubits - t - 1 for t in targets]
         tptr, tkeep, nt = i32_array(tbits)

@@ -1,12 +1,15 @@
-// Fused 5-qubit blocks on complex64 states as FP32 matrix-core GEMMs.
+// Dense 4- and 5-target blocks as matrix-core GEMMs: FP32 MFMA on complex64
+// states, FP64 MFMA on complex128 states.
 //
-// A 5-target block multiplies every group of 32 amplitudes (the targets'
-// combinations for one assignment of the other bits) by the 32x32 complex
-// matrix U: the state is a 32 x 2^(n-5) complex matrix X (rows = target
-// index, columns = groups) and the block computes U X.  Written over the
-// reals, [Yr; Yi] = [[Ur, -Ui], [Ui, Ur]] [Xr; Xi]: a 64x64 by 64 x ncols
-// GEMM, which v_mfma_f32_32x32x2f32 (32x32 outputs, K = 2 per instruction,
-// FP32 products and sums) runs as 2 row tiles x 32 K-steps per 32 columns.
+// A K-target block multiplies every group of 2^K amplitudes (the targets'
+// combinations for one assignment of the other bits) by the 2^K x 2^K complex
+// matrix U: the state is a 2^K x ncols complex matrix X (rows = target index,
+// columns = groups) and the block computes U X.  The default kernel
+// (k_block_mfma3m below) uses three real products (the 3M form); two real
+// 64x64 forms for uncontrolled complex64 5-target blocks are kept as
+// measured alternatives: [Yr; Yi] = [[Ur, -Ui], [Ui, Ur]] [Xr; Xi], run with
+// v_mfma_f32_32x32x2f32 (32x32 outputs, K = 2 per instruction) as 2 row
+// tiles x 32 K-steps per 32 columns, or with v_mfma_f32_16x16x4f32.
 //
 // Lane layout of v_mfma_f32_32x32x2f32 for lane l, i = l % 32, h = l / 32:
 //   A: A[i][h]    B: B[h][i]    D: register r holds D[8(r/4) + 4h + r%4][i].
@@ -19,8 +22,9 @@
 // a column finish before its writes (the MFMAs consume every lane's loads),
 // so the update is in place.
 //
-// Matrix convention as qj_apply_multi_qubit: bit (4 - u) of the row index is
-// target target_bits[u] (the first target is the most significant bit).
+// Matrix convention as qj_apply_multi_qubit: bit (K - 1 - u) of the row
+// index is target target_bits[u] (the first target is the most significant
+// bit).
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -121,99 +125,160 @@ __global__ void __launch_bounds__(kMfmaBlock)
   }
 }
 
-// 3M form (default): three real 32x32 products instead of the 64x64 one,
+// 3M form (default), for K = 4 or 5 targets (NR = 2^K rows) on complex64
+// (v_mfma_f32_16x16x4f32) or complex128 (v_mfma_f64_16x16x4f64) states:
+// three real NR x NR products instead of the 2NR x 2NR one,
 //   P1 = Ur Xr,  P2 = Ui Xi,  P3 = (Ur + Ui)(Xr + Xi),
 //   Re Y = P1 - P2,  Im Y = P3 - P1 - P2,
-// with v_mfma_f32_16x16x4f32: 3 products x 2 row tiles x 8 K-steps = 48
-// MFMAs per 16 columns (64 for the real 64x64 form).  A operands per lane:
+// 3 products x NR/16 row tiles x NR/4 K-steps MFMAs per 16 columns (48 for
+// K = 5, 12 for K = 4).  Lane l, n = l % 16, q = l / 16 supplies A[n][q]
+// and B[q][n] in both instructions; the results sit in register r as row
+// 4q + r (f32) or q + 4r (f64) of column n.  A operands per lane:
 // ur[t][s] = Ur[16t + n][4s + q] (and ui, us = ur + ui); lane (n, q) loads
-// rows 4j + q of column n (its B operands for K-step j) and holds outputs
-// rows 16t + 4q + r of column n.
-// STAGE (every target below bit 9): a wave's 16 columns are one contiguous
-// run of 512 amplitudes, moved through LDS with coalesced loads and stores
-// (the direct loads of low targets are strided); slot a sits at a + a / 32
-// so the column-wise reads spread over the banks.  The loop is uniform per
-// block (ncols is a multiple of 64 there) so the block barriers are safe.
-constexpr int kStageAmps = 512;
-constexpr int kStageSlots = kStageAmps + kStageAmps / 32;
+// rows 4j + q of column n (its B operands for K-step j) and stores its
+// output rows 16t + out_row(q, r) of column n.
+// Columns are the assignments of the bits outside targets and controls
+// (controls inserted as ones), so a controlled block runs on its 2^-c share.
+// STAGE (targets all below bit K + 4, no control there): a wave's 16 columns
+// are one contiguous run of 16 NR amplitudes, moved through the wave's own
+// LDS region with coalesced loads and stores (direct loads of low targets
+// are strided); slot a sits at a + a / NR so the column-wise reads spread
+// over the banks.  Only the wave's lanes touch its region, so a wave barrier
+// orders the LDS traffic (WAVE_SYNC; a workgroup barrier otherwise, which
+// then needs ncols to be a multiple of 64 so that the loop is uniform per
+// workgroup).
+// The two instructions read A and B alike but place D differently: register
+// r of lane n + 16q holds row 4q + r (f32) or row q + 4r (f64) of column n
+// (measured with one-hot operands on gfx950).
+template <typename R>
+struct Mfma16;
+template <>
+struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int out_row(int q, int r) { return 4 * q + r; }
+};
+template <>
+struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int out_row(int q, int r) { return q + 4 * r; }
+};
 
-__device__ __forceinline__ unsigned stage_slot(unsigned a) { return a + (a >> 5); }
+template <int K>
+constexpr int stage_amps() {
+  return 16 << K;
+}
+template <int K>
+constexpr int stage_slots() {
+  return stage_amps<K>() + 16;
+}
 
-template <bool STAGE>
+// One pad slot per column of 2^K amplitudes: the 16 columns a K-step reads
+// start on 16 different banks.
+template <int K>
+__device__ __forceinline__ unsigned stage_slot(unsigned a) {
+  return a + (a >> K);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename R, int K, bool STAGE, bool WAVE_SYNC>
 __global__ void __launch_bounds__(kMfmaBlock)
-    k_block5_mfma3m(cplx<float>* __restrict__ st, u64 ncols, Ins ins, Offs32 off, const float* __restrict__ u3) {
-  __shared__ cplx<float> stage[STAGE ? kMfmaBlock / 64 : 1][STAGE ? kStageSlots : 1];
+    k_block_mfma3m(cplx<R>* __restrict__ st, u64 ncols, Ins ins, Offs32 off, u64 lowmask, const R* __restrict__ u3) {
+  typedef typename Mfma16<R>::v4 v4;
+  constexpr int NR = 1 << K, T = NR / 16, S = NR / 4;
+  constexpr int AMPS = stage_amps<K>();
+  __shared__ cplx<R> stage[STAGE ? kMfmaBlock / 64 : 1][STAGE ? stage_slots<K>() : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int q = lane >> 4, n = lane & 15;
-  float ur[2][8], ui[2][8], us[2][8];
+  R ur[T][S], ui[T][S], us[T][S];
 #pragma unroll
-  for (int t = 0; t < 2; ++t)
+  for (int t = 0; t < T; ++t)
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const int e = (16 * t + n) * 32 + 4 * s + q;
+    for (int s = 0; s < S; ++s) {
+      const int e = (16 * t + n) * NR + 4 * s + q;
       ur[t][s] = u3[e];
-      ui[t][s] = u3[1024 + e];
-      us[t][s] = u3[2048 + e];
+      ui[t][s] = u3[NR * NR + e];
+      us[t][s] = u3[2 * NR * NR + e];
     }
   // chunk-local index of this lane's column (targets inserted as zeros)
-  const unsigned colbase = STAGE ? (unsigned)insert_bits((u64)n, ins) : 0u;
+  const unsigned colbase = STAGE ? (unsigned)(insert_bits((u64)n, ins) & lowmask) : 0u;
   const u64 step = (u64)gridDim.x * (kMfmaBlock / 64) * 16;
   for (u64 c0 = ((u64)blockIdx.x * (kMfmaBlock / 64) + wave) * 16; c0 < ncols; c0 += step) {
     const u64 base = insert_bits(c0 + (STAGE ? 0 : n), ins);
-    float xr[8], xi[8], xs[8];
+    R xr[S], xi[S], xs[S];
     if (STAGE) {
 #pragma unroll
-      for (int k = 0; k < kStageAmps / 64; ++k) stage[wave][stage_slot(64 * k + lane)] = st[base + 64 * k + lane];
-      __syncthreads();
+      for (int k = 0; k < AMPS / 64; ++k) stage[wave][stage_slot<K>(64 * k + lane)] = st[base + 64 * k + lane];
+      if (WAVE_SYNC)
+        wave_lds_sync();
+      else
+        __syncthreads();
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const cplx<float> v = stage[wave][stage_slot(colbase + (unsigned)off.o[4 * j + q])];
+      for (int j = 0; j < S; ++j) {
+        const cplx<R> v = stage[wave][stage_slot<K>(colbase + (unsigned)off.o[4 * j + q])];
         xr[j] = v.re;
         xi[j] = v.im;
         xs[j] = v.re + v.im;
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const cplx<float> v = st[base + off.o[4 * j + q]];
+      for (int j = 0; j < S; ++j) {
+        const cplx<R> v = st[base + off.o[4 * j + q]];
         xr[j] = v.re;
         xi[j] = v.im;
         xs[j] = v.re + v.im;
       }
     }
-    v4f p1[2], p2[2], p3[2];
+    v4 p1[T], p2[T], p3[T];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) p1[t] = p2[t] = p3[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < T; ++t) p1[t] = p2[t] = p3[t] = v4{R(0), R(0), R(0), R(0)};
 #pragma unroll
-    for (int s = 0; s < 8; ++s)
+    for (int s = 0; s < S; ++s)
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        p1[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[t][s], xr[s], p1[t], 0, 0, 0);
-        p2[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[t][s], xi[s], p2[t], 0, 0, 0);
-        p3[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(us[t][s], xs[s], p3[t], 0, 0, 0);
+      for (int t = 0; t < T; ++t) {
+        p1[t] = Mfma16<R>::mma(ur[t][s], xr[s], p1[t]);
+        p2[t] = Mfma16<R>::mma(ui[t][s], xi[s], p2[t]);
+        p3[t] = Mfma16<R>::mma(us[t][s], xs[s], p3[t]);
       }
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < T; ++t)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const cplx<float> y = mk<float>(p1[t][r] - p2[t][r], p3[t][r] - p1[t][r] - p2[t][r]);
-        const unsigned row = (unsigned)off.o[16 * t + 4 * q + r];
+        const cplx<R> y = mk<R>(p1[t][r] - p2[t][r], p3[t][r] - p1[t][r] - p2[t][r]);
+        const unsigned row = (unsigned)off.o[16 * t + Mfma16<R>::out_row(q, r)];
         if (STAGE)
-          stage[wave][stage_slot(colbase + row)] = y;
+          stage[wave][stage_slot<K>(colbase + row)] = y;
         else
           st[base + row] = y;
       }
     if (STAGE) {
-      __syncthreads();
+      if (WAVE_SYNC)
+        wave_lds_sync();
+      else
+        __syncthreads();
 #pragma unroll
-      for (int k = 0; k < kStageAmps / 64; ++k) st[base + 64 * k + lane] = stage[wave][stage_slot(64 * k + lane)];
-      __syncthreads();
+      for (int k = 0; k < AMPS / 64; ++k) st[base + 64 * k + lane] = stage[wave][stage_slot<K>(64 * k + lane)];
+      if (WAVE_SYNC)
+        wave_lds_sync();
+      else
+        __syncthreads();
     }
   }
 }
 
 // QJ_MFMA_SHAPE: 3 (3M form, default), 16 (real 64x64 form, 16x16x4) or 32
-// (real 64x64 form, 32x32x2).
+// (real 64x64 form, 32x32x2); the real forms cover uncontrolled complex64
+// 5-target blocks only.
 static int mfma_shape() {
   static const int shape = [] {
     const char* e = std::getenv("QJ_MFMA_SHAPE");
@@ -223,75 +288,139 @@ static int mfma_shape() {
   return shape;
 }
 
-static int block5_impl(void* state, int nqubits, const int32_t* target_bits, const void* gate, hipStream_t s) {
-  if (nqubits < 10 || nqubits > 40)
-    return set_error(QJ_EVALUE, "apply_block_mfma: needs 10..40 qubits (32-column tiles), got %d", nqubits);
-  Ins ins;
-  std::memset(&ins, 0, sizeof(ins));
-  int pos[5];
-  for (int u = 0; u < 5; ++u) {
+// QJ_MFMA_SYNC: "wave" (default) orders a wave's staged LDS traffic with a
+// wave barrier, "block" with a workgroup barrier.
+static bool mfma_wave_sync() {
+  static const bool wave = [] {
+    const char* e = std::getenv("QJ_MFMA_SYNC");
+    return !(e && std::strcmp(e, "block") == 0);
+  }();
+  return wave;
+}
+
+template <typename R, int K, bool STAGE>
+static void launch_3m(bool wave_sync, unsigned grid, cplx<R>* st, u64 ncols, const Ins& ins, const Offs32& off,
+                      u64 lowmask, const R* u3, hipStream_t s) {
+  if (wave_sync)
+    k_block_mfma3m<R, K, STAGE, true><<<dim3(grid), dim3(kMfmaBlock), 0, s>>>(st, ncols, ins, off, lowmask, u3);
+  else
+    k_block_mfma3m<R, K, STAGE, false><<<dim3(grid), dim3(kMfmaBlock), 0, s>>>(st, ncols, ins, off, lowmask, u3);
+}
+
+template <typename R>
+static int block_impl(void* state, int nqubits, const int32_t* target_bits, int K, const int32_t* qubits, int nq,
+                      const void* gate, hipStream_t s) {
+  if (K != 4 && K != 5) return set_error(QJ_EVALUE, "apply_block_mfma: 4 or 5 targets, got %d", K);
+  if (nqubits < 1 || nqubits > 62) return set_error(QJ_EVALUE, "apply_block_mfma: bad nqubits %d", nqubits);
+  for (int u = 0; u < K; ++u) {
     if (target_bits[u] < 0 || target_bits[u] >= nqubits)
       return set_error(QJ_EVALUE, "apply_block_mfma: target bit %d out of range", target_bits[u]);
     for (int v = 0; v < u; ++v)
       if (target_bits[v] == target_bits[u]) return set_error(QJ_EVALUE, "apply_block_mfma: repeated target");
-    pos[u] = target_bits[u];
   }
-  std::sort(pos, pos + 5);
-  ins.npos = 5;
-  for (int u = 0; u < 5; ++u) ins.pos[u] = pos[u];
+  // every bit position (targets and controls), ascending; controls are the
+  // positions of `qubits` that are not targets
+  std::vector<int> pos(target_bits, target_bits + K);
+  u64 ones = 0;
+  if (qubits) {
+    if (nq < K || nq > kMaxPos) return set_error(QJ_EVALUE, "apply_block_mfma: bad number of qubits %d", nq);
+    int ntarget_seen = 0;
+    for (int j = 0; j < nq; ++j) {
+      const int b = qubits[j];
+      if (b < 0 || b >= nqubits) return set_error(QJ_EVALUE, "apply_block_mfma: qubit %d out of range", b);
+      if (j > 0 && qubits[j - 1] >= b) return set_error(QJ_EVALUE, "apply_block_mfma: qubits must be ascending");
+      const bool is_target = std::find(target_bits, target_bits + K, b) != target_bits + K;
+      if (is_target) {
+        ++ntarget_seen;
+      } else {
+        pos.push_back(b);
+        ones |= u64(1) << b;
+      }
+    }
+    if (ntarget_seen != K) return set_error(QJ_EVALUE, "apply_block_mfma: qubits must contain every target");
+  }
+  std::sort(pos.begin(), pos.end());
+  const int npos = (int)pos.size();
+  if (nqubits - npos < 4)
+    return set_error(QJ_EVALUE, "apply_block_mfma: needs at least 4 free qubits (16-column tiles), got %d",
+                     nqubits - npos);
+  Ins ins;
+  std::memset(&ins, 0, sizeof(ins));
+  ins.npos = npos;
+  for (int j = 0; j < npos; ++j) ins.pos[j] = pos[j];
+  ins.ones = ones;
   Offs32 off;
-  for (int r = 0; r < 32; ++r) {
+  std::memset(&off, 0, sizeof(off));
+  const int NR = 1 << K;
+  for (int r = 0; r < NR; ++r) {
     u64 o = 0;
-    for (int u = 0; u < 5; ++u)
-      if ((r >> (4 - u)) & 1) o |= u64(1) << target_bits[u];
+    for (int u = 0; u < K; ++u)
+      if ((r >> (K - 1 - u)) & 1) o |= u64(1) << target_bits[u];
     off.o[r] = o;
   }
-  const cplx<float>* u = static_cast<const cplx<float>*>(gate);
+  const cplx<R>* u = static_cast<const cplx<R>*>(gate);
+  const u64 ncols = u64(1) << (nqubits - npos);
+  cplx<R>* st = static_cast<cplx<R>*>(state);
   const int shape = mfma_shape();
-  std::vector<float> w;
-  if (shape == 3) {
-    // Ur, Ui, Ur + Ui as three 32x32 row-major blocks
-    w.resize(3 * 1024);
-    for (int e = 0; e < 1024; ++e) {
-      w[e] = u[e].re;
-      w[1024 + e] = u[e].im;
-      w[2048 + e] = u[e].re + u[e].im;
-    }
-  } else {
+  if (shape != 3 && sizeof(R) == 4 && K == 5 && npos == 5 && ncols >= 32) {
     // real 64x64 form [[Ur, -Ui], [Ui, Ur]], row-major
-    w.resize(64 * 64);
+    std::vector<float> w(64 * 64);
     for (int r = 0; r < 32; ++r)
       for (int c = 0; c < 32; ++c) {
-        const cplx<float> z = u[r * 32 + c];
-        w[r * 64 + c] = z.re;
-        w[r * 64 + 32 + c] = -z.im;
-        w[(32 + r) * 64 + c] = z.im;
-        w[(32 + r) * 64 + 32 + c] = z.re;
+        const cplx<R> z = u[r * 32 + c];
+        w[r * 64 + c] = (float)z.re;
+        w[r * 64 + 32 + c] = (float)-z.im;
+        w[(32 + r) * 64 + c] = (float)z.im;
+        w[(32 + r) * 64 + 32 + c] = (float)z.re;
       }
+    void* dw = nullptr;
+    int ticket = -1;
+    int rc = ring_upload(w.data(), w.size() * sizeof(float), s, &dw, &ticket);
+    if (rc) return rc;
+    const int cols_per_wave = shape == 32 ? 32 : 16;
+    u64 grid = (ncols / cols_per_wave + (kMfmaBlock / 64) - 1) / (kMfmaBlock / 64);
+    if (grid > kMfmaMaxGrid) grid = kMfmaMaxGrid;
+    cplx<float>* sf = reinterpret_cast<cplx<float>*>(state);
+    if (shape == 32)
+      k_block5_mfma<<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(sf, ncols, ins, off, static_cast<const float*>(dw));
+    else
+      k_block5_mfma16<<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(sf, ncols, ins, off,
+                                                                       static_cast<const float*>(dw));
+    rc = check_launch("apply_block_mfma launch");
+    ring_release(ticket, s);
+    return rc;
+  }
+  // Ur, Ui, Ur + Ui as three NR x NR row-major blocks
+  std::vector<R> w(3 * NR * NR);
+  for (int e = 0; e < NR * NR; ++e) {
+    w[e] = u[e].re;
+    w[NR * NR + e] = u[e].im;
+    w[2 * NR * NR + e] = u[e].re + u[e].im;
   }
   void* dw = nullptr;
   int ticket = -1;
-  int rc = ring_upload(w.data(), w.size() * sizeof(float), s, &dw, &ticket);
+  int rc = ring_upload(w.data(), w.size() * sizeof(R), s, &dw, &ticket);
   if (rc) return rc;
-  const u64 ncols = u64(1) << (nqubits - 5);
-  const int cols_per_wave = shape == 32 ? 32 : 16;
-  const u64 tasks = ncols / cols_per_wave;  // wave steps
-  u64 grid = (tasks + (kMfmaBlock / 64) - 1) / (kMfmaBlock / 64);
+  const R* du = static_cast<const R*>(dw);
+  u64 grid = (ncols / 16 + (kMfmaBlock / 64) - 1) / (kMfmaBlock / 64);
   if (grid > kMfmaMaxGrid) grid = kMfmaMaxGrid;
-  // targets below bit 9: a wave's columns form one contiguous run (staged)
-  const bool stage = shape == 3 && pos[4] < 9 && nqubits >= 11;
-  if (stage)
-    k_block5_mfma3m<true><<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(
-        static_cast<cplx<float>*>(state), ncols, ins, off, static_cast<const float*>(dw));
-  else if (shape == 3)
-    k_block5_mfma3m<false><<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(
-        static_cast<cplx<float>*>(state), ncols, ins, off, static_cast<const float*>(dw));
-  else if (cols_per_wave == 32)
-    k_block5_mfma<<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(static_cast<cplx<float>*>(state), ncols, ins,
-                                                                   off, static_cast<const float*>(dw));
-  else
-    k_block5_mfma16<<<dim3((unsigned)grid), dim3(kMfmaBlock), 0, s>>>(static_cast<cplx<float>*>(state), ncols,
-                                                                     ins, off, static_cast<const float*>(dw));
+  // The lowest K + 4 bits hold exactly the targets and 4 column bits: a
+  // wave's 16 columns are one contiguous run (staged through LDS).
+  const int low = K + 4;
+  bool stage = nqubits >= low && (ones & ((u64(1) << low) - 1)) == 0;
+  for (int u2 = 0; u2 < K; ++u2) stage = stage && target_bits[u2] < low;
+  const bool wave_sync = mfma_wave_sync();
+  // a workgroup barrier needs every wave of a workgroup in the loop alike
+  if (stage && !wave_sync && ncols % ((kMfmaBlock / 64) * 16) != 0) stage = false;
+  const u64 lowmask = (u64(1) << low) - 1;
+  const unsigned g = (unsigned)grid;
+  if (K == 5) {
+    if (stage) launch_3m<R, 5, true>(wave_sync, g, st, ncols, ins, off, lowmask, du, s);
+    else launch_3m<R, 5, false>(wave_sync, g, st, ncols, ins, off, lowmask, du, s);
+  } else {
+    if (stage) launch_3m<R, 4, true>(wave_sync, g, st, ncols, ins, off, lowmask, du, s);
+    else launch_3m<R, 4, false>(wave_sync, g, st, ncols, ins, off, lowmask, du, s);
+  }
   rc = check_launch("apply_block_mfma launch");
   ring_release(ticket, s);
   return rc;
@@ -304,11 +433,14 @@ using namespace qj;
 extern "C" {
 
 int qj_apply_block_mfma(void* state, int dtype, int nqubits, const int32_t* target_bits, int ntargets,
-                        const void* gate, void* stream) {
+                        const int32_t* qubits, int nqubits_list, const void* gate, void* stream) {
   if (!state || !target_bits || !gate) return set_error(QJ_EVALUE, "apply_block_mfma: null pointer");
-  if (dtype != QJ_COMPLEX64) return set_error(QJ_EVALUE, "apply_block_mfma: complex64 only (FP32 matrix cores)");
-  if (ntargets != 5) return set_error(QJ_EVALUE, "apply_block_mfma: exactly 5 targets, got %d", ntargets);
-  return block5_impl(state, nqubits, target_bits, gate, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == QJ_COMPLEX64)
+    return block_impl<float>(state, nqubits, target_bits, ntargets, qubits, nqubits_list, gate, s);
+  if (dtype == QJ_COMPLEX128)
+    return block_impl<double>(state, nqubits, target_bits, ntargets, qubits, nqubits_list, gate, s);
+  return set_error(QJ_EVALUE, "apply_block_mfma: dtype must be complex64/complex128");
 }
 
 }  // extern "C"

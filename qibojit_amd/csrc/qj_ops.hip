@@ -16,6 +16,9 @@
 namespace qj {
 
 constexpr int kRedBlocks = 1024;  // workgroups of a grid-stride reduction
+// Workgroups of the marginal-probability kernels: 32 per CU, so the loads of
+// a full chip are in flight (2^k partials each, k <= 5: 256 Ki doubles).
+constexpr u64 kProbsMaxBlocks = 8192;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -71,37 +74,103 @@ __global__ void __launch_bounds__(kBlock) k_fill(cplx<R>* __restrict__ st, u64 n
 }
 
 // --------------------------------------------------------------- collapse --
-// Every amplitude i splits into (free bits, measured bits); it is kept iff
-// (i & mask) == keep.  Pass 1 zeroes the rest (write-only) and accumulates
-// |kept|^2; pass 2 divides the kept amplitudes by the norm read from device
-// memory (no host round trip).
+// Amplitude i is kept iff (i & mask) == keep.  The state is walked in 16-byte
+// chunks (one complex128 or two complex64 amplitudes) grouped into 128-byte
+// lines; the measured bits split into line bits (hi_*) and bits inside a line
+// (lo_*).  A line whose line bits miss `keep` holds no kept amplitude and is
+// overwritten with zeros without being read.
+//
+// Pass 1 zeroes those lines and, when the norm is wanted, reads the kept
+// amplitudes of the other lines for FP64 partial sums of |kept|^2 (one per
+// workgroup).  With `zero_in_kept` (no normalisation pass follows) it also
+// zeroes the dropped amplitudes inside kept lines.  Pass 2 (normalisation)
+// rewrites each kept line once: kept amplitudes divided by the norm, the
+// others zero.  Each workgroup walks tiles of kBlock x kCollapseUnroll chunks,
+// every unrolled step one contiguous 4 KiB span.
+constexpr int kCollapseUnroll = 4;
+constexpr u64 kCollapseMaxBlocks = u64(1) << 16;  // partials per pass-1 launch
+
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
-    k_collapse_zero(cplx<R>* __restrict__ st, u64 n, u64 mask, u64 keep, int want_norm,
-                    double* __restrict__ partials) {
+    k_collapse_lines(cplx<R>* __restrict__ st, u64 nchunks, u64 hi_mask, u64 hi_keep, u64 lo_mask, u64 lo_keep,
+                     int want_norm, int zero_in_kept, double* __restrict__ partials) {
+  constexpr int V = 16 / (int)sizeof(cplx<R>);
+  constexpr int U = kCollapseUnroll;
   __shared__ double wsum[kBlock / 64];
+  const u64 tile = (u64)kBlock * U;
+  const u64 ntiles = (nchunks + tile - 1) / tile;
+  cvec<R, V> zero;
+#pragma unroll
+  for (int j = 0; j < V; ++j) zero.v[j] = mk<R>(R(0), R(0));
   double acc = 0.0;
-  const u64 stride = (u64)gridDim.x * kBlock;
-  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    if ((i & mask) == keep) {
-      if (want_norm) acc += abs2(st[i]);
-    } else {
-      st[i] = mk<R>(R(0), R(0));
+  for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    cvec<R, V> x[U];
+    bool in[U], kept_line[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const u64 c = t * tile + (u64)u * kBlock + threadIdx.x;
+      in[u] = c < nchunks;
+      kept_line[u] = in[u] && ((c * V) & hi_mask) == hi_keep;
+      if (kept_line[u] && want_norm) x[u] = ldv<R, V>(st + c * V);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!in[u]) continue;
+      const u64 i0 = (t * tile + (u64)u * kBlock + threadIdx.x) * V;
+      if (!kept_line[u]) {
+        stv<R, V>(st + i0, zero);
+        continue;
+      }
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        const bool kept = ((i0 + j) & lo_mask) == lo_keep;
+        if (kept) {
+          if (want_norm) acc += abs2(x[u].v[j]);
+        } else if (zero_in_kept) {
+          st[i0 + j] = mk<R>(R(0), R(0));
+        }
+      }
     }
   }
   acc = block_sum(acc, wsum);
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
+// Pass 2 over the kept lines only: kept chunk k is chunk (k % cpl) of kept
+// line k / cpl, whose line number comes from inserting the measured line bits
+// (`lines`, positions relative to the line size 2^lb, `ones` = outcome).
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
-    k_collapse_scale(cplx<R>* __restrict__ st, u64 nkept, Ins ins, const double* __restrict__ norm2) {
+    k_collapse_rescale(cplx<R>* __restrict__ st, u64 nkept_chunks, int lb, int cpl_log2, Ins lines, u64 lo_mask,
+                       u64 lo_keep, const double* __restrict__ norm2) {
+  constexpr int V = 16 / (int)sizeof(cplx<R>);
+  constexpr int U = kCollapseUnroll;
   const double norm = sqrt(*norm2);
-  const u64 stride = (u64)gridDim.x * kBlock;
-  for (u64 g = (u64)blockIdx.x * kBlock + threadIdx.x; g < nkept; g += stride) {
-    const u64 i = insert_bits(g, ins);
-    const cplx<R> x = st[i];
-    st[i] = mk<R>((R)((double)x.re / norm), (R)((double)x.im / norm));
+  const u64 tile = (u64)kBlock * U;
+  const u64 ntiles = (nkept_chunks + tile - 1) / tile;
+  const u64 cpl_mask = (u64(1) << cpl_log2) - 1;
+  for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    cvec<R, V> x[U];
+    u64 i0[U];
+    bool in[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const u64 k = t * tile + (u64)u * kBlock + threadIdx.x;
+      in[u] = k < nkept_chunks;
+      i0[u] = (insert_bits(k >> cpl_log2, lines) << lb) + (k & cpl_mask) * V;
+      if (in[u]) x[u] = ldv<R, V>(st + i0[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!in[u]) continue;
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        const bool kept = ((i0[u] + j) & lo_mask) == lo_keep;
+        const cplx<R> a = x[u].v[j];
+        x[u].v[j] = kept ? mk<R>((R)((double)a.re / norm), (R)((double)a.im / norm)) : mk<R>(R(0), R(0));
+      }
+      stv<R, V>(st + i0[u], x[u]);
+    }
   }
 }
 
@@ -134,6 +203,44 @@ __global__ void __launch_bounds__(kBlock)
   for (int h = 0; h < NB; ++h) {
     const double t = block_sum(acc[h], wsum);
     if (threadIdx.x == 0) partials[(u64)h * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// Bit 0 measured on a complex64 state: the two values of bit 0 are adjacent
+// amplitudes, so a thread reads them as one 16-byte pair (consecutive
+// threads then read consecutive pairs) and adds them to the pair's two
+// outcomes.  Pair p starts at offset poff.o[p] (bit 0 clear); its outcomes
+// are bins.even[p] and bins.odd[p].
+struct PairBins {
+  int even[16], odd[16];
+};
+
+template <int NP>
+__global__ void __launch_bounds__(kBlock)
+    k_probs_pairs(const cplx<float>* __restrict__ st, u64 nfree, Ins fins, Offs32 poff, PairBins bins,
+                  double* __restrict__ partials) {
+  __shared__ double wsum[kBlock / 64];
+  double a0[NP], a1[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) a0[p] = a1[p] = 0.0;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 g = (u64)blockIdx.x * kBlock + threadIdx.x; g < nfree; g += stride) {
+    const u64 base = insert_bits(g, fins);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const cvec<float, 2> v = ldv<float, 2>(st + base + poff.o[p]);
+      a0[p] += abs2(v.v[0]);
+      a1[p] += abs2(v.v[1]);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const double t0 = block_sum(a0[p], wsum);
+    const double t1 = block_sum(a1[p], wsum);
+    if (threadIdx.x == 0) {
+      partials[(u64)bins.even[p] * gridDim.x + blockIdx.x] = t0;
+      partials[(u64)bins.odd[p] * gridDim.x + blockIdx.x] = t1;
+    }
   }
 }
 
@@ -519,26 +626,43 @@ static int collapse_impl(void* state, int nqubits, const int32_t* qk, int nm, in
   DeviceScratch* sc = nullptr;
   int rc = scratch_for_current_device(&sc);
   if (rc) return rc;
+  constexpr int V = 16 / (int)sizeof(cplx<R>);
+  constexpr int kLineLog2 = sizeof(R) == 8 ? 3 : 4;  // amplitudes per 128-byte line
+  const int lb = std::min(kLineLog2, nqubits);
   const u64 n = u64(1) << nqubits;
+  const u64 lo_bits = (u64(1) << lb) - 1;
+  const u64 hi_mask = mask & ~lo_bits, hi_keep = keep & ~lo_bits;
+  const u64 lo_mask = mask & lo_bits, lo_keep = keep & lo_bits;
   cplx<R>* st = static_cast<cplx<R>*>(state);
-  const u64 grid = std::min<u64>(kRedBlocks, grid_for(n, kBlock));
-  k_collapse_zero<R><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, n, mask, keep, normalize || norm_out, sc->partials);
+  const bool want_norm = normalize || norm_out;
+  const u64 nchunks = n / V;
+  const u64 tile = (u64)kBlock * kCollapseUnroll;
+  const u64 grid = std::min<u64>(kCollapseMaxBlocks, (nchunks + tile - 1) / tile);
+  if (grid > sc->partials_cap) return set_error(QJ_EHIP, "collapse_state: partials buffer too small");
+  k_collapse_lines<R><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, nchunks, hi_mask, hi_keep, lo_mask, lo_keep,
+                                                                   want_norm ? 1 : 0, normalize ? 0 : 1, sc->partials);
   rc = check_launch("collapse launch");
   if (rc) return rc;
-  if (!normalize && !norm_out) return QJ_OK;
+  if (!want_norm) return QJ_OK;
   k_finalize<double><<<dim3(1), dim3(kBlock), 0, s>>>(sc->partials, grid, 0, 1, sc->scalars);
   rc = check_launch("collapse finalize launch");
   if (rc) return rc;
   if (normalize) {
-    Ins ins;
-    ins.npos = 0;
-    for (int b = 0; b < nqubits; ++b)
-      if ((mask >> b) & 1) ins.pos[ins.npos++] = b;
-    for (int j = ins.npos; j < kMaxPos; ++j) ins.pos[j] = 0;
-    ins.ones = keep;
-    const u64 nkept = n >> nm;
-    k_collapse_scale<R><<<dim3((unsigned)grid_for(nkept, kBlock)), dim3(kBlock), 0, s>>>(st, nkept, ins, sc->scalars);
-    rc = check_launch("collapse scale launch");
+    Ins lines;
+    lines.npos = 0;
+    lines.ones = 0;
+    for (int b = lb; b < nqubits; ++b)
+      if ((mask >> b) & 1) {
+        lines.pos[lines.npos++] = b - lb;
+        lines.ones |= ((keep >> b) & 1) << (b - lb);
+      }
+    for (int j = lines.npos; j < kMaxPos; ++j) lines.pos[j] = 0;
+    const int cpl_log2 = lb - (V == 2 ? 1 : 0);  // chunks per line
+    const u64 nkept_chunks = (n >> lines.npos) / V;
+    const u64 grid2 = grid_for((nkept_chunks + tile - 1) / tile, 1);
+    k_collapse_rescale<R><<<dim3((unsigned)grid2), dim3(kBlock), 0, s>>>(st, nkept_chunks, lb, cpl_log2, lines,
+                                                                        lo_mask, lo_keep, sc->scalars);
+    rc = check_launch("collapse rescale launch");
     if (rc) return rc;
   }
   if (norm_out) {
@@ -595,8 +719,36 @@ static int probs_impl(const void* state, int nqubits, const int32_t* qubits, int
           if ((h >> (nm - 1 - j)) & 1) o |= u64(1) << obits.pos[j];
       hoff.o[h] = o;
     }
-    const u64 grid = std::min<u64>(kRedBlocks, grid_for(nfree, kBlock));
+    const u64 grid = std::min<u64>(kProbsMaxBlocks, grid_for(nfree, kBlock));
     dim3 gd((unsigned)grid), bd(kBlock);
+    if (sizeof(R) == 4 && dstride == 0 && fins.pos[0] == 0) {
+      // bit 0 measured, complex64: 16-byte pairs
+      Offs32 poff;
+      PairBins bins;
+      std::memset(&poff, 0, sizeof(poff));
+      std::memset(&bins, 0, sizeof(bins));
+      int np = 0;
+      for (u64 h = 0; h < nbins; ++h)
+        if (!(hoff.o[h] & 1)) {
+          poff.o[np] = hoff.o[h];
+          bins.even[np] = (int)h;
+          for (u64 h1 = 0; h1 < nbins; ++h1)
+            if (hoff.o[h1] == (hoff.o[h] | 1)) bins.odd[np] = (int)h1;
+          ++np;
+        }
+      const cplx<float>* sf = reinterpret_cast<const cplx<float>*>(st);
+      switch (np) {
+        case 1: k_probs_pairs<1><<<gd, bd, 0, s>>>(sf, nfree, fins, poff, bins, sc->partials); break;
+        case 2: k_probs_pairs<2><<<gd, bd, 0, s>>>(sf, nfree, fins, poff, bins, sc->partials); break;
+        case 4: k_probs_pairs<4><<<gd, bd, 0, s>>>(sf, nfree, fins, poff, bins, sc->partials); break;
+        case 8: k_probs_pairs<8><<<gd, bd, 0, s>>>(sf, nfree, fins, poff, bins, sc->partials); break;
+        default: k_probs_pairs<16><<<gd, bd, 0, s>>>(sf, nfree, fins, poff, bins, sc->partials); break;
+      }
+      rc = check_launch("probabilities launch");
+      if (rc) return rc;
+      k_finalize<T><<<dim3((unsigned)nbins), dim3(kBlock), 0, s>>>(sc->partials, grid, grid, 1, out);
+      return check_launch("probabilities finalize launch");
+    }
     switch (nm) {
       case 1: k_probs_bins<R, 2><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, dstride, sc->partials); break;
       case 2: k_probs_bins<R, 4><<<gd, bd, 0, s>>>(st, nfree, fins, hoff, dstride, sc->partials); break;

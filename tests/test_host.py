@@ -100,18 +100,3 @@ def test_qubits_tensor_and_custom_matrix():
 def test_unknown_platform():
     with pytest.raises(ValueError):
         qibojit_amd.MetaBackend.load("tpu")
-
-
-@pytest.mark.parametrize("nqubits,targets,bit", [
-    (32, (28, 29, 30, 31), 4),     # bits 3..0: lowest free bit keeps the block staged
-    (32, (29, 31, 27, 30), 3),     # bits 2, 0, 4, 1: the gap at bit 3
-    (32, (10, 11, 12, 13), 17),    # bits 21..18: the bit below the block
-    (32, (0, 1, 2, 3), 27),        # bits 31..28: the bit below the block
-    (12, (1, 2, 3, 4), 11),        # bits 10..7: the bit above the block
-])
-def test_spectator_for_four_qubit_blocks(nqubits, targets, bit):
-    from qibojit_amd.backends.hip import HipBackend
-
-    q = HipBackend._spectator(nqubits, list(targets))
-    assert q not in targets
-    assert nqubits - 1 - q == bit
