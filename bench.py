@@ -53,6 +53,9 @@ def parse():
     p.add_argument("--only-mfma", action="store_true", help="the 5-qubit block kernels alone (for counter runs)")
     p.add_argument("--only-measure", action="store_true", help="probabilities, sampling and collapse only")
     p.add_argument("--only-qft", action="store_true", help="32-qubit QFT through the tile executor only")
+    p.add_argument("--blocks", default="low,mid,high,low_ctrl",
+                   help="placements of --only-mfma (comma-separated: low, mid, high, low_ctrl)")
+    p.add_argument("--block-paths", default="mfma,valu", help="kernels of --only-mfma: mfma, valu or both")
     return p.parse_args()
 
 
@@ -315,7 +318,8 @@ def _timed(fn):
     return out, time.perf_counter() - t0
 
 
-def run_block_kernels(backend, launches=5, seed=11):
+def run_block_kernels(backend, launches=5, seed=11, placements=("low", "mid", "high", "low_ctrl"),
+                      paths=(True, False)):
     """Dense 4- and 5-target blocks on the 32-qubit complex64 and 30-qubit
     complex128 states (34.36 / 17.18 GB), matrix-core 3M kernel against the
     VALU register kernel, targets on the lowest, middle and highest qubit
@@ -339,14 +343,14 @@ def run_block_kernels(backend, launches=5, seed=11):
             nr = 2**k
             z = (rng.standard_normal((nr, nr)) + 1j * rng.standard_normal((nr, nr))) / np.sqrt(2)
             u, _ = np.linalg.qr(z)
-            placements = (("low", tuple(range(n - k, n)), ()), ("mid", tuple(range(10, 10 + k)), ()),
-                          ("high", tuple(range(k)), ()), ("low_ctrl", tuple(range(n - k, n)), (0,)))
-            for label, targets, controls in placements:
+            placements_all = (("low", tuple(range(n - k, n)), ()), ("mid", tuple(range(10, 10 + k)), ()),
+                              ("high", tuple(range(k)), ()), ("low_ctrl", tuple(range(n - k, n)), (0,)))
+            for label, targets, controls in [p for p in placements_all if p[0] in placements]:
                 gate = gates.Unitary(u, *targets)
                 if controls:
                     gate = gate.controlled_by(*controls)
                 share = 2.0 ** -len(controls)
-                for mfma in (True, False):
+                for mfma in paths:
                     backend.mfma = mfma
                     backend.apply_gate(gate, state, n)  # warm-up
                     ev = [torch.cuda.Event(enable_timing=True) for _ in range(launches + 1)]
@@ -432,6 +436,11 @@ def run_precision(backend, n=32, depth=40, seed=5):
     cmp = backend.compare_states(st, ref)
     del st
     torch.cuda.empty_cache()
+    # complex128 fused blocks on FP64 matrix cores against the gate-by-gate run
+    st128, t_mfma128 = run(fused.queue)
+    cmp128 = backend.compare_states(st128, ref)
+    del st128
+    torch.cuda.empty_cache()
     backend.set_dtype("complex64")
     backend.mfma = False
     st_valu, t_valu = run(fused.queue)
@@ -442,6 +451,7 @@ def run_precision(backend, n=32, depth=40, seed=5):
         "nqubits": n, "depth": depth, "gates": len(circ.queue), "fused_blocks": len(fused.queue),
         "block_sizes": {str(k): v for k, v in sorted(sizes.items())},
         "seconds_c64_fused_mfma": t_mfma, "seconds_c64_fused_valu": t_valu, "seconds_c128_unfused": t_ref,
+        "seconds_c128_fused_mfma": t_mfma128, "c128_fused_max_abs_error": cmp128["max_abs_diff"],
         "infidelity": cmp["infidelity"], "max_abs_error": cmp["max_abs_diff"], "norm2_c64": cmp["norm2_a"],
     }
     del ref
@@ -937,7 +947,9 @@ def main():
         print(json.dumps({"batch_1024": run_batch(backend)}), flush=True)
         return
     if args.only_mfma:
-        print(json.dumps({"block_kernels": run_block_kernels(backend)}), flush=True)
+        paths = tuple(p == "mfma" for p in args.block_paths.split(","))
+        print(json.dumps({"block_kernels": run_block_kernels(backend, placements=tuple(args.blocks.split(",")),
+                                                             paths=paths)}), flush=True)
         return
     if args.only_qft:
         for dt in ("complex64", "complex128"):

@@ -791,13 +791,13 @@ class HipBackend(Backend):
         nbits = 2 * nqubits if density_matrix else nqubits
         low, max_high = self.tile_geometry(nbits, flat.dtype)
         npdt = np.complex64 if flat.dtype == torch.complex64 else np.complex128
-        pending = []
+        planner = tiles.Planner(nbits, low, max_high)
 
-        def flush():
-            if pending:
-                passes = tiles.plan(pending, nbits, low, max_high)
+        def launch(passes):
+            # each pass is launched as soon as the planner closes it, so the
+            # host converts the next pass's gates while this one runs
+            if passes:
                 tiles.run_passes(call, _ptr(flat), _NATIVE_CODES[flat.dtype], nbits, passes, npdt, self._stream())
-                pending.clear()
 
         for gate in gates:
             seq = tiles.expand_fanout(gate) if gate.name == "fanout" else [gate]
@@ -806,15 +806,15 @@ class HipBackend(Backend):
                 # and cast to the state's precision when a pass is encoded
                 op = tiles.gate_to_op(self._custom_matrix_c128, g, nbits) if self._is_unitary_gate(g) else None
                 if op is None or not tiles.fits_alone(op, low, max_high):
-                    flush()
+                    launch(planner.finish())
                     st = flat.reshape(shape)
                     st = g.apply_density_matrix(self, st, nqubits) if density_matrix else g.apply(self, st, nqubits)
                     flat = self._as_state(st).reshape(-1)
                     continue
-                pending.append(op)
+                launch(planner.push(op))
                 if density_matrix:
-                    pending.append(tiles.column_op(op, nqubits))
-        flush()
+                    launch(planner.push(tiles.column_op(op, nqubits)))
+        launch(planner.finish())
         return flat.reshape(shape)
 
     def assert_allclose(self, value, target, rtol=1e-7, atol=0.0):

@@ -179,57 +179,81 @@ def fits_alone(op, low, max_high):
     return Pass(low).fits(op, max_high)
 
 
-def _swap_run(ops, i):
-    """End of the run of SwapOps on distinct bits starting at ops[i]."""
-    used, j = set(), i
-    while j < len(ops) and isinstance(ops[j], SwapOp) and not used & set(ops[j].targets):
-        used |= set(ops[j].targets)
-        j += 1
-    return j
+class Planner:
+    """Greedy cut of an op stream into passes (circuit order preserved);
+    every op must satisfy ``fits_alone``.  ``push(op)`` returns the passes
+    that the op closed, ready to launch, so a caller can run a pass while
+    it is still converting the gates of the next one; ``finish()`` returns
+    the rest.  A run of two or more swaps on distinct bits that does not fit
+    the current pass becomes a SwapPass (swaps are held back until the run
+    ends)."""
+
+    def __init__(self, nqubits, low, max_high):
+        self.nqubits, self.low, self.max_high = nqubits, low, max_high
+        self.cur = Pass(low)
+        self.run = []  # pending SwapOps on distinct bits
+
+    def _close(self, p):
+        if isinstance(p, Pass):
+            # widen the tile to the full budget with the lowest unused bits
+            free = [b for b in range(self.low, self.nqubits) if b not in p.high]
+            p.high = sorted(p.high + free[: max(0, min(self.max_high, self.nqubits - self.low) - len(p.high))])
+        return p
+
+    def _add(self, op, out):
+        if not self.cur.fits(op, self.max_high):
+            out.append(self._close(self.cur))
+            self.cur = Pass(self.low)
+        self.cur.add(op)
+
+    def _flush_run(self, out):
+        run, self.run = self.run, []
+        if len(run) == 1:
+            self._add(run[0], out)
+        elif run:
+            trial = Pass(self.low)
+            trial.high = list(self.cur.high)
+            fits = True
+            for o in run:
+                fits = fits and trial.fits(o, self.max_high)
+                trial.add(o)
+            if fits:
+                for o in run:
+                    self.cur.add(o)
+            else:
+                if self.cur.ops:
+                    out.append(self._close(self.cur))
+                out.append(SwapPass([o.targets for o in run]))
+                self.cur = Pass(self.low)
+
+    def push(self, op):
+        out = []
+        if isinstance(op, SwapOp):
+            used = {b for o in self.run for b in o.targets}
+            if used & set(op.targets):
+                self._flush_run(out)
+            self.run.append(op)
+            return out
+        self._flush_run(out)
+        self._add(op, out)
+        return out
+
+    def finish(self):
+        out = []
+        self._flush_run(out)
+        if self.cur.ops:
+            out.append(self._close(self.cur))
+        self.cur = Pass(self.low)
+        return out
 
 
 def plan(ops, nqubits, low, max_high):
-    """Greedy cut of an op list into passes (circuit order preserved); every
-    op must satisfy ``fits_alone``.  A run of two or more swaps on distinct
-    bits that does not fit the current pass becomes a SwapPass."""
+    """The passes of a whole op list (see Planner)."""
+    pl = Planner(nqubits, low, max_high)
     passes = []
-    cur = Pass(low)
-    i = 0
-    while i < len(ops):
-        op = ops[i]
-        if isinstance(op, SwapOp):
-            j = _swap_run(ops, i)
-            if j - i >= 2:
-                trial = Pass(low)
-                trial.high = list(cur.high)
-                fits = True
-                for o in ops[i:j]:
-                    fits = fits and trial.fits(o, max_high)
-                    trial.add(o)
-                if fits:
-                    for o in ops[i:j]:
-                        cur.add(o)
-                else:
-                    if cur.ops:
-                        passes.append(cur)
-                    passes.append(SwapPass([o.targets for o in ops[i:j]]))
-                    cur = Pass(low)
-                i = j
-                continue
-        if not cur.fits(op, max_high):
-            passes.append(cur)
-            cur = Pass(low)
-        cur.add(op)
-        i += 1
-    if cur.ops:
-        passes.append(cur)
-    for p in passes:
-        if isinstance(p, SwapPass):
-            continue
-        # widen the tile to the full budget with the lowest unused bits
-        free = [b for b in range(low, nqubits) if b not in p.high]
-        p.high = sorted(p.high + free[: max(0, min(max_high, nqubits - low) - len(p.high))])
-    return passes
+    for op in ops:
+        passes += pl.push(op)
+    return passes + pl.finish()
 
 
 def encode(p, nqubits, npdtype):
