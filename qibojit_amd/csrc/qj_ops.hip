@@ -16,8 +16,9 @@
 namespace qj {
 
 constexpr int kRedBlocks = 1024;  // workgroups of a grid-stride reduction
-// Workgroups of the marginal-probability kernels: 32 per CU, so the loads of
-// a full chip are in flight (2^k partials each, k <= 5: 256 Ki doubles).
+// Workgroups of the marginal-probability, norm and overlap kernels: 32 per
+// CU, so the loads of a full chip are in flight (at most 32 partials each:
+// 256 Ki doubles).
 constexpr u64 kProbsMaxBlocks = 8192;
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -477,30 +478,45 @@ __global__ void __launch_bounds__(64)
 }
 
 // ------------------------------------------------------- FP64 reductions --
-template <typename R>
+// Norm and overlap read V amplitudes per lane and step (V = 2 for 16-byte
+// aligned complex64 states: 16-byte loads); a tail of n mod V amplitudes is
+// taken by thread 0 of workgroup 0.
+template <typename R, int V>
 __global__ void __launch_bounds__(kBlock)
     k_norm2(const cplx<R>* __restrict__ st, u64 n, double* __restrict__ partials) {
   __shared__ double wsum[kBlock / 64];
   double acc = 0.0;
-  const u64 stride = (u64)gridDim.x * kBlock;
-  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) acc += abs2(st[i]);
+  const u64 stride = (u64)gridDim.x * kBlock, nv = n / V;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < nv; i += stride) {
+    const cvec<R, V> x = ldv<R, V>(st + i * V);
+#pragma unroll
+    for (int k = 0; k < V; ++k) acc += abs2(x.v[k]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (u64 i = nv * V; i < n; ++i) acc += abs2(st[i]);
   acc = block_sum(acc, wsum);
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
-template <typename R>
+template <typename R, int V>
 __global__ void __launch_bounds__(kBlock)
     k_vdot(const cplx<R>* __restrict__ a, const cplx<R>* __restrict__ b, u64 n,
            double* __restrict__ partials) {
   __shared__ double wsum[kBlock / 64];
   double re = 0.0, im = 0.0;
-  const u64 stride = (u64)gridDim.x * kBlock;
-  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const cplx<R> x = a[i], y = b[i];
+  auto add = [&](cplx<R> x, cplx<R> y) {
     const double xr = x.re, xi = x.im, yr = y.re, yi = y.im;
     re += xr * yr + xi * yi;
     im += xr * yi - xi * yr;
+  };
+  const u64 stride = (u64)gridDim.x * kBlock, nv = n / V;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < nv; i += stride) {
+    const cvec<R, V> x = ldv<R, V>(a + i * V), y = ldv<R, V>(b + i * V);
+#pragma unroll
+    for (int k = 0; k < V; ++k) add(x.v[k], y.v[k]);
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (u64 i = nv * V; i < n; ++i) add(a[i], b[i]);
   re = block_sum(re, wsum);
   im = block_sum(im, wsum);
   if (threadIdx.x == 0) {
@@ -1153,11 +1169,14 @@ int qj_norm_squared(const void* state, int dtype, uint64_t n, double* out, void*
   DeviceScratch* sc = nullptr;
   int rc = scratch_for_current_device(&sc);
   if (rc) return rc;
-  const u64 grid = std::min<u64>(kRedBlocks, grid_for(n, kBlock));
-  if (dtype == QJ_COMPLEX64)
-    k_norm2<float><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<float>*)state, n, sc->partials);
+  const u64 grid = std::min<u64>(kProbsMaxBlocks, grid_for(n / 2 + 1, kBlock));
+  const bool a16 = ((uintptr_t)state & 15) == 0;
+  if (dtype == QJ_COMPLEX64 && a16)
+    k_norm2<float, 2><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<float>*)state, n, sc->partials);
+  else if (dtype == QJ_COMPLEX64)
+    k_norm2<float, 1><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<float>*)state, n, sc->partials);
   else if (dtype == QJ_COMPLEX128)
-    k_norm2<double><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<double>*)state, n, sc->partials);
+    k_norm2<double, 1><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<double>*)state, n, sc->partials);
   else
     return set_error(QJ_EVALUE, "norm_squared: dtype must be complex");
   rc = check_launch("norm launch");
@@ -1171,11 +1190,15 @@ int qj_vdot(const void* a, const void* b, int dtype, uint64_t n, double* out, vo
   DeviceScratch* sc = nullptr;
   int rc = scratch_for_current_device(&sc);
   if (rc) return rc;
-  const u64 grid = std::min<u64>(kRedBlocks, grid_for(n, kBlock));
-  if (dtype == QJ_COMPLEX64)
-    k_vdot<float><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<float>*)a, (const cplx<float>*)b, n, sc->partials);
+  const u64 grid = std::min<u64>(kProbsMaxBlocks, grid_for(n / 2 + 1, kBlock));
+  const bool a16 = (((uintptr_t)a | (uintptr_t)b) & 15) == 0;
+  const dim3 gd((unsigned)grid), bd(kBlock);
+  if (dtype == QJ_COMPLEX64 && a16)
+    k_vdot<float, 2><<<gd, bd, 0, s>>>((const cplx<float>*)a, (const cplx<float>*)b, n, sc->partials);
+  else if (dtype == QJ_COMPLEX64)
+    k_vdot<float, 1><<<gd, bd, 0, s>>>((const cplx<float>*)a, (const cplx<float>*)b, n, sc->partials);
   else if (dtype == QJ_COMPLEX128)
-    k_vdot<double><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>((const cplx<double>*)a, (const cplx<double>*)b, n, sc->partials);
+    k_vdot<double, 1><<<gd, bd, 0, s>>>((const cplx<double>*)a, (const cplx<double>*)b, n, sc->partials);
   else
     return set_error(QJ_EVALUE, "vdot: dtype must be complex");
   rc = check_launch("vdot launch");

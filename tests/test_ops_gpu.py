@@ -150,6 +150,22 @@ def test_reductions(backend, dtype, rng):
         assert abs(backend.expectation_tfim(da, n, g) - er.tfim_energy(a, g)) < tol * n
 
 
+@pytest.mark.parametrize("size,offset", [(1001, 0), (1000, 1), (4097, 1), (3, 0)])
+def test_reductions_odd_sizes_and_offsets(backend, dtype, rng, size, offset):
+    """Norm and overlap over vectors of odd length and from an amplitude
+    offset (complex64 then starts off the 16-byte grid the kernel's vector
+    loads use): the one-amplitude path and the tail are exact sums too."""
+    a = (rng.normal(size=size + offset) + 1j * rng.normal(size=size + offset)).astype(dtype)
+    b = (rng.normal(size=size + offset) + 1j * rng.normal(size=size + offset)).astype(dtype)
+    da, db = backend.cast(a, dtype=dtype)[offset:], backend.cast(b, dtype=dtype)[offset:]
+    a, b = a[offset:].astype(complex), b[offset:].astype(complex)
+    # the products are exact in FP64 for both dtypes; only the summation
+    # order differs from NumPy's
+    want_n, want_v = np.vdot(a, a).real, np.vdot(a, b)
+    assert abs(backend.norm_squared(da) - want_n) < 1e-12 * (1 + abs(want_n))
+    assert abs(backend.vdot(da, db) - want_v) < 1e-12 * (1 + abs(want_v))
+
+
 def test_reductions_deterministic(backend, rng):
     a = backend.cast(gm.random_state(24, rng))
     assert backend.norm_squared(a) == backend.norm_squared(a)
