@@ -259,22 +259,48 @@ __host__ __device__ __forceinline__ void reg_dense_any(cplx<R> (&x)[1 << NRB], i
 // factors over the set lane, wave and register slots (tables built by the
 // planner).  Registers whose bits fail the controls `rcm` are skipped; the
 // caller masks the threads whose thread bits fail theirs.
-template <typename R, int NRB>
-__host__ __device__ __forceinline__ void reg_phase(cplx<R> (&x)[1 << NRB], cplx<R> f,
-                                                   const cplx<R>* __restrict__ tab, int NT, int tid, int rcm) {
+// RCM >= 0: the register controls as a compile-time mask (no per-register
+// branches); RCM < 0: the runtime mask `rcm`.
+template <typename R, int NRB, int RCM>
+__host__ __device__ __forceinline__ void reg_phase_m(cplx<R> (&x)[1 << NRB], cplx<R> f,
+                                                     const cplx<R>* __restrict__ tab, int NT, int tid, int rcm) {
   constexpr int NR = 1 << NRB;
+  // register factors read ahead in groups (their LDS reads issued together,
+  // not one read and one wait per register)
+  constexpr int G = (sizeof(R) == 8 ? 4 : 8) < NR ? (sizeof(R) == 8 ? 4 : 8) : NR;
   const cplx<R>* tl = tab;
   const cplx<R>* tw = tl + 64;
   const cplx<R>* tr = tw + (1 << (NT - 6));
   const cplx<R> g = cmul(cmul(f, tl[tid & 63]), tw[tid >> 6]);
 #pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    if ((r & rcm) != rcm) continue;
-    const cplx<R> h = cmul(g, tr[r]);
-    // x *= h with the final FMAs as the last uses of x.re and x.im
-    const R t_re = x[r].im * h.im, t_im = x[r].re * h.im;
-    x[r].re = qfma(x[r].re, h.re, -t_re);
-    x[r].im = qfma(x[r].im, h.re, t_im);
+  for (int r0 = 0; r0 < NR; r0 += G) {
+    cplx<R> t[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k)
+      if (RCM < 0 || ((r0 + k) & RCM) == RCM) t[k] = tr[r0 + k];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      const int r = r0 + k;
+      if (RCM >= 0 ? (r & RCM) != RCM : (r & rcm) != rcm) continue;
+      const cplx<R> h = cmul(g, t[k]);
+      // x *= h with the final FMAs as the last uses of x.re and x.im
+      const R t_re = x[r].im * h.im, t_im = x[r].re * h.im;
+      x[r].re = qfma(x[r].re, h.re, -t_re);
+      x[r].im = qfma(x[r].im, h.re, t_im);
+    }
+  }
+}
+
+template <typename R, int NRB>
+__host__ __device__ __forceinline__ void reg_phase(cplx<R> (&x)[1 << NRB], cplx<R> f,
+                                                   const cplx<R>* __restrict__ tab, int NT, int tid, int rcm) {
+  switch (rcm) {
+    case 0: reg_phase_m<R, NRB, 0>(x, f, tab, NT, tid, rcm); break;
+    case 1: reg_phase_m<R, NRB, 1>(x, f, tab, NT, tid, rcm); break;
+    case 2: reg_phase_m<R, NRB, 2>(x, f, tab, NT, tid, rcm); break;
+    case 4: reg_phase_m<R, NRB, 4>(x, f, tab, NT, tid, rcm); break;
+    case 8: reg_phase_m<R, NRB, 8>(x, f, tab, NT, tid, rcm); break;
+    default: reg_phase_m<R, NRB, -1>(x, f, tab, NT, tid, rcm); break;
   }
 }
 

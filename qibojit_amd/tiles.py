@@ -24,11 +24,12 @@ Passes keep the gates in circuit order, so the result equals gate-by-gate
 execution up to rounding.
 """
 
+import cmath
 import ctypes
 
 import numpy as np
 
-from qibojit_amd.fusion import target_block
+from qibojit_amd.fusion import GATE_OPS, target_block
 
 TILE_DENSE, TILE_PHASE, TILE_SWAP = 0, 1, 2
 MAX_K = 5
@@ -107,16 +108,22 @@ def gate_to_op(custom_matrix_of, gate, nqubits):
     targets = list(gate.target_qubits)
     if len(targets) > MAX_K:
         return None
-    u = target_block(custom_matrix_of, gate)
     tbits = [nqubits - 1 - q for q in targets]
     cbits = [nqubits - 1 - q for q in gate.control_qubits]
+    if GATE_OPS.get(type(gate).__name__) == "apply_z_pow" and len(targets) == 1:
+        # U1 / CU1: diag(1, z) without building the matrix (the QFT's ladders
+        # are hundreds of these)
+        z = complex(np.asarray(custom_matrix_of(gate)))
+        if abs(abs(z) - 1) < 1e-14:
+            return PhaseOp(cbits, 0.0, {tbits[0]: cmath.phase(z)})
+    u = target_block(custom_matrix_of, gate)
     if type(gate).__name__ == "SWAP" and not cbits:
         return SwapOp(*tbits)
     if len(targets) == 1 and u[0, 1] == 0 and u[1, 0] == 0:
         d0, d1 = complex(u[0, 0]), complex(u[1, 1])
         if abs(abs(d0) - 1) < 1e-14 and abs(abs(d1) - 1) < 1e-14:
-            c0 = float(np.angle(d0))
-            return PhaseOp(cbits, c0, {tbits[0]: float(np.angle(d1)) - c0})
+            c0 = cmath.phase(d0)
+            return PhaseOp(cbits, c0, {tbits[0]: cmath.phase(d1) - c0})
     return DenseOp(tbits, cbits, u)
 
 
