@@ -226,9 +226,11 @@ def dominant_kernel(rows):
 
 
 # ---------------------------------------------------------------- extras --
-def run_qft(backend, n, dtype, fused=False):
+def run_qft(backend, n, dtype, fused=False, reps=1):
     """QFT (H + controlled-phase ladder + final swaps) from a seeded state,
-    gate by gate or through the fused tile executor."""
+    gate by gate or through the fused tile executor.  With reps > 1 the
+    circuit runs that many times on the evolving state (every run moves the
+    same bytes) and `seconds` is the median run, host planning included."""
     import torch
 
     from qibojit_amd._qibo import Circuit, gates
@@ -253,15 +255,18 @@ def run_qft(backend, n, dtype, fused=False):
         low, max_high = backend.tile_geometry(n, tdt)
         ops = [tiles.gate_to_op(backend._as_custom_matrix, g, n) for g in c.queue]  # pylint: disable=protected-access
         npasses = len(tiles.plan(ops, n, low, max_high))
-    e0.record()
-    if fused:
-        state = backend.apply_gates_fused(c.queue, state, n)
-    else:
-        for g in c.queue:
-            state = backend.apply_gate(g, state, n)
-    e1.record()
-    torch.cuda.synchronize()
-    sec = e0.elapsed_time(e1) / 1e3
+    runs = []
+    for _ in range(reps):
+        e0.record()
+        if fused:
+            state = backend.apply_gates_fused(c.queue, state, n)
+        else:
+            for g in c.queue:
+                state = backend.apply_gate(g, state, n)
+        e1.record()
+        torch.cuda.synchronize()
+        runs.append(e0.elapsed_time(e1) / 1e3)
+    sec = float(np.median(runs))
     amp = 8 if dtype == "complex64" else 16
     # algorithmic bytes: H full pass; CU1 touches control=1 & target=1 (1/4);
     # SWAP touches bit1 != bit2 (1/2); line rounding for low bits.
@@ -278,7 +283,7 @@ def run_qft(backend, n, dtype, fused=False):
     norm = backend.norm_squared(state)
     del state
     torch.cuda.empty_cache()
-    out = {"nqubits": n, "dtype": dtype, "gates": len(c.queue), "seconds": sec, "final_norm2": norm}
+    out = {"nqubits": n, "dtype": dtype, "gates": len(c.queue), "seconds": sec, "runs": runs, "final_norm2": norm}
     if fused:
         # one read + one write of the state per pass
         passes_bytes = npasses * 2 * amp * 2**n
@@ -953,7 +958,7 @@ def main():
         return
     if args.only_qft:
         for dt in ("complex64", "complex128"):
-            print(json.dumps({f"qft_32q_{dt}_fused": run_qft(backend, 32, dt, fused=True)}), flush=True)
+            print(json.dumps({f"qft_32q_{dt}_fused": run_qft(backend, 32, dt, fused=True, reps=3)}), flush=True)
         return
     if args.only_measure:
         for dt in ("complex128", "complex64"):
@@ -981,9 +986,11 @@ def main():
         summary["sweep_c64_GBps"] = round(c64["GBps"], 1)
     if not args.no_extras and not args.only_sweep:
         print("# measurement ops", file=sys.stderr, flush=True)
-        emit("measurement_complex128", run_measurement_ops(backend, n, "complex128"))
+        meas = run_measurement_ops(backend, n, "complex128")
+        emit("measurement_complex128", meas)
+        summary["collapse_c128_ms"] = round(meas["collapse_normalized"]["ms"], 3)
         print("# qft 32q complex64", file=sys.stderr, flush=True)
-        qft = run_qft(backend, 32, "complex64", fused=True)
+        qft = run_qft(backend, 32, "complex64", fused=True, reps=3)
         emit("qft_32q_complex64_fused", qft)
         summary["qft_32q_c64_s"] = round(qft["seconds"], 4)
         emit("qft_32q_complex64", run_qft(backend, 32, "complex64"))
@@ -995,7 +1002,10 @@ def main():
         summary["capacity_GBps"] = {"33q_c128": round(cap.get("GBps", float("nan")), 1),
                                     "34q_c64": round(cap64.get("GBps", float("nan")), 1)}
         print("# precision 32q complex64 vs complex128", file=sys.stderr, flush=True)
-        emit("block_kernels_k4_k5", run_block_kernels(backend))
+        blocks = run_block_kernels(backend)
+        emit("block_kernels_k4_k5", blocks)
+        summary["block5_bits0_4_hbm_frac"] = {"c128": blocks["complex128_k5_low_mfma"]["hbm_frac"],
+                                              "c64": blocks["complex64_k5_low_mfma"]["hbm_frac"]}
         emit("precision_32q_complex64", run_precision(backend))
         print("# adjoint gradients 24q", file=sys.stderr, flush=True)
         adj = run_adjoint(backend)

@@ -332,6 +332,7 @@ __global__ void __launch_bounds__(BSMAX, WPE)
   }
   __syncthreads();
   const RegSeg* segs = reinterpret_cast<const RegSeg*>(lp + h.segs_off);
+  const uint16_t* ub_seg = h.ub_off ? reinterpret_cast<const uint16_t*>(lp + h.ub_off) : nullptr;
   const RegOp* ops = reinterpret_cast<const RegOp*>(lp + h.ops_off);
   const u64* ftab = reinterpret_cast<const u64*>(lp + h.ftab_off);
   const char* ldata = lp + h.ldata_off;
@@ -367,13 +368,38 @@ __global__ void __launch_bounds__(BSMAX, WPE)
         RegLay<NRB> nxt = reg_layout<NRB>(sg, NT, tid);
 #pragma unroll
         for (int j = 0; j < NRB; ++j) nxt.rb[j] = uni(nxt.rb[j]);
-        const int sw_cur = lds_swz<R>(cur.ethr), sw_nxt = lds_swz<R>(nxt.ethr);
-        __syncthreads();
+        // LDS byte address of register r = the thread part's offset XOR the
+        // segment's precomputed register part (the swizzle is linear); plans
+        // without the offsets compute the register part per lane
+        constexpr int AB = (int)sizeof(cplx<R>);
+        const int swb_cur = lds_swz<R>(cur.ethr) * AB, swb_nxt = lds_swz<R>(nxt.ethr) * AB;
+        char* lb = reinterpret_cast<char*>(lds);
+        if (ub_seg) {
+          constexpr int NU = NR < 8 ? 1 : NR / 8;  // 16-byte groups of 8 offsets
+          uint4 u[NU];
+          auto off = [&](int r) {
+            const uint32_t w = (&u[r >> 3].x)[(r >> 1) & 3];
+            return (int)((r & 1) ? (w >> 16) : (w & 0xffffu));
+          };
 #pragma unroll
-        for (int r = 0; r < NR; ++r) lds[reg_lds<R, NRB>(sw_cur, cur, r)] = x[r];
-        __syncthreads();
+          for (int i = 0; i < NU; ++i) u[i] = reinterpret_cast<const uint4*>(ub_seg + (s - 1) * NR)[i];
+          __syncthreads();
 #pragma unroll
-        for (int r = 0; r < NR; ++r) x[r] = lds[reg_lds<R, NRB>(sw_nxt, nxt, r)];
+          for (int r = 0; r < NR; ++r) *reinterpret_cast<cplx<R>*>(lb + (swb_cur ^ off(r))) = x[r];
+#pragma unroll
+          for (int i = 0; i < NU; ++i) u[i] = reinterpret_cast<const uint4*>(ub_seg + s * NR)[i];
+          __syncthreads();
+#pragma unroll
+          for (int r = 0; r < NR; ++r) x[r] = *reinterpret_cast<const cplx<R>*>(lb + (swb_nxt ^ off(r)));
+        } else {
+          const int sw_cur = lds_swz<R>(cur.ethr), sw_nxt = lds_swz<R>(nxt.ethr);
+          __syncthreads();
+#pragma unroll
+          for (int r = 0; r < NR; ++r) lds[reg_lds<R, NRB>(sw_cur, cur, r)] = x[r];
+          __syncthreads();
+#pragma unroll
+          for (int r = 0; r < NR; ++r) x[r] = lds[reg_lds<R, NRB>(sw_nxt, nxt, r)];
+        }
         cur = nxt;
       }
       const int o1 = uni(sg.o1);
@@ -693,6 +719,12 @@ static int tile_pass_impl(void* state, int nqubits, int L, const int32_t* high, 
     in.nops = nops;
     in.data = cdata;
     in.lds_cap = kRegPlanLds;
+    // QJ_TILE_OFFSETS=0: re-layouts compute every register's swizzle per lane
+    static const bool offsets = [] {
+      const char* e = std::getenv("QJ_TILE_OFFSETS");
+      return !(e && std::strcmp(e, "0") == 0);
+    }();
+    in.reg_offsets = offsets;
     std::vector<char> plan;
     RegHead hh;
     size_t lds_reg = 0;

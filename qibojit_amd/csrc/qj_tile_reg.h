@@ -41,6 +41,13 @@ struct RegHead {
   int32_t TB, NT, NRB, nseg, nops, nphase, ga, gb;
   u64 lds_off, lds_bytes;      // LDS part: blob[lds_off, lds_off + lds_bytes)
   u64 segs_off, ops_off, ftab_off, ldata_off;  // offsets inside the LDS part
+  // Optional per-segment register offsets (2^NRB uint16 per segment, 16-byte
+  // aligned): the LDS byte offset of register r's part of the tile element
+  // in segment s's layout (the swizzle of the register bits alone, times the
+  // amplitude size).  A re-layout address is then the thread part's offset
+  // XOR this uniform value, instead of a per-lane swizzle.  Left out
+  // (ub_off = 0) when the plan would not fit the LDS budget with it.
+  u64 ub_off;
   uint8_t gbit_ld[kRegMaxBits];  // physical tile bit -> state bit (load)
   uint8_t gbit_st[kRegMaxBits];  // physical tile bit -> state bit (store)
 };
@@ -320,6 +327,7 @@ struct RegPlanIn {
   int nops;
   const char* data;
   u64 lds_cap;                 // largest LDS part a workgroup can hold
+  bool reg_offsets = true;     // add the per-segment register offsets when they fit
 };
 
 inline int popc32(uint32_t v) { return __builtin_popcount(v); }
@@ -449,6 +457,15 @@ inline bool reg_plan(const RegPlanIn& in, std::vector<char>& blob) {
       if ((regs[s] >> b) & 1) sg.slot[slot++] = (int8_t)b;
     if (slot != TB) return false;
   }
+  // per-segment register offsets (see RegHead::ub_off)
+  std::vector<uint16_t> ub((size_t)nseg << NRB);
+  for (int s = 0; s < nseg; ++s)
+    for (int r = 0; r < (1 << NRB); ++r) {
+      int e = 0;
+      for (int j = 0; j < NRB; ++j)
+        if ((r >> j) & 1) e |= 1 << segs[s].slot[NT + j];
+      ub[((size_t)s << NRB) + r] = (uint16_t)((in.esize == 8 ? lds_swz<float>(e) : lds_swz<double>(e)) * in.esize);
+    }
 
   // 3. ops, LDS data (matrices, W in slot order) and out-of-tile tables
   std::vector<RegOp> rops(pl.size());
@@ -588,6 +605,11 @@ inline bool reg_plan(const RegPlanIn& in, std::vector<char>& blob) {
   h.ldata_off = align16(h.ftab_off + (u64)ftab.size() * sizeof(u64));
   h.lds_bytes = align16(h.ldata_off + ldata.size());
   if (h.lds_bytes > in.lds_cap) return false;
+  const u64 ub_bytes = align16(ub.size() * sizeof(uint16_t));
+  if (in.reg_offsets && NRB >= 3 && h.lds_bytes + ub_bytes <= in.lds_cap) {
+    h.ub_off = h.lds_bytes;  // > 0: segments come first
+    h.lds_bytes += ub_bytes;
+  }
   h.lds_off = align16(sizeof(RegHead));
   const u64 gtab_off = h.lds_off + h.lds_bytes;
   for (u64& f : ftab) f += gtab_off;  // out-of-tile tables: offsets from the blob start
@@ -598,6 +620,7 @@ inline bool reg_plan(const RegPlanIn& in, std::vector<char>& blob) {
   if (!rops.empty()) std::memcpy(lp + h.ops_off, rops.data(), rops.size() * sizeof(RegOp));
   if (!ftab.empty()) std::memcpy(lp + h.ftab_off, ftab.data(), ftab.size() * sizeof(u64));
   if (!ldata.empty()) std::memcpy(lp + h.ldata_off, ldata.data(), ldata.size());
+  if (h.ub_off) std::memcpy(lp + h.ub_off, ub.data(), ub.size() * sizeof(uint16_t));
   if (!gtab.empty()) std::memcpy(blob.data() + gtab_off, gtab.data(), gtab.size());
   return true;
 }

@@ -208,6 +208,13 @@ static bool run_kernel(const Case<R>& c, const std::vector<char>& blob, std::vec
           for (int r = 0; r < NR; ++r) {
             const int a = reg_lds<R, NRB>(lds_swz<R>(cur[tid].ethr), cur[tid], r);
             if (a != lds_swz<R>(reg_elem<NRB>(cur[tid], r))) return false;  // linearity of the swizzle
+            // the planner's per-register byte offsets give the same address
+            if (h.ub_off) {
+              const uint16_t* ub = reinterpret_cast<const uint16_t*>(blob.data() + h.lds_off + h.ub_off);
+              if ((lds_swz<R>(cur[tid].ethr) * (int)sizeof(cplx<R>) ^ ub[(s - 1) * NR + r]) !=
+                  a * (int)sizeof(cplx<R>))
+                return false;
+            }
             lds[a] = x[tid][r];
             ++hits[a];
           }
