@@ -43,7 +43,7 @@
 
 namespace qj {
 
-constexpr int kTileMaxHighBits = 8;
+constexpr int kTileMaxHighBits = 12;  // QJ_TILE_MAX_BITS - 1 (one low bit at least)
 constexpr int kTileLA = 8;  // tile bits indexing phase table A
 
 struct TileSpec {
@@ -303,6 +303,18 @@ static void launch_tile(int bs, u64 grid, size_t lds, hipStream_t s, cplx<R>* st
 }
 
 // ------------------------------------------------------ register kernel --
+typedef __attribute__((address_space(3))) char lds_byte;
+// Read-only plan data at wave-uniform addresses, read through the constant
+// address space so that the loads are scalar.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(4))) const u32x4 const_u32x4;
+// One amplitude as a two-element vector (8- or 16-byte LDS accesses at an
+// integer LDS address).
+template <typename R>
+using amp2 = R __attribute__((ext_vector_type(2)));
+template <typename R>
+using lds_amp2 = __attribute__((address_space(3))) amp2<R>;
+
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ u64 uni64(u64 v) {
   return ((u64)(uint32_t)uni((int)(v >> 32)) << 32) | (uint32_t)uni((int)v);
@@ -332,7 +344,17 @@ __global__ void __launch_bounds__(BSMAX, WPE)
   }
   __syncthreads();
   const RegSeg* segs = reinterpret_cast<const RegSeg*>(lp + h.segs_off);
-  const uint16_t* ub_seg = h.ub_off ? reinterpret_cast<const uint16_t*>(lp + h.ub_off) : nullptr;
+  // LDS byte address of the tile buffer: re-layout addresses are formed as
+  // (base | thread part) ^ register part, which needs a 64 KiB-aligned base
+  // (the buffer is the kernel's only LDS, so it starts at 0)
+  const uint32_t lds_base = (uint32_t)(uintptr_t)(lds_byte*)smem_raw;
+  // (the register parts are read from the plan in global memory: uniform
+  // addresses, so they arrive in scalar registers)
+  const uint16_t* ub_seg = h.ub_off && (lds_base & 0xffffu) == 0
+                               ? reinterpret_cast<const uint16_t*>(plan + h.lds_off + h.ub_off)
+                               : nullptr;
+  const bool had = h.nhad != 0;
+  const R had_scale = (R)h.had_scale;
   const RegOp* ops = reinterpret_cast<const RegOp*>(lp + h.ops_off);
   const u64* ftab = reinterpret_cast<const u64*>(lp + h.ftab_off);
   const char* ldata = lp + h.ldata_off;
@@ -372,25 +394,29 @@ __global__ void __launch_bounds__(BSMAX, WPE)
         // segment's precomputed register part (the swizzle is linear); plans
         // without the offsets compute the register part per lane
         constexpr int AB = (int)sizeof(cplx<R>);
-        const int swb_cur = lds_swz<R>(cur.ethr) * AB, swb_nxt = lds_swz<R>(nxt.ethr) * AB;
-        char* lb = reinterpret_cast<char*>(lds);
         if (ub_seg) {
+          const uint32_t swb_cur = lds_base | (uint32_t)(lds_swz<R>(cur.ethr) * AB);
+          const uint32_t swb_nxt = lds_base | (uint32_t)(lds_swz<R>(nxt.ethr) * AB);
+          auto at = [](uint32_t a) { return (lds_amp2<R>*)(uintptr_t)a; };
           constexpr int NU = NR < 8 ? 1 : NR / 8;  // 16-byte groups of 8 offsets
-          uint4 u[NU];
+          u32x4 u[NU];
           auto off = [&](int r) {
-            const uint32_t w = (&u[r >> 3].x)[(r >> 1) & 3];
-            return (int)((r & 1) ? (w >> 16) : (w & 0xffffu));
+            const uint32_t w = u[r >> 3][(r >> 1) & 3];
+            return (r & 1) ? (w >> 16) : (w & 0xffffu);
           };
 #pragma unroll
-          for (int i = 0; i < NU; ++i) u[i] = reinterpret_cast<const uint4*>(ub_seg + (s - 1) * NR)[i];
+          for (int i = 0; i < NU; ++i) u[i] = ((const_u32x4*)(ub_seg + (s - 1) * NR))[i];
           __syncthreads();
 #pragma unroll
-          for (int r = 0; r < NR; ++r) *reinterpret_cast<cplx<R>*>(lb + (swb_cur ^ off(r))) = x[r];
+          for (int r = 0; r < NR; ++r) *at(swb_cur ^ off(r)) = amp2<R>{x[r].re, x[r].im};
 #pragma unroll
-          for (int i = 0; i < NU; ++i) u[i] = reinterpret_cast<const uint4*>(ub_seg + s * NR)[i];
+          for (int i = 0; i < NU; ++i) u[i] = ((const_u32x4*)(ub_seg + s * NR))[i];
           __syncthreads();
 #pragma unroll
-          for (int r = 0; r < NR; ++r) x[r] = *reinterpret_cast<const cplx<R>*>(lb + (swb_nxt ^ off(r)));
+          for (int r = 0; r < NR; ++r) {
+            const amp2<R> v = *at(swb_nxt ^ off(r));
+            x[r] = mk(v.x, v.y);
+          }
         } else {
           const int sw_cur = lds_swz<R>(cur.ethr), sw_nxt = lds_swz<R>(nxt.ethr);
           __syncthreads();
@@ -415,6 +441,13 @@ __global__ void __launch_bounds__(BSMAX, WPE)
           reg_phase<R, NRB>(x, f[uni(op.fidx)], d, NT, tid, rcm);
         else
           reg_dense_any<R, NRB>(x, kind, uni(op.mask), d, rcm);
+      }
+    }
+    if (had) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        x[r].re *= had_scale;
+        x[r].im *= had_scale;
       }
     }
 #pragma unroll

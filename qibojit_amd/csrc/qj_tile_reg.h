@@ -48,6 +48,10 @@ struct RegHead {
   // XOR this uniform value, instead of a per-lane swizzle.  Left out
   // (ub_off = 0) when the plan would not fit the LDS budget with it.
   u64 ub_off;
+  // Hadamard-form ops run as butterflies (a + b, a - b); the product of
+  // their factors h is applied once to every amplitude at the store.
+  double had_scale;              // 1 when nhad = 0
+  int32_t nhad, pad;
   uint8_t gbit_ld[kRegMaxBits];  // physical tile bit -> state bit (load)
   uint8_t gbit_st[kRegMaxBits];  // physical tile bit -> state bit (store)
 };
@@ -60,9 +64,13 @@ struct RegSeg {
 constexpr int kRegDense = 0;      // complex matrix
 constexpr int kRegPhase = 1;
 constexpr int kRegDenseReal = 2;  // 2x2 matrix with zero imaginary parts
+constexpr int kRegHad = 3;        // h * [[1, 1], [1, -1]], no controls: factor h deferred
+// Deferred factors per plan: the unscaled butterflies grow amplitudes by
+// at most sqrt(2) each, 2^16 for 32 of them, far from overflow in FP32.
+constexpr int kRegMaxHad = 32;
 
 struct RegOp {
-  int32_t kind;     // kRegDense, kRegPhase or kRegDenseReal
+  int32_t kind;     // kRegDense, kRegPhase, kRegDenseReal or kRegHad
   int32_t mask;     // dense: register-index bits of the targets
   uint32_t lct;     // local controls on thread bits (mask of tile bits)
   int32_t rcm;      // local controls on register bits (register-index bits)
@@ -230,9 +238,38 @@ __host__ __device__ __forceinline__ void reg_dense(cplx<R> (&x)[1 << NRB], const
   }
 }
 
+// Unscaled Hadamard butterfly on the register bit MASK: (a, b) -> (a + b, a - b).
+template <typename R, int NRB, int MASK>
+__host__ __device__ __forceinline__ void reg_had(cplx<R> (&x)[1 << NRB]) {
+  constexpr int NR = 1 << NRB;
+#pragma unroll
+  for (int g = 0; g < NR; ++g) {
+    if (g & MASK) continue;
+    cplx<R>& a = x[g];
+    cplx<R>& b = x[g | MASK];
+    const R s_re = a.re + b.re, s_im = a.im + b.im;
+    b.re = a.re - b.re;
+    b.im = a.im - b.im;
+    a.re = s_re;
+    a.im = s_im;
+  }
+}
+
 template <typename R, int NRB>
 __host__ __device__ __forceinline__ void reg_dense_any(cplx<R> (&x)[1 << NRB], int kind, int mask,
                                                        const cplx<R>* __restrict__ m, int rcm) {
+  if (kind == kRegHad) {
+    switch (mask) {
+#define QJ_REG_HAD(M)                                                          \
+  case M:                                                                      \
+    if constexpr (M < (1 << NRB)) reg_had<R, NRB, M>(x);                       \
+    break;
+      QJ_REG_HAD(1) QJ_REG_HAD(2) QJ_REG_HAD(4) QJ_REG_HAD(8) QJ_REG_HAD(16)
+#undef QJ_REG_HAD
+      default: break;
+    }
+    return;
+  }
   if (kind == kRegDenseReal) {
     switch (mask) {
 #define QJ_REG_REAL(M)                                                         \
@@ -471,6 +508,8 @@ inline bool reg_plan(const RegPlanIn& in, std::vector<char>& blob) {
   std::vector<RegOp> rops(pl.size());
   std::vector<char> ldata, gtab;
   std::vector<u64> ftab;
+  double had_scale = 1.0;
+  int nhad = 0;
   auto put = [](std::vector<char>& d, const void* src, size_t bytes) {
     const u64 off = d.size();
     d.resize(off + ((bytes + 15) & ~size_t(15)), 0);
@@ -525,6 +564,18 @@ inline bool reg_plan(const RegPlanIn& in, std::vector<char>& blob) {
             real = real && zero_im;
           }
         if (real) ro.kind = kRegDenseReal;
+        if (real && k == 1 && !ro.gctrl && !ro.lct && !ro.rcm && nhad < kRegMaxHad) {
+          // Hadamard form: m00 = m01 = m10 = -m11 exactly, in the state's precision
+          double e[4];
+          for (int q = 0; q < 4; ++q)
+            e[q] = in.esize == 8 ? (double)*reinterpret_cast<const float*>(mat.data() + q * 8)
+                                 : *reinterpret_cast<const double*>(mat.data() + q * 16);
+          if (e[0] != 0.0 && e[1] == e[0] && e[2] == e[0] && e[3] == -e[0]) {
+            ro.kind = kRegHad;
+            had_scale *= e[0];
+            ++nhad;
+          }
+        }
         ro.loff = put(ldata, mat.data(), mat.size());
       } else {
         // ABI block: int32 {ga, gb, tb, pad}, OA[2^ga], OB[2^gb], W[tb] (logical bits)
@@ -594,6 +645,8 @@ inline bool reg_plan(const RegPlanIn& in, std::vector<char>& blob) {
   h.nphase = (int)ftab.size();
   h.ga = in.ga;
   h.gb = in.gb;
+  h.had_scale = had_scale;
+  h.nhad = nhad;
   for (int b = 0; b < TB; ++b) {
     h.gbit_ld[b] = (uint8_t)in.gpos[b];
     h.gbit_st[b] = (uint8_t)in.gpos[inv[b]];

@@ -84,8 +84,14 @@ static Case<R> make_case(std::mt19937_64& rng, int nq, int L, int nhigh, int nrb
       if (rng() % 3 == 0) op.local_ctrl = u64(1) << bits[op.k];
       if (nout && rng() % 3 == 0) op.global_ctrl = u64(1) << outside[rng() % nout];
       std::vector<cplx<R>> m((size_t)1 << (2 * op.k));
-      const bool real = op.k == 1 && rng() % 2;  // real 2x2 matrices take their own kernel path
-      for (auto& v : m) v = real ? mk((R)U(rng), (R)0) : rc();
+      const int form = op.k == 1 ? (int)(rng() % 3) : 0;  // real and Hadamard 2x2 take their own paths
+      for (auto& v : m) v = form == 1 ? mk((R)U(rng), (R)0) : rc();
+      if (form == 2) {
+        // Hadamard form; deferred when it has no controls (cleared for half of them)
+        const R h = (R)(rng() % 2 ? 0.70710678118654752 : -0.5);
+        m = {mk(h, (R)0), mk(h, (R)0), mk(h, (R)0), mk(-h, (R)0)};
+        if (rng() % 2) op.local_ctrl = op.global_ctrl = 0;
+      }
       op.offset = put<R>(c.data, m.data(), m.size() * sizeof(cplx<R>));
     } else {
       op.kind = QJ_TILE_PHASE;
@@ -244,6 +250,10 @@ static bool run_kernel(const Case<R>& c, const std::vector<char>& blob, std::vec
       reg_state_offsets<NRB>(segs[nseg - 1], h.gbit_st, NT, tid, thr, rb);
       for (int r = 0; r < NR; ++r) {
         const u64 i = base | thr | reg_state_offset<NRB>(rb, r);
+        if (h.nhad) {
+          x[tid][r].re *= (R)h.had_scale;
+          x[tid][r].im *= (R)h.had_scale;
+        }
         st[i] = x[tid][r];
         ++written[i];
       }
@@ -298,8 +308,8 @@ static int check(std::mt19937_64& rng, int nq, int L, int nhigh, int nrb, int no
     scale = std::max(scale, std::abs(psi[i]));
   }
   const bool pass = ok && err <= tol * scale;
-  std::printf("%s %s nq=%d TB=%d nrb=%d ops=%d segs=%d rel_err=%.3e\n", pass ? "ok" : "FAIL",
-              sizeof(R) == 4 ? "c64" : "c128", nq, c.TB, nrb, nops, h.nseg, err / scale);
+  std::printf("%s %s nq=%d TB=%d nrb=%d ops=%d segs=%d deferred_h=%d rel_err=%.3e\n", pass ? "ok" : "FAIL",
+              sizeof(R) == 4 ? "c64" : "c128", nq, c.TB, nrb, nops, h.nseg, h.nhad, err / scale);
   return pass ? 0 : 1;
 }
 
@@ -313,6 +323,7 @@ int main() {
     bad += check<float>(rng, 13, 6, 7, 5, 16, 5, 1e-4);
     bad += check<double>(rng, 10, 5, 5, 4, 20, 4, 1e-12);
     bad += check<float>(rng, 11, 6, 4, 4, 20, 4, 1e-4);
+    bad += check<float>(rng, 14, 3, 10, 4, 24, 4, 1e-4);  // ten high bits, 64-byte runs
   }
   return bad ? 1 : 0;
 }

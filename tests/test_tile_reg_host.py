@@ -21,4 +21,4 @@ def test_register_tile_planner_matches_op_semantics(tmp_path):
     proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     lines = proc.stdout.strip().splitlines()
     assert proc.returncode == 0, proc.stdout[-2000:]
-    assert len(lines) == 36 and all(line.startswith("ok ") for line in lines)
+    assert len(lines) == 42 and all(line.startswith("ok ") for line in lines)

@@ -75,7 +75,8 @@ def _ops_with_swaps(n, rng, kmax):
 
 
 @pytest.mark.parametrize("kernel", ["register", "lds"])
-@pytest.mark.parametrize("n,low,nhigh", [(10, 5, 5), (12, 6, 6), (11, 1, 7), (13, 4, 8), (14, 6, 7)])
+@pytest.mark.parametrize("n,low,nhigh", [(10, 5, 5), (12, 6, 6), (11, 1, 7), (13, 4, 8), (14, 6, 7),
+                                          (14, 3, 10), (13, 1, 11)])
 def test_tile_kernel_matches_emulator(backend, monkeypatch, kernel, n, low, nhigh, dtype, rng):
     backend.set_dtype(dtype)
     npdt = np.complex64 if dtype == "complex64" else np.complex128
