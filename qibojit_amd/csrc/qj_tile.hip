@@ -358,6 +358,7 @@ __global__ void __launch_bounds__(BSMAX, WPE)
   const RegOp* ops = reinterpret_cast<const RegOp*>(lp + h.ops_off);
   const u64* ftab = reinterpret_cast<const u64*>(lp + h.ftab_off);
   const char* ldata = lp + h.ldata_off;
+  constexpr int AB = (int)sizeof(cplx<R>);
   u64 ld_thr, st_thr, ld_rb[NRB], st_rb[NRB];
   reg_state_offsets<NRB>(segs[0], h.gbit_ld, NT, tid, ld_thr, ld_rb);
   reg_state_offsets<NRB>(segs[nseg - 1], h.gbit_st, NT, tid, st_thr, st_rb);
@@ -366,37 +367,58 @@ __global__ void __launch_bounds__(BSMAX, WPE)
     ld_rb[j] = uni64(ld_rb[j]);
     st_rb[j] = uni64(st_rb[j]);
   }
+  // amplitude addresses: a wave-uniform part (the tile base and the register
+  // bits, scalar arithmetic) plus the lane's thread part in bytes
+  const u64 ld_thr_b = ld_thr * AB, st_thr_b = st_thr * AB;
+  // each thread's layout of segment 0 (the same for every tile) and, with
+  // the segment table, its tile bits and swizzled LDS offset in segments
+  // 1.. (written and read by that thread only)
+  RegLay<NRB> lay0 = reg_layout<NRB>(segs[0], NT, tid);
+#pragma unroll
+  for (int j = 0; j < NRB; ++j) lay0.rb[j] = uni(lay0.rb[j]);
+  const uint32_t swb0 = lds_base | (uint32_t)(lds_swz<R>(lay0.ethr) * AB);
+  const bool segtab = ub_seg && h.segtab;
+  uint32_t* stab = reinterpret_cast<uint32_t*>(fv + 2 * nphase) + tid;
+  const int nthr = 1 << NT;
+  if (segtab)
+    for (int s = 1; s < nseg; ++s) {
+      const int e = reg_layout<NRB>(segs[s], NT, tid).ethr;
+      stab[(s - 1) * nthr] = (uint32_t)e | ((uint32_t)(lds_swz<R>(e) * AB) << 16);
+    }
   int parity = 0;
   for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x, parity ^= 1) {
     const u64 base = insert_bits(t, tins);
     cplx<R>* f = fv + parity * nphase;
     if (tid < nphase) f[tid] = reg_phase_outside<R>(reinterpret_cast<const cplx<R>*>(plan + ftab[tid]), ga, t);
-    // opaque per tile: keeps the compiler from holding 2 x 2^NRB 64-bit
-    // addresses live across the tile loop (it recomputes them instead)
-    __asm__ volatile("" : "+v"(ld_thr), "+v"(st_thr));
     cplx<R> x[NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) x[r] = st[base | ld_thr | reg_state_offset<NRB>(ld_rb, r)];
+    for (int r = 0; r < NR; ++r) {
+      const char* p = reinterpret_cast<const char*>(st + (base | reg_state_offset<NRB>(ld_rb, r)));
+      x[r] = *reinterpret_cast<const cplx<R>*>(p + ld_thr_b);
+    }
     // the factors are written; every thread is done with the previous tile
     __syncthreads();
-    RegLay<NRB> cur = reg_layout<NRB>(segs[0], NT, tid);
-#pragma unroll
-    for (int j = 0; j < NRB; ++j) cur.rb[j] = uni(cur.rb[j]);
+    RegLay<NRB> cur = lay0;
+    uint32_t swb_cur = swb0;  // LDS byte address of the thread part in the current layout
     for (int s = 0; s < nseg; ++s) {
       const RegSeg& sg = segs[s];
       if (s > 0) {
         // re-layout through LDS; the first barrier keeps the previous
-        // re-layout's reads ahead of these writes
-        RegLay<NRB> nxt = reg_layout<NRB>(sg, NT, tid);
-#pragma unroll
-        for (int j = 0; j < NRB; ++j) nxt.rb[j] = uni(nxt.rb[j]);
-        // LDS byte address of register r = the thread part's offset XOR the
-        // segment's precomputed register part (the swizzle is linear); plans
-        // without the offsets compute the register part per lane
-        constexpr int AB = (int)sizeof(cplx<R>);
+        // re-layout's reads ahead of these writes.  LDS byte address of
+        // register r = the thread part's offset XOR the segment's
+        // precomputed register part (the swizzle is linear); plans without
+        // the offsets compute the register part per lane
         if (ub_seg) {
-          const uint32_t swb_cur = lds_base | (uint32_t)(lds_swz<R>(cur.ethr) * AB);
-          const uint32_t swb_nxt = lds_base | (uint32_t)(lds_swz<R>(nxt.ethr) * AB);
+          RegLay<NRB> nxt;
+          uint32_t swb_nxt;
+          if (segtab) {
+            const uint32_t v = stab[(s - 1) * nthr];
+            nxt.ethr = (int)(v & 0xffffu);
+            swb_nxt = lds_base | (v >> 16);
+          } else {
+            nxt.ethr = reg_layout<NRB>(sg, NT, tid).ethr;
+            swb_nxt = lds_base | (uint32_t)(lds_swz<R>(nxt.ethr) * AB);
+          }
           auto at = [](uint32_t a) { return (lds_amp2<R>*)(uintptr_t)a; };
           constexpr int NU = NR < 8 ? 1 : NR / 8;  // 16-byte groups of 8 offsets
           u32x4 u[NU];
@@ -417,7 +439,12 @@ __global__ void __launch_bounds__(BSMAX, WPE)
             const amp2<R> v = *at(swb_nxt ^ off(r));
             x[r] = mk(v.x, v.y);
           }
+          cur.ethr = nxt.ethr;
+          swb_cur = swb_nxt;
         } else {
+          RegLay<NRB> nxt = reg_layout<NRB>(sg, NT, tid);
+#pragma unroll
+          for (int j = 0; j < NRB; ++j) nxt.rb[j] = uni(nxt.rb[j]);
           const int sw_cur = lds_swz<R>(cur.ethr), sw_nxt = lds_swz<R>(nxt.ethr);
           __syncthreads();
 #pragma unroll
@@ -425,8 +452,8 @@ __global__ void __launch_bounds__(BSMAX, WPE)
           __syncthreads();
 #pragma unroll
           for (int r = 0; r < NR; ++r) x[r] = lds[reg_lds<R, NRB>(sw_nxt, nxt, r)];
+          cur = nxt;
         }
-        cur = nxt;
       }
       const int o1 = uni(sg.o1);
       for (int o = uni(sg.o0); o < o1; ++o) {
@@ -451,7 +478,10 @@ __global__ void __launch_bounds__(BSMAX, WPE)
       }
     }
 #pragma unroll
-    for (int r = 0; r < NR; ++r) st[base | st_thr | reg_state_offset<NRB>(st_rb, r)] = x[r];
+    for (int r = 0; r < NR; ++r) {
+      char* p = reinterpret_cast<char*>(st + (base | reg_state_offset<NRB>(st_rb, r)));
+      *reinterpret_cast<cplx<R>*>(p + st_thr_b) = x[r];
+    }
   }
 }
 
@@ -771,6 +801,13 @@ static int tile_pass_impl(void* state, int nqubits, int L, const int32_t* high, 
     }
     if (planned && lds_reg <= max_lds) {
       const int bs = 1 << (TB - nrb);
+      // per-thread table of the later segments' layouts, when it keeps two
+      // workgroups per CU
+      const size_t tab = hh.ub_off ? (size_t)(hh.nseg - 1) * (size_t)bs * 4 : 0;
+      if (tab && (lds_reg + tab) * 2 <= kLdsPerCu && lds_reg + tab <= max_lds) {
+        lds_reg += tab;
+        reinterpret_cast<RegHead*>(plan.data())->segtab = 1;
+      }
       int per_cu = (int)(kLdsPerCu / lds_reg);
       per_cu = std::min(per_cu, 2048 / bs);
       per_cu = std::max(per_cu, 1);

@@ -51,7 +51,8 @@ struct RegHead {
   // Hadamard-form ops run as butterflies (a + b, a - b); the product of
   // their factors h is applied once to every amplitude at the store.
   double had_scale;              // 1 when nhad = 0
-  int32_t nhad, pad;
+  int32_t nhad;
+  int32_t segtab;                // set by the launcher: per-thread table of segment layouts in LDS
   uint8_t gbit_ld[kRegMaxBits];  // physical tile bit -> state bit (load)
   uint8_t gbit_st[kRegMaxBits];  // physical tile bit -> state bit (store)
 };
@@ -651,6 +652,7 @@ inline bool reg_plan(const RegPlanIn& in, std::vector<char>& blob) {
     h.gbit_ld[b] = (uint8_t)in.gpos[b];
     h.gbit_st[b] = (uint8_t)in.gpos[inv[b]];
   }
+
   auto align16 = [](u64 v) { return (v + 15) & ~u64(15); };
   h.segs_off = 0;
   h.ops_off = align16((u64)nseg * sizeof(RegSeg));
