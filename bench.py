@@ -127,6 +127,9 @@ def kernel_name(backend, gate, n, dtype):
         return f"k_wave<{r},{k},{ilp}>"  # targets inside a 64-amplitude chunk: in-wave exchange
     if k == 1:
         kind = {"apply_x": 1, "apply_y": 2, "apply_z": 3, "apply_z_pow": 4}.get(GATE_OPS.get(name), 0)
+        m = n - 1 - gate.target_qubits[0]
+        if v == 2 and kind in (0, 1, 2) and (m >= 22 or m <= 10):
+            v = 1  # one complex64 amplitude per lane at these target bits (qj_gates.hip)
         return f"k_one_qubit<{r},{kind},{v}>"
     if k == 2:
         kind = {"apply_swap": 1, "apply_fsim": 2}.get(GATE_OPS.get(name), 0)

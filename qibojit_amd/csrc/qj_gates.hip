@@ -474,6 +474,17 @@ static int build_spec(int nqubits, const int* targets, int ntargets, const int32
   return QJ_OK;
 }
 
+// complex64: amplitudes per lane access of the one- and two-qubit kernels;
+// QJ_C64_LANES=1 or 2 forces one for every target position.
+static int c64_lanes_env() {
+  static const int v = [] {
+    const char* e = std::getenv("QJ_C64_LANES");
+    const int x = e ? std::atoi(e) : 0;
+    return (x == 1 || x == 2) ? x : 0;
+  }();
+  return v;
+}
+
 template <typename R, int KIND, int V>
 static void launch_one(cplx<R>* st, u64 ngroups, u64 tmask, const Ins& ins, const M1<R>& g,
                        hipStream_t s) {
@@ -502,8 +513,15 @@ static int one_qubit_impl(void* state, int nqubits, int kind, int m, const int32
     if (try_wave<R>(static_cast<cplx<R>*>(state), nqubits, &m, 1, ins, dense, s, &wrc)) return wrc;
   }
   const u64 npairs = u64(1) << (nqubits - ins.npos);
-  // complex64 lanes move two amplitudes (16 B) when bit 0 is free
-  const bool vec = sizeof(R) == 4 && ins.pos[0] >= 1;
+  // complex64 lanes move two amplitudes (16 B) when bit 0 is free, except
+  // in the kernels that read both halves of each pair (dense, X, Y) with the
+  // target at bit >= 22 or <= 10: there one amplitude (8 B) per lane was
+  // measured faster at 30 qubits (H on bit 29: 5.2 -> 5.9 TB/s; bits 1-10:
+  // +6-8%), while bits 11-21 and the one-sided Z kernels are faster with 16 B
+  const bool two_sided = kind != QJ_APPLY_Z && kind != QJ_APPLY_Z_POW;
+  const int lanes_env = c64_lanes_env();
+  const bool narrow = lanes_env ? lanes_env == 1 : (two_sided && (m >= 22 || m <= 10));
+  const bool vec = sizeof(R) == 4 && ins.pos[0] >= 1 && !narrow;
   const u64 ngroups = vec ? npairs / 2 : npairs;
   const u64 tmask = u64(1) << m;
   cplx<R>* st = static_cast<cplx<R>*>(state);
@@ -550,7 +568,7 @@ static int two_qubit_impl(void* state, int nqubits, int kind, int bit1, int bit2
     if (try_wave<R>(static_cast<cplx<R>*>(state), nqubits, tb, 2, ins, dense, s, &wrc)) return wrc;
   }
   const u64 nquads = u64(1) << (nqubits - ins.npos);
-  const bool vec = sizeof(R) == 4 && ins.pos[0] >= 1;
+  const bool vec = sizeof(R) == 4 && ins.pos[0] >= 1 && c64_lanes_env() != 1;
   const u64 ngroups = vec ? nquads / 2 : nquads;
   const u64 q1 = u64(1) << bit1, q2 = u64(1) << bit2;
   cplx<R>* st = static_cast<cplx<R>*>(state);

@@ -194,6 +194,9 @@ HIGH_BIT_ONE = [
     (22, 0, []),  # bit 21
     (22, 10, [21]),  # bit 11, control on bit 0
     (22, 15, [2, 9]),  # bit 6
+    (24, 0, []),  # bit 23 (complex64: one amplitude per lane)
+    (24, 1, [0]),  # bit 22, control on bit 23
+    (24, 13, [23]),  # bit 10 (complex64: one amplitude per lane), control on bit 0
 ]
 
 # two-target gates with at least one target bit >= 6
