@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale from a plain 4 GiB copy):
 # profiles/sweep_30q_c128_hbm_traffic.md.  Keyed by (kernel, nqubits).
-PROFILED_TRAFFIC = {("k_wave<double,1,1>", 30): 31.496e9, ("k_one_qubit<double,0,1>", 30): 28.633e9}
+PROFILED_TRAFFIC = {("k_wave<double,1,1>", 30): 34.360e9, ("k_one_qubit<double,0,1>", 30): 28.633e9}
 
 
 def parse():
