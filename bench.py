@@ -124,6 +124,8 @@ def kernel_name(backend, gate, n, dtype):
         ilp = os.environ.get("QJ_WAVE_ILP", "")
         if ilp not in ("1", "2", "4"):
             ilp = "2" if dtype == "complex64" else "1"
+        if k == 2 and dtype == "complex128" and GATE_OPS.get(name) in ("apply_swap", "apply_fsim"):
+            return f"k_wave_pair<{r},{ilp}>"  # matrix coupling r and r ^ 3 only: one shuffle
         return f"k_wave<{r},{k},{ilp}>"  # targets inside a 64-amplitude chunk: in-wave exchange
     if k == 1:
         kind = {"apply_x": 1, "apply_y": 2, "apply_z": 3, "apply_z_pow": 4}.get(GATE_OPS.get(name), 0)

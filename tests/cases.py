@@ -199,7 +199,7 @@ HIGH_BIT_ONE = [
     (24, 13, [23]),  # bit 10 (complex64: one amplitude per lane), control on bit 0
 ]
 
-# two-target gates with at least one target bit >= 6
+# two-target gates at strided and in-wave target bits
 HIGH_BIT_TWO = [
     (12, [0, 5], []),  # bits 11, 6
     (12, [5, 0], []),
@@ -210,4 +210,8 @@ HIGH_BIT_TWO = [
     (16, [14, 0], [7]),  # bits 1, 15
     (22, [0, 21], []),  # bits 21, 0
     (22, [10, 11], [21, 3]),  # bits 11, 10
+    # both targets inside the wave's 6 bits (the in-wave kernels, including
+    # the one-shuffle form for SWAP / fSim), controls inside and above it
+    (12, [10, 11], [8]),  # bits 1, 0, control on bit 3
+    (12, [11, 9], [0]),  # bits 0, 2, control on bit 11
 ]
