@@ -401,7 +401,10 @@ __device__ __forceinline__ unsigned tile_t(unsigned tid, int r) {
 
 // Rotations of the phase's four register bits (tile bits B0..B0+3); NROT
 // steps per bit (compile time, so the FP64 accumulators stay in registers).
-template <bool ADJ, int NROT, int B0, int A>
+// ZL: the last step of every rotated bit is a Z rotation done for the
+// whole tile by tile_zlast (before these steps in adjoint mode, after them
+// forward), so it is skipped here.
+template <bool ADJ, int NROT, int B0, int A, bool ZL = false>
 __device__ __forceinline__ void tile_rot(cplx<float> (&x)[16], cplx<float> (&y)[16], const cplx<float>* m,
                                          const RotTileArgs& a, double (&acc)[kTileBits * NROT]) {
   if constexpr (A < 4) {
@@ -409,10 +412,11 @@ __device__ __forceinline__ void tile_rot(cplx<float> (&x)[16], cplx<float> (&y)[
     if ((a.rotmask >> j) & 1) {
       if (!ADJ) {
 #pragma unroll
-        for (int s = 0; s < NROT; ++s) rot_apply_g<float, 4, A, false>(x, m + (j * NROT + s) * 4, a.gen[j][s]);
+        for (int s = 0; s < NROT - (ZL ? 1 : 0); ++s)
+          rot_apply_g<float, 4, A, false>(x, m + (j * NROT + s) * 4, a.gen[j][s]);
       } else {
 #pragma unroll
-        for (int s = NROT - 1; s >= 0; --s) {
+        for (int s = NROT - 1 - (ZL ? 1 : 0); s >= 0; --s) {
           const int g = a.gen[j][s];
           if (g) acc[j * NROT + s] += rot_gen<float, 4, A>(x, y, g);
           rot_apply_g<float, 4, A, true>(x, m + (j * NROT + s) * 4, g);
@@ -420,7 +424,70 @@ __device__ __forceinline__ void tile_rot(cplx<float> (&x)[16], cplx<float> (&y)[
         }
       }
     }
-    tile_rot<ADJ, NROT, B0, A + 1>(x, y, m, a, acc);
+    tile_rot<ADJ, NROT, B0, A + 1, ZL>(x, y, m, a, acc);
+  }
+}
+
+// Gradients Im<lam|Z_j|psi> = sum over k of (-1)^(k_j) Im(conj(lam_k) psi_k)
+// of the last (Z) steps of every rotated bit, from x = psi and y = lam in
+// the phase-A layout (thread bits t[0..7] = tid, register bits t[8..11] = r).
+template <int NROT>
+__device__ __forceinline__ void tile_zgrad(const cplx<float> (&x)[16], const cplx<float> (&y)[16],
+                                           const RotTileArgs& a, double (&acc)[kTileBits * NROT]) {
+  float c[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) c[r] = rfma(y[r].re, x[r].im, -y[r].im * x[r].re);
+  // thread bits: the thread's total with its bit's sign
+  float tot = c[0];
+#pragma unroll
+  for (int r = 1; r < 16; ++r) tot += c[r];
+#pragma unroll
+  for (int b = 0; b < 8; ++b)
+    if (((a.rotmask >> b) & 1) && a.gen[b][NROT - 1]) acc[b * NROT + NROT - 1] += (double)(((threadIdx.x >> b) & 1) ? -tot : tot);
+  // register bits 8..11
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int j = 8 + u;
+    if (((a.rotmask >> j) & 1) && a.gen[j][NROT - 1]) {
+      float sg = 0.f;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) sg += ((r >> u) & 1) ? -c[r] : c[r];
+      acc[j * NROT + NROT - 1] += (double)sg;
+    }
+  }
+}
+
+// Steps that are Z rotations (diagonal) and come last for every rotated
+// bit: rotations on different bits commute, so these steps of all 12 bits
+// are one diagonal (the product over the bits of d_j[k_j], conjugated in
+// adjoint mode) applied once per amplitude in the phase-A layout instead
+// of per bit -- after the other steps forward, before them in adjoint mode,
+// where the gradients of these steps are taken first (tile_zgrad).  gq: the
+// thread's factor times the register bits' bit-0 factors; q[u]: the ratio
+// a set register bit u adds (bit-1 over bit-0 factor).
+template <bool ADJ, int NROT>
+__device__ __forceinline__ void tile_zlast(cplx<float> (&x)[16], cplx<float> (&y)[16], const RotTileArgs& a,
+                                           double (&acc)[kTileBits * NROT], cplx<float> gq, const cplx<float> (&q)[4]) {
+  if constexpr (ADJ) tile_zgrad<NROT>(x, y, a, acc);
+  // registers 0..7 (bit 3 clear) from a product tree over bits 0..2, then
+  // the same factors times q[3] for registers 8..15
+  cplx<float> h[8];
+  h[0] = gq;
+  h[1] = cmul(gq, q[0]);
+  h[2] = cmul(gq, q[1]);
+  h[3] = cmul(h[1], q[1]);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) h[r + 4] = cmul(h[r], q[2]);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    x[r] = cmul(h[r], x[r]);
+    if (ADJ) y[r] = cmul(h[r], y[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const cplx<float> hh = cmul(h[r], q[3]);
+    x[r + 8] = cmul(hh, x[r + 8]);
+    if (ADJ) y[r + 8] = cmul(hh, y[r + 8]);
   }
 }
 
@@ -448,7 +515,7 @@ __device__ __forceinline__ void tile_transpose(cplx<float>* sm, cplx<float> (&x)
   __syncthreads();  // the buffers are free again
 }
 
-template <bool ADJ, int NROT>
+template <bool ADJ, int NROT, bool ZL = false>
 __global__ void __launch_bounds__(kTileBlock)
     k_rot_tile(const cplx<float>* psi_in, cplx<float>* psi_out, const cplx<float>* lam_in, cplx<float>* lam_out,
                const int32_t* __restrict__ sel, const cplx<float>* __restrict__ mats,
@@ -473,6 +540,31 @@ __global__ void __launch_bounds__(kTileBlock)
 #pragma unroll
   for (int j = 0; j < 8; ++j)
     if ((threadIdx.x >> j) & 1) offt |= u64(1) << a.tpos[j];
+  // ZL: the diagonal undo's thread factor (times the register bits'
+  // bit-0 factors) and the register bits' ratios, for every tile
+  // (products taken in FP64, rounded once)
+  cplx<float> gq = mk<float>(1.f, 0.f), q[4];
+  if constexpr (ZL) {
+    cplx<double> g = mk<double>(1.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < kTileBits; ++j) {
+      if (!((a.rotmask >> j) & 1)) {
+        if (j >= 8) q[j - 8] = mk<float>(1.f, 0.f);
+        continue;
+      }
+      const cplx<float>* d = m + (j * NROT + NROT - 1) * 4;  // diag(d[0], d[3])
+      const cplx<double> d0 = mk<double>(d[0].re, d[0].im), d1 = mk<double>(d[3].re, d[3].im);
+      const cplx<double> e0 = ADJ ? cconj(d0) : d0, e1 = ADJ ? cconj(d1) : d1;
+      if (j < 8) {
+        g = cmul(g, ((threadIdx.x >> j) & 1) ? e1 : e0);
+      } else {
+        g = cmul(g, e0);
+        const cplx<double> r = cmul(e1, cconj(e0));  // e1 / e0 (|e0| = 1)
+        q[j - 8] = mk<float>((float)r.re, (float)r.im);
+      }
+    }
+    gq = mk<float>((float)g.re, (float)g.im);
+  }
   for (u64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const u64 base = insert_bits(tile, a.ins) + offt;
     cplx<float> x[16], y[16];
@@ -487,12 +579,14 @@ __global__ void __launch_bounds__(kTileBlock)
         if (ADJ) y[r] = lin[src];
       }
     }
-    tile_rot<ADJ, NROT, 8, 0>(x, y, m, a, acc);
+    if constexpr (ZL && ADJ) tile_zlast<ADJ, NROT>(x, y, a, acc, gq, q);
+    tile_rot<ADJ, NROT, 8, 0, ZL>(x, y, m, a, acc);
     tile_transpose<ADJ, 0, 1>(sm, x, y);
-    tile_rot<ADJ, NROT, 4, 0>(x, y, m, a, acc);
+    tile_rot<ADJ, NROT, 4, 0, ZL>(x, y, m, a, acc);
     tile_transpose<ADJ, 1, 2>(sm, x, y);
-    tile_rot<ADJ, NROT, 0, 0>(x, y, m, a, acc);
+    tile_rot<ADJ, NROT, 0, 0, ZL>(x, y, m, a, acc);
     tile_transpose<ADJ, 2, 0>(sm, x, y);
+    if constexpr (ZL && !ADJ) tile_zlast<ADJ, NROT>(x, y, a, acc, gq, q);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       pout[base + a.offr[r]] = x[r];
@@ -838,13 +932,22 @@ int qj_rot_tile_pass(const void* psi_in, void* psi_out, const void* lam_in, void
   cplx<float>* po = (cplx<float>*)psi_out;
   const cplx<float>* li = (const cplx<float>*)lam_in;
   cplx<float>* lo = (cplx<float>*)lam_out;
+  // passes whose last step is a Z rotation on every rotated bit (RY then
+  // RZ layers) run those steps for the whole tile at once
+  bool zlast = nrot == 2;
+  for (int j = 0; zlast && j < kTileBits; ++j)
+    if (((rotmask >> j) & 1) && a.gen[j][nrot - 1] != 3) zlast = false;
 #define QJ_TILE(ADJ, NR) k_rot_tile<ADJ, NR><<<gd, dim3(kTileBlock), lds, st>>>(pi, po, li, lo, sel, m, partials, ntiles, a)
   if (adj) {
     if (nrot == 1) QJ_TILE(true, 1);
+    else if (nrot == 2 && zlast)
+      k_rot_tile<true, 2, true><<<gd, dim3(kTileBlock), lds, st>>>(pi, po, li, lo, sel, m, partials, ntiles, a);
     else if (nrot == 2) QJ_TILE(true, 2);
     else QJ_TILE(true, 3);
   } else {
     if (nrot == 1) QJ_TILE(false, 1);
+    else if (nrot == 2 && zlast)
+      k_rot_tile<false, 2, true><<<gd, dim3(kTileBlock), lds, st>>>(pi, po, li, lo, sel, m, partials, ntiles, a);
     else if (nrot == 2) QJ_TILE(false, 2);
     else QJ_TILE(false, 3);
   }

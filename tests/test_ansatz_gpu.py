@@ -355,18 +355,20 @@ def test_rot_tile_pass_against_numpy(torch_mod, rng, load):
             return x[np.argsort(idx ^ (idx >> 1))]
         return x
 
-    # forward
-    src = torch.from_numpy(host).cuda()
-    dst = src.clone() if load == 0 else torch.zeros_like(src)
-    _native.call("qj_rot_tile_pass", vp(src), vp(dst), None, None, _native.QJ_COMPLEX64, n, 1 << n, None, nstates,
-                 bits_ptr, rotmask, nrot, None, load, vp(dm), 12 * nrot * 4, None, 2, None)
-    got = _np(dst)
-    for s in range(nstates):
-        x = permuted(host[s].astype(np.complex128))
-        for j in rot:
-            for st in range(nrot):
-                x = einsum_ref.apply_gate(x, mats[s, j, st], [n - 1 - tile_bits[j]], nqubits=n)
-        assert np.max(np.abs(got[s] - x)) < AMP["complex64"]
+    # forward, without and with the generators (with them, the Z steps of
+    # every rotated bit run as one diagonal after the RY steps)
+    for gen_ptr in (None, gptr):
+        src = torch.from_numpy(host).cuda()
+        dst = src.clone() if load == 0 else torch.zeros_like(src)
+        _native.call("qj_rot_tile_pass", vp(src), vp(dst), None, None, _native.QJ_COMPLEX64, n, 1 << n, None,
+                     nstates, bits_ptr, rotmask, nrot, gen_ptr, load, vp(dm), 12 * nrot * 4, None, 2, None)
+        got = _np(dst)
+        for s in range(nstates):
+            x = permuted(host[s].astype(np.complex128))
+            for j in rot:
+                for st in range(nrot):
+                    x = einsum_ref.apply_gate(x, mats[s, j, st], [n - 1 - tile_bits[j]], nqubits=n)
+            assert np.max(np.abs(got[s] - x)) < AMP["complex64"]
     # adjoint
     src = torch.from_numpy(host).cuda()
     lsrc = torch.from_numpy(lamh).cuda()
