@@ -11,7 +11,8 @@ state holding an amplitude the gate can change) -- read + write of every
 touched line, counted from the gate's index set, not measured.
 
 Also reported (not `value`): the same sweep in complex64, per-gate GB/s, the
-dominant kernel's roofline, QFT on 32 qubits complex64 (seconds), collapse /
+dominant kernel's roofline, the 20-qubit depth-20 random circuit (seconds),
+QFT on 32 qubits complex64 (seconds), collapse /
 probabilities / 10^6-shot sampling on qubits {0, 7, 29}, a 33-qubit
 complex128 capacity run, and the CPU baseline: the C/OpenMP restatement of
 the reference's numba kernels on this host's cores.
@@ -53,6 +54,7 @@ def parse():
     p.add_argument("--only-mfma", action="store_true", help="the 5-qubit block kernels alone (for counter runs)")
     p.add_argument("--only-measure", action="store_true", help="probabilities, sampling and collapse only")
     p.add_argument("--only-qft", action="store_true", help="32-qubit QFT through the tile executor only")
+    p.add_argument("--only-random", action="store_true", help="20-qubit depth-20 random circuit only")
     p.add_argument("--blocks", default="low,mid,high,low_ctrl",
                    help="placements of --only-mfma (comma-separated: low, mid, high, low_ctrl)")
     p.add_argument("--block-paths", default="mfma,valu", help="kernels of --only-mfma: mfma, valu or both")
@@ -316,6 +318,74 @@ def brickwork(n, depth, seed):
             qm, r = np.linalg.qr(z)
             c.add(gates.Unitary(qm * (np.diag(r) / np.abs(np.diag(r))), q, q + 1))
     return c
+
+
+def random_circuit_20q(n=20, depth=20, seed=1234):
+    """Depth-`depth` layers of U3 (angles U[0, 2pi)) on every qubit, then CZ
+    on the pairs of a seeded random permutation redrawn per layer."""
+    from qibojit_amd._qibo import Circuit, gates
+
+    rng = np.random.default_rng(seed)
+    c = Circuit(n)
+    for _ in range(depth):
+        for q in range(n):
+            c.add(gates.U3(q, *rng.uniform(0, 2 * np.pi, 3)))
+        perm = rng.permutation(n)
+        for j in range(n // 2):
+            c.add(gates.CZ(int(perm[2 * j]), int(perm[2 * j + 1])))
+    return c
+
+
+def run_random_circuit(backend, n=20, depth=20, reps=5, cpu=True):
+    """The 20-qubit complex128 random circuit from |0...0>: the state is
+    16.8 MB and stays in the L2/MALL caches, so this is a time, not a
+    bandwidth, figure.  Times execute_circuit with the fused executor
+    (host planning included) and the gate-by-gate apply_gate loop (median of
+    `reps` runs each, wall clock after a device synchronisation), and the
+    C/OpenMP restatement of the numba kernels on the host cores once."""
+    import torch
+
+    c = random_circuit_20q(n, depth)
+    backend.set_dtype("complex128")
+    out = {"nqubits": n, "depth": depth, "gates": len(c.queue)}
+
+    def fused():
+        return backend.execute_circuit(c).state()
+
+    def plain():
+        st = backend.zero_state(n)
+        for g in c.queue:
+            st = backend.apply_gate(g, st, n)
+        return st
+
+    results = {}
+    for label, fn in (("fused", fused), ("gate_by_gate", plain)):
+        fn()  # first call builds plans and warms the allocator
+        runs = []
+        for _ in range(reps):
+            st, sec = _timed(fn)
+            runs.append(sec)
+        results[label] = st
+        med = float(np.median(runs))
+        out[f"{label}_s"] = med
+        out[f"{label}_gates_per_s"] = len(c.queue) / med
+    got_f = backend.to_numpy(results["fused"])
+    got_p = backend.to_numpy(results["gate_by_gate"])
+    out["max_abs_diff_fused_vs_gate_by_gate"] = float(np.abs(got_f - got_p).max())
+    if cpu:
+        from oracle import port
+
+        threads = len(os.sched_getaffinity(0))
+        pb = port.PortBackend(nthreads=threads)
+        t = time.perf_counter()
+        want = pb.execute_circuit(c)
+        cpu_s = time.perf_counter() - t
+        out.update({"cpu_port_s": cpu_s, "cpu_port_threads": threads,
+                    "max_abs_err_vs_cpu_port": float(np.abs(got_f - want).max()),
+                    "speedup_fused_vs_cpu_port": cpu_s / out["fused_s"]})
+    del results
+    torch.cuda.empty_cache()
+    return out
 
 
 def _timed(fn):
@@ -965,6 +1035,10 @@ def main():
         for dt in ("complex64", "complex128"):
             print(json.dumps({f"qft_32q_{dt}_fused": run_qft(backend, 32, dt, fused=True, reps=3)}), flush=True)
         return
+    if args.only_random:
+        print(json.dumps({"random_circuit_20q_complex128": run_random_circuit(backend, cpu=not args.no_cpu)}),
+              flush=True)
+        return
     if args.only_measure:
         for dt in ("complex128", "complex64"):
             print(json.dumps({f"measurement_{dt}": run_measurement_ops(backend, args.nqubits, dt)}), flush=True)
@@ -994,6 +1068,10 @@ def main():
         meas = run_measurement_ops(backend, n, "complex128")
         emit("measurement_complex128", meas)
         summary["collapse_c128_ms"] = round(meas["collapse_normalized"]["ms"], 3)
+        print("# random circuit 20q complex128", file=sys.stderr, flush=True)
+        rc = run_random_circuit(backend, cpu=not args.no_cpu)
+        emit("random_circuit_20q_complex128", rc)
+        summary["random_20q_c128_fused_ms"] = round(rc["fused_s"] * 1e3, 3)
         print("# qft 32q complex64", file=sys.stderr, flush=True)
         qft = run_qft(backend, 32, "complex64", fused=True, reps=3)
         emit("qft_32q_complex64_fused", qft)

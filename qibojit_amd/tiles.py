@@ -26,6 +26,9 @@ execution up to rounding.
 
 import cmath
 import ctypes
+import functools
+import math
+import struct
 
 import numpy as np
 
@@ -45,6 +48,14 @@ TILE_OP_DTYPE = np.dtype(
     ]
 )
 assert TILE_OP_DTYPE.itemsize == 64
+# one TILE_OP_DTYPE record packed in a single call (kind, k, target[6],
+# local_ctrl, global_ctrl, offset, reserved); the phase-op data header
+_OP_RECORD = struct.Struct("<ii6iQQQQ")
+assert _OP_RECORD.size == TILE_OP_DTYPE.itemsize
+_PHASE_HEAD = struct.Struct("<4i")
+_ZEROS = (0,) * (MAX_K + 1)
+_PAULI_X = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+_PAULI_X.flags.writeable = False
 
 
 class DenseOp:
@@ -110,15 +121,22 @@ def gate_to_op(custom_matrix_of, gate, nqubits):
         return None
     tbits = [nqubits - 1 - q for q in targets]
     cbits = [nqubits - 1 - q for q in gate.control_qubits]
-    if GATE_OPS.get(type(gate).__name__) == "apply_z_pow" and len(targets) == 1:
+    kind = GATE_OPS.get(type(gate).__name__)
+    # fixed-matrix families skip the matrix encoding (small circuits spend
+    # most of their host time here)
+    if kind == "apply_z" and len(targets) == 1:
+        return PhaseOp(cbits, 0.0, {tbits[0]: math.pi})  # diag(1, -1)
+    if kind == "apply_x" and len(targets) == 1:
+        return DenseOp(tbits, cbits, _PAULI_X)
+    if kind == "apply_swap" and not cbits:
+        return SwapOp(*tbits)
+    if kind == "apply_z_pow" and len(targets) == 1:
         # U1 / CU1: diag(1, z) without building the matrix (the QFT's ladders
         # are hundreds of these)
         z = complex(np.asarray(custom_matrix_of(gate)))
         if abs(abs(z) - 1) < 1e-14:
             return PhaseOp(cbits, 0.0, {tbits[0]: cmath.phase(z)})
     u = target_block(custom_matrix_of, gate)
-    if type(gate).__name__ == "SWAP" and not cbits:
-        return SwapOp(*tbits)
     if len(targets) == 1 and u[0, 1] == 0 and u[1, 0] == 0:
         d0, d1 = complex(u[0, 0]), complex(u[1, 1])
         if abs(abs(d0) - 1) < 1e-14 and abs(abs(d1) - 1) < 1e-14:
@@ -263,6 +281,14 @@ def plan(ops, nqubits, low, max_high):
     return passes + pl.finish()
 
 
+@functools.lru_cache(maxsize=256)
+def _bit_column(size, sh):
+    """Bit ``sh`` of 0 .. size-1 as float64 (a read-only table)."""
+    col = ((np.arange(size) >> sh) & 1).astype(np.float64)
+    col.flags.writeable = False
+    return col
+
+
 def encode(p, nqubits, npdtype):
     """(ops array, data bytes, high bits) of a pass for qj_apply_tile_pass."""
     low, high = p.low, p.high
@@ -274,50 +300,42 @@ def encode(p, nqubits, npdtype):
     nout = len(outside)
     ga = (nout + 1) // 2
     rank = {b: r for r, b in enumerate(outside)}
-    ops = np.zeros(len(p.ops), dtype=TILE_OP_DTYPE)
-    chunks, off = [], 0
-
-    def put(buf):
-        nonlocal off
-        pad = (-len(buf)) % 16
-        chunks.append(buf + b"\0" * pad)
-        start = off
-        off += len(buf) + pad
-        return start
-
-    for i, op in enumerate(p.ops):
+    recs, chunks = [], []
+    off = 0
+    for op in p.ops:
         if isinstance(op, SwapOp):
-            ops[i]["kind"] = TILE_SWAP
-            ops[i]["k"] = 2
-            ops[i]["target"][:2] = [local[b] for b in op.targets]
+            a, b = op.targets
+            recs.append(_OP_RECORD.pack(TILE_SWAP, 2, local[a], local[b], 0, 0, 0, 0, 0, 0, 0, 0))
             continue
-        lmask = sum(1 << local[c] for c in op.controls if c in local)
-        gmask = sum(1 << c for c in op.controls if c not in local)
-        ops[i]["local_ctrl"] = lmask
-        ops[i]["global_ctrl"] = gmask
+        lmask = gmask = 0
+        for c in op.controls:
+            if c in local:
+                lmask |= 1 << local[c]
+            else:
+                gmask |= 1 << c
         if isinstance(op, DenseOp):
-            ops[i]["kind"] = TILE_DENSE
-            ops[i]["k"] = len(op.targets)
-            ops[i]["target"][: len(op.targets)] = [local[b] for b in op.targets]
-            ops[i]["offset"] = put(np.ascontiguousarray(op.matrix, dtype=npdtype).tobytes())
+            tg = [local[b] for b in op.targets]
+            blob = np.ascontiguousarray(op.matrix, dtype=npdtype).tobytes()
+            recs.append(_OP_RECORD.pack(TILE_DENSE, len(tg), *tg, *_ZEROS[len(tg):], lmask, gmask, off, 0))
         else:
-            ops[i]["kind"] = TILE_PHASE
-            head = np.array([ga, nout - ga, tb, 0], dtype="<i4")
             # OA/OB: angle tables over the tile index t (c0 folded into OA);
             # W: one angle per tile-local bit
-            oa, ob, w = np.zeros(2**ga), np.zeros(2 ** (nout - ga)), np.zeros(tb)
-            oa += op.c0
+            oa, ob, w = np.full(2**ga, op.c0), np.zeros(2 ** (nout - ga)), np.zeros(tb)
             for b, th in op.thetas.items():
                 if b in local:
                     w[local[b]] += th
                 else:
                     r = rank[b]
                     tab, sh = (oa, r) if r < ga else (ob, r - ga)
-                    tab += th * ((np.arange(tab.size) >> sh) & 1)
-            blob = head.tobytes() + b"".join(np.exp(1j * a).astype(npdtype).tobytes() for a in (oa, ob, w))
-            ops[i]["offset"] = put(blob)
-    data = b"".join(chunks)
-    return ops, data, high
+                    tab += th * _bit_column(tab.size, sh)
+            ang = np.concatenate((oa, ob, w))
+            blob = _PHASE_HEAD.pack(ga, nout - ga, tb, 0) + np.exp(1j * ang).astype(npdtype).tobytes()
+            recs.append(_OP_RECORD.pack(TILE_PHASE, 0, *_ZEROS, lmask, gmask, off, 0))
+        pad = (-len(blob)) % 16
+        chunks.append(blob + b"\0" * pad if pad else blob)
+        off += len(blob) + pad
+    ops = np.frombuffer(b"".join(recs), dtype=TILE_OP_DTYPE)
+    return ops, b"".join(chunks), high
 
 
 def run_passes(lib_call, state_ptr, dtype_code, nqubits, passes, npdtype, stream):
