@@ -67,6 +67,11 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+# The current stream of a device as a raw pointer without building a Stream
+# object (torch.cuda.current_stream costs a few microseconds per launch)
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _device_empty(numel, tdtype, device):
     """Uninitialised device tensor; an allocation the device cannot hold
     raises MemoryError (numpy's error for the same case on NumbaBackend),
@@ -147,6 +152,8 @@ class HipBackend(Backend):
 
     # ------------------------------------------------------------ arrays --
     def _stream(self):
+        if _RAW_STREAM is not None:
+            return ctypes.c_void_p(_RAW_STREAM(self._torch_device.index))
         return ctypes.c_void_p(torch.cuda.current_stream(self._torch_device).cuda_stream)
 
     def cast(self, array, dtype=None, copy=False):
