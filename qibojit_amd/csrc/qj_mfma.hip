@@ -38,6 +38,7 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int kMfmaBlock = 256;  // 4 waves, 32 columns each per step
 constexpr u64 kMfmaMaxGrid = 8192;
+constexpr u64 kMfmaMaxGrid4 = 131072;
 
 __global__ void __launch_bounds__(kMfmaBlock)
     k_block5_mfma(cplx<float>* __restrict__ st, u64 ncols, Ins ins, Offs32 off, const float* __restrict__ w) {
@@ -403,7 +404,13 @@ static int block_impl(void* state, int nqubits, const int32_t* target_bits, int 
   if (rc) return rc;
   const R* du = static_cast<const R*>(dw);
   u64 grid = (ncols / 16 + (kMfmaBlock / 64) - 1) / (kMfmaBlock / 64);
-  if (grid > kMfmaMaxGrid) grid = kMfmaMaxGrid;
+  // grid cap: each wave loads the matrix once, so 5-target blocks (12 / 24
+  // KiB of operands per wave) keep 8192 workgroups; 4-target blocks (3 / 6
+  // KiB) run faster on a larger grid (131072: +1-7% at 32 qubits complex64
+  // and 30 qubits complex128, most on the lowest bits; 8x larger grids
+  // lost on the middle bits)
+  const u64 cap = K == 4 ? kMfmaMaxGrid4 : kMfmaMaxGrid;
+  if (grid > cap) grid = cap;
   // The lowest K + 4 bits hold exactly the targets and 4 column bits: a
   // wave's 16 columns are one contiguous run (staged through LDS).
   const int low = K + 4;
