@@ -38,7 +38,15 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int kMfmaBlock = 256;  // 4 waves, 32 columns each per step
 constexpr u64 kMfmaMaxGrid = 8192;
-constexpr u64 kMfmaMaxGrid4 = 131072;
+
+// Workgroups of the 3M kernel: complex64 K = 4 131072 and K = 5 32768,
+// complex128 K = 4 524288 and K = 5 8192 (the complex128 5-target kernel,
+// 2 waves per SIMD, lost 15-60% on larger grids; the others gained 2-9% on
+// the lowest bits over 8192 and stayed within noise elsewhere).
+static u64 mfma_grid_cap(bool f64, int K) {
+  if (K == 4) return f64 ? 524288 : 131072;
+  return f64 ? 8192 : 32768;
+}
 
 __global__ void __launch_bounds__(kMfmaBlock)
     k_block5_mfma(cplx<float>* __restrict__ st, u64 ncols, Ins ins, Offs32 off, const float* __restrict__ w) {
@@ -201,15 +209,20 @@ __global__ void __launch_bounds__(kMfmaBlock)
   __shared__ cplx<R> stage[STAGE ? kMfmaBlock / 64 : 1][STAGE ? stage_slots<K>() : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int q = lane >> 4, n = lane & 15;
+  // the three real NR x NR blocks are read from global memory once per
+  // workgroup (not once per wave), then each wave takes its A operands
+  __shared__ R umat[3 * NR * NR];
+  for (int i = threadIdx.x; i < 3 * NR * NR; i += kMfmaBlock) umat[i] = u3[i];
+  __syncthreads();
   R ur[T][S], ui[T][S], us[T][S];
 #pragma unroll
   for (int t = 0; t < T; ++t)
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int e = (16 * t + n) * NR + 4 * s + q;
-      ur[t][s] = u3[e];
-      ui[t][s] = u3[NR * NR + e];
-      us[t][s] = u3[2 * NR * NR + e];
+      ur[t][s] = umat[e];
+      ui[t][s] = umat[NR * NR + e];
+      us[t][s] = umat[2 * NR * NR + e];
     }
   // chunk-local index of this lane's column (targets inserted as zeros)
   const unsigned colbase = STAGE ? (unsigned)(insert_bits((u64)n, ins) & lowmask) : 0u;
@@ -404,13 +417,16 @@ static int block_impl(void* state, int nqubits, const int32_t* target_bits, int 
   if (rc) return rc;
   const R* du = static_cast<const R*>(dw);
   u64 grid = (ncols / 16 + (kMfmaBlock / 64) - 1) / (kMfmaBlock / 64);
-  // grid cap: each wave loads the matrix once, so 5-target blocks (12 / 24
-  // KiB of operands per wave) keep 8192 workgroups; 4-target blocks (3 / 6
-  // KiB) run faster on a larger grid (131072: +1-7% at 32 qubits complex64
-  // and 30 qubits complex128, most on the lowest bits; 8x larger grids
-  // lost on the middle bits)
-  const u64 cap = K == 4 ? kMfmaMaxGrid4 : kMfmaMaxGrid;
+  // grid cap per precision and block width (each workgroup loads the
+  // matrix once: 3 / 6 KiB for K = 4, 12 / 24 KiB for K = 5), measured at
+  // 32 qubits complex64 and 30 qubits complex128 over caps of 8192..2^21
+  const u64 cap = mfma_grid_cap(sizeof(R) == 8, K);
   if (grid > cap) grid = cap;
+  // at least two 16-column groups per wave: one group per wave reloads the
+  // matrix for every 2^K x 16 amplitudes (complex128 K = 4 with a control:
+  // 58% of the HBM peak at one group per wave, 68% at two or more)
+  const u64 two_per_wave = ncols / (2 * 16 * (kMfmaBlock / 64));
+  if (two_per_wave >= 1 && grid > two_per_wave) grid = two_per_wave;
   // The lowest K + 4 bits hold exactly the targets and 4 column bits: a
   // wave's 16 columns are one contiguous run (staged through LDS).
   const int low = K + 4;
