@@ -254,17 +254,22 @@ def test_rot_pass_zero_load_builds_product_state(torch_mod, rng):
     np.testing.assert_allclose(_np(dst), x, atol=1e-14)
 
 
+_TFIM_MATRICES = {}
+
+
+# n = 11 with ncols 1 and 3: several grid-stride iterations per workgroup
+# and unevenly loaded workgroups
+@pytest.mark.parametrize("n,ncols", [(8, 2), (11, 1), (11, 3)])
 @pytest.mark.parametrize("ladder", [0, 1])
-def test_tfim_batch_energy_and_apply(torch_mod, dtype, rng, ladder):
+def test_tfim_batch_energy_and_apply(torch_mod, dtype, rng, ladder, n, ncols):
     from qibojit_amd import _native
 
     torch = torch_mod
-    n, nstates, g = 8, 3, 0.6
+    nstates, g = 3, 0.6
     code = _native.QJ_COMPLEX64 if dtype == "complex64" else _native.QJ_COMPLEX128
     host = np.stack([random_state(n, rng) for _ in range(nstates)]).astype(dtype)
     psi = torch.from_numpy(host).cuda()
     out = torch.zeros_like(psi)
-    ncols = 2
     part = torch.zeros((nstates, ncols), dtype=torch.float64).cuda()
     part2 = torch.zeros_like(part)
     sums = torch.zeros(nstates, dtype=torch.float64).cuda()
@@ -274,7 +279,9 @@ def test_tfim_batch_energy_and_apply(torch_mod, dtype, rng, ladder):
     _native.call("qj_tfim_batch", vp(psi), None, code, n, 1 << n, None, nstates, g, ladder, vp(part2), ncols, None)
     _native.call("qj_reduce_rows", vp(part), nstates, ncols, vp(sums), None)
     _native.call("qj_reduce_rows", vp(part2), nstates, ncols, vp(sums2), None)
-    h = einsum_ref.tfim_matrix(n, g)
+    if (n, g) not in _TFIM_MATRICES:
+        _TFIM_MATRICES[(n, g)] = einsum_ref.tfim_matrix(n, g)
+    h = _TFIM_MATRICES[(n, g)]
     perm = ansatz_ref.ladder_permutation(n)
     inv = np.argsort(perm)
     for s in range(nstates):
