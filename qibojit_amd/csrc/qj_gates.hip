@@ -160,20 +160,34 @@ __global__ void __launch_bounds__(kBlock)
                 const cplx<R>* __restrict__ mat) {
   constexpr int N = 1 << K;
   __shared__ cplx<R> sm[N * N];
-  for (int t = threadIdx.x; t < N * N; t += kBlock) sm[t] = mat[t];
-  __syncthreads();
   const u64 stride = (u64)gridDim.x * kBlock;
-  for (u64 g = (u64)blockIdx.x * kBlock + threadIdx.x; g < ngroups; g += stride) {
-    const u64 base = insert_bits(g, ins);
-    cplx<R> x[N];
+  u64 g = (u64)blockIdx.x * kBlock + threadIdx.x;
+  // the first group's loads are issued before the matrix is staged, so the
+  // workgroup barrier does not delay them (one group per thread is the
+  // usual launch)
+  u64 base = 0;
+  cplx<R> x[N];
+  if (g < ngroups) {
+    base = insert_bits(g, ins);
 #pragma unroll
     for (int c = 0; c < N; ++c) x[c] = st[base + off.o[c]];
+  }
+  for (int t = threadIdx.x; t < N * N; t += kBlock) sm[t] = mat[t];
+  __syncthreads();
+  for (; g < ngroups; g += stride) {
 #pragma unroll 1
     for (int r = 0; r < N; ++r) {
       cplx<R> acc = cmul(sm[r * N], x[0]);
 #pragma unroll
       for (int c = 1; c < N; ++c) acc = cadd(acc, cmul(sm[r * N + c], x[c]));
       st[base + off.o[r]] = acc;
+    }
+    // the next group's amplitudes (every row of this one is stored: the
+    // update stays in place)
+    if (g + stride < ngroups) {
+      base = insert_bits(g + stride, ins);
+#pragma unroll
+      for (int c = 0; c < N; ++c) x[c] = st[base + off.o[c]];
     }
   }
 }
