@@ -5,9 +5,21 @@ first for the named controlled gates, e.g. CNOT(control, target)).  These are
 the textbook forms qibo documents for each gate.
 """
 
+import cmath
+import math
 from functools import cached_property
 
 import numpy as np
+
+_REAL_SCALARS = (float, int, np.floating, np.integer)
+
+
+def _scalars(*xs):
+    """Whether every angle is a real Python / NumPy scalar: those take the
+    scalar `math` / `cmath` forms below (one array built per matrix, a few
+    microseconds less per gate than NumPy scalar arithmetic; the same
+    formulas, equal to within an ulp)."""
+    return all(isinstance(x, _REAL_SCALARS) and not isinstance(x, bool) for x in xs)
 
 
 class NumpyMatrices:
@@ -65,18 +77,29 @@ class NumpyMatrices:
         return self._cast([[1, 0], [0, np.exp(-1j * np.pi / 4)]], dtype=self.dtype)
 
     def RX(self, theta):
+        if _scalars(theta):
+            cos, isin = math.cos(theta / 2), -1j * math.sin(theta / 2)
+            return np.array([[cos, isin], [isin, cos]], dtype=self.dtype)
         cos, isin = np.cos(theta / 2), -1j * np.sin(theta / 2)
         return self._cast([[cos, isin], [isin, cos]], dtype=self.dtype)
 
     def RY(self, theta):
+        if _scalars(theta):
+            cos, sin = math.cos(theta / 2), math.sin(theta / 2)
+            return np.array([[cos, -sin], [sin, cos]], dtype=self.dtype)
         cos, sin = np.cos(theta / 2), np.sin(theta / 2)
         return self._cast([[cos, -sin], [sin, cos]], dtype=self.dtype)
 
     def RZ(self, theta):
+        if _scalars(theta):
+            phase = cmath.exp(0.5j * theta)
+            return np.array([[phase.conjugate(), 0], [0, phase]], dtype=self.dtype)
         phase = np.exp(0.5j * theta)
         return self._cast([[np.conj(phase), 0], [0, phase]], dtype=self.dtype)
 
     def U1(self, theta):
+        if _scalars(theta):
+            return np.array([[1, 0], [0, cmath.exp(1j * theta)]], dtype=self.dtype)
         return self._cast([[1, 0], [0, np.exp(1j * theta)]], dtype=self.dtype)
 
     def U2(self, phi, lam):
@@ -88,6 +111,14 @@ class NumpyMatrices:
         )
 
     def U3(self, theta, phi, lam):
+        if _scalars(theta, phi, lam):
+            cost, sint = math.cos(theta / 2), math.sin(theta / 2)
+            eplus = cmath.exp(1j * (phi + lam) / 2.0)
+            eminus = cmath.exp(1j * (phi - lam) / 2.0)
+            return np.array(
+                [[eplus.conjugate() * cost, -eminus.conjugate() * sint], [eminus * sint, eplus * cost]],
+                dtype=self.dtype,
+            )
         cost, sint = np.cos(theta / 2), np.sin(theta / 2)
         eplus = np.exp(1j * (phi + lam) / 2.0)
         eminus = np.exp(1j * (phi - lam) / 2.0)
