@@ -11,14 +11,16 @@ from functools import cached_property
 
 import numpy as np
 
-_REAL_SCALARS = (float, int, np.floating, np.integer)
+# float covers numpy.float64; float32 / float16 angles keep the NumPy forms,
+# which evaluate in their own precision
+_REAL_SCALARS = (float, int, np.integer)
 
 
 def _scalars(*xs):
-    """Whether every angle is a real Python / NumPy scalar: those take the
-    scalar `math` / `cmath` forms below (one array built per matrix, a few
-    microseconds less per gate than NumPy scalar arithmetic; the same
-    formulas, equal to within an ulp)."""
+    """Whether every angle is a real double-precision or integer scalar:
+    those take the scalar `math` / `cmath` forms below (one array built per
+    matrix, a few microseconds less per gate than NumPy scalar arithmetic;
+    the same formulas, equal to within an ulp)."""
     return all(isinstance(x, _REAL_SCALARS) and not isinstance(x, bool) for x in xs)
 
 
