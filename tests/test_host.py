@@ -100,3 +100,25 @@ def test_qubits_tensor_and_custom_matrix():
 def test_unknown_platform():
     with pytest.raises(ValueError):
         qibojit_amd.MetaBackend.load("tpu")
+
+
+def test_scalar_angle_matrices_match_array_forms(dtype):
+    """RX, RY, RZ, U1, U3 built from scalar math (double or integer angles)
+    against the same formulas in NumPy, taken for angles the scalar forms
+    do not accept (0-d arrays): equal to within an ulp of each precision,
+    and float32 angles keep NumPy's single-precision evaluation."""
+    from qibojit_amd.qibo_compat.npmatrices import NumpyMatrices, _scalars
+
+    nm = NumpyMatrices(dtype)
+    tol = 1e-15 if dtype == "complex128" else 2e-7
+    rng = np.random.default_rng(31)
+    for _ in range(200):
+        a, b, c = rng.uniform(-8, 8, 3)
+        assert _scalars(a, b, c) and not _scalars(np.array(a))
+        for name, args in (("RX", (a,)), ("RY", (a,)), ("RZ", (a,)), ("U1", (a,)), ("U3", (a, b, c))):
+            fast = getattr(nm, name)(*args)
+            slow = getattr(nm, name)(*(np.array(x) for x in args))
+            assert fast.dtype == slow.dtype == np.dtype(dtype)
+            assert np.max(np.abs(fast - slow)) <= tol, (name, args)
+    assert not _scalars(np.float32(0.3)) and not _scalars(True)
+    assert nm.RY(3).dtype == np.dtype(dtype)
