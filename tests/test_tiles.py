@@ -156,6 +156,29 @@ def test_diagonal_gates_become_phase_ops():
     np.testing.assert_allclose(got, er.apply_gate(psi, gm.RZ(0.3), [1]), atol=1e-15)
 
 
+def test_fixed_matrix_shortcuts_match_the_matrix_path():
+    """X / CNOT / TOFFOLI, Z / CZ / CCZ and SWAP skip the matrix encoding in
+    gate_to_op: their ops equal the ones built from the dense target block
+    (pinned by running both through the emulator against the port)."""
+    rng = np.random.default_rng(5)
+    n = 5
+    cases_ = [gates.X(1), gates.CNOT(3, 0), gates.TOFFOLI(0, 4, 2), gates.Z(2), gates.CZ(4, 1),
+              gates.CCZ(0, 1, 3), gates.SWAP(1, 3), gates.SWAP(0, 4).controlled_by(2), gates.X(3).controlled_by(0, 1)]
+    for g in cases_:
+        op = tiles.gate_to_op(port.custom_matrix, g, n)
+        u = tiles.target_block(port.custom_matrix, g)
+        if isinstance(op, tiles.PhaseOp):
+            np.testing.assert_allclose(np.diag(u), [1, np.exp(1j * sum(op.thetas.values()))], atol=1e-15)
+            assert op.c0 == 0.0 and list(op.thetas) == [n - 1 - g.target_qubits[0]]
+        elif isinstance(op, tiles.DenseOp):
+            np.testing.assert_array_equal(op.matrix, u)
+        assert sorted(op.controls) == sorted(n - 1 - q for q in g.control_qubits)
+        psi = gm.random_state(n, rng)
+        got, _ = _run_planned([g], psi, n, 2, 3)
+        want = port.PortBackend(1).apply_gate(g, psi.copy(), n)
+        np.testing.assert_allclose(got, want, atol=1e-14)
+
+
 def test_plan_blocks_respects_dependencies_and_width():
     import itertools
 
