@@ -139,7 +139,8 @@ def kernel_name(backend, gate, n, dtype):
         kind = {"apply_swap": 1, "apply_fsim": 2}.get(GATE_OPS.get(name), 0)
         return f"k_two_qubit<{r},{kind},{v}>"
     if k == 3 and len(gate.control_qubits) == 0 and n >= 11 and all(n - 1 - t < 3 for t in gate.target_qubits):
-        return f"k_multi_low3<{r}>"  # the three lowest bits: LDS-staged 2048-amplitude chunks
+        chunk = 1024 if dtype == "complex128" else 2048
+        return f"k_multi_low3<{r}, {chunk}>"  # the three lowest bits: LDS-staged chunks
     return f"k_multi_reg<{r},{k}>"
 
 

@@ -349,9 +349,10 @@ __global__ void __launch_bounds__(kBlock)
 
 // ---- three targets on bits {0, 1, 2}, uncontrolled: LDS-staged chunks --
 // Each thread of the register kernel would own 8 consecutive amplitudes, so
-// its loads stride 128 B across a wave.  Here a block moves 2048 consecutive
-// amplitudes with coalesced loads into LDS, each thread then takes one
-// 8-amplitude group (2048 / 8 = 256 groups), applies the 8x8 matrix in
+// its loads stride 128 B across a wave.  Here a block moves CHUNK (1024 for
+// complex128, 2048 for complex64) consecutive amplitudes with coalesced
+// loads into LDS, each of its CHUNK / 8 threads then takes one
+// 8-amplitude group, applies the 8x8 matrix in
 // registers (rows summed in ascending column order), writes the group back
 // to LDS, and the block stores the chunk coalesced.  LDS slot s lives at
 // s ^ ((s >> 3) & 7): both the coalesced and the per-group accesses then hit
@@ -365,15 +366,16 @@ constexpr size_t mat_bytes_of(int n) {
 
 __device__ __forceinline__ unsigned low3_swz(unsigned s) { return s ^ ((s >> 3) & 7u); }
 
-template <typename R>
-__global__ void __launch_bounds__(kBlock)
+template <typename R, int CHUNK>
+__global__ void __launch_bounds__(CHUNK / 8)
     k_multi_low3(cplx<R>* __restrict__ st, u64 nchunks, Offs32 roff, const cplx<R>* __restrict__ mat) {
-  __shared__ cplx<R> buf[kLow3Chunk];
+  constexpr int NT = CHUNK / 8;  // one 8-amplitude group per thread
+  __shared__ cplx<R> buf[CHUNK];
   const unsigned t = threadIdx.x;
   for (u64 c = blockIdx.x; c < nchunks; c += gridDim.x) {  // uniform per block
-    cplx<R>* p = st + c * kLow3Chunk;
+    cplx<R>* p = st + c * CHUNK;
 #pragma unroll
-    for (int i = 0; i < kLow3Chunk / kBlock; ++i) buf[low3_swz(i * kBlock + t)] = p[i * kBlock + t];
+    for (int i = 0; i < 8; ++i) buf[low3_swz(i * NT + t)] = p[i * NT + t];
     __syncthreads();
     cplx<R> x[8];
 #pragma unroll
@@ -387,7 +389,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kLow3Chunk / kBlock; ++i) p[i * kBlock + t] = buf[low3_swz(i * kBlock + t)];
+    for (int i = 0; i < 8; ++i) p[i * NT + t] = buf[low3_swz(i * NT + t)];
     __syncthreads();
   }
 }
@@ -704,9 +706,23 @@ static int multi_qubit_impl(void* state, int nqubits, const int32_t* target_bits
       int ticket = -1;
       rc = ring_upload(gate, mat_bytes_of<R>(8), s, &dmat, &ticket);
       if (rc) return rc;
-      const u64 nchunks = (u64(1) << nqubits) / kLow3Chunk;
-      k_multi_low3<R><<<dim3((unsigned)grid_for(nchunks, 1)), dim3(kBlock), 0, s>>>(
-          st, nchunks, roff, static_cast<const cplx<R>*>(dmat));
+      static const int chunk_env = [] {
+        const char* e = std::getenv("QJ_LOW3_CHUNK");
+        const int v = e ? std::atoi(e) : 0;
+        return (v == 1024 || v == 2048 || v == 4096) ? v : 0;
+      }();
+      // 1024 amplitudes per workgroup for complex128 (5.2 -> 5.45 TB/s at
+      // 30 qubits; 4096: 5.42), 2048 for complex64 (1024: -3%, 4096: -4%)
+      const int dflt = sizeof(R) == 8 ? 1024 : kLow3Chunk;
+      const int chunk = (chunk_env && (u64(1) << nqubits) >= (u64)chunk_env) ? chunk_env : dflt;
+      const u64 nchunks = (u64(1) << nqubits) / (u64)chunk;
+      const cplx<R>* dm = static_cast<const cplx<R>*>(dmat);
+      if (chunk == 1024)
+        k_multi_low3<R, 1024><<<dim3((unsigned)grid_for(nchunks, 1)), dim3(128), 0, s>>>(st, nchunks, roff, dm);
+      else if (chunk == 4096)
+        k_multi_low3<R, 4096><<<dim3((unsigned)grid_for(nchunks, 1)), dim3(512), 0, s>>>(st, nchunks, roff, dm);
+      else
+        k_multi_low3<R, 2048><<<dim3((unsigned)grid_for(nchunks, 1)), dim3(256), 0, s>>>(st, nchunks, roff, dm);
       rc = check_launch("apply_multi_qubit launch");
       ring_release(ticket, s);
       return rc;
