@@ -154,14 +154,14 @@ struct Offs {
   u64 o[1 << K];
 };
 
-template <typename R, int K>
-__global__ void __launch_bounds__(kBlock)
+template <typename R, int K, int NT = kBlock>
+__global__ void __launch_bounds__(NT)
     k_multi_reg(cplx<R>* __restrict__ st, u64 ngroups, Ins ins, Offs<K> off,
                 const cplx<R>* __restrict__ mat) {
   constexpr int N = 1 << K;
   __shared__ cplx<R> sm[N * N];
-  const u64 stride = (u64)gridDim.x * kBlock;
-  u64 g = (u64)blockIdx.x * kBlock + threadIdx.x;
+  const u64 stride = (u64)gridDim.x * NT;
+  u64 g = (u64)blockIdx.x * NT + threadIdx.x;
   // the first group's loads are issued before the matrix is staged, so the
   // workgroup barrier does not delay them (one group per thread is the
   // usual launch)
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int c = 0; c < N; ++c) x[c] = st[base + off.o[c]];
   }
-  for (int t = threadIdx.x; t < N * N; t += kBlock) sm[t] = mat[t];
+  for (int t = threadIdx.x; t < N * N; t += NT) sm[t] = mat[t];
   __syncthreads();
   for (; g < ngroups; g += stride) {
 #pragma unroll 1
@@ -668,8 +668,30 @@ static void launch_multi_reg(cplx<R>* st, u64 ngroups, const Ins& ins, const int
       if ((r >> (K - 1 - u)) & 1) o |= u64(1) << tb[u];
     off.o[r] = o;
   }
-  const u64 grid = grid_for(ngroups, kBlock);
-  k_multi_reg<R, K><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, ngroups, ins, off, dmat);
+  static const int nt_env = [] {
+    const char* e = std::getenv("QJ_MREG_BLOCK");
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 128 || v == 256 || v == 512) ? v : 0;
+  }();
+  // Workgroup size of the 3-target kernel by precision and lowest target
+  // bit m, measured for targets m..m+2 at 30 qubits (m = 3, 6, ..., 27):
+  // complex128 256 threads for 9 <= m <= 21 (128 threads lost 5-18% there)
+  // and 128 elsewhere (+2-6%); complex64 128 threads for m >= 9 (+5-30%,
+  // -1.5% at m = 24) and 256 below.  4 and 5 targets keep 256.
+  int nt = kBlock;
+  if (K == 3) {
+    int m = tb[0];
+    for (int u = 1; u < K; ++u) m = tb[u] < m ? tb[u] : m;
+    nt = sizeof(R) == 8 ? ((m >= 9 && m <= 21) ? 256 : 128) : (m >= 9 ? 128 : 256);
+  }
+  if (nt_env) nt = nt_env;
+  const u64 grid = grid_for(ngroups, (u64)nt);
+  if (nt == 128)
+    k_multi_reg<R, K, 128><<<dim3((unsigned)grid), dim3(128), 0, s>>>(st, ngroups, ins, off, dmat);
+  else if (nt == 512)
+    k_multi_reg<R, K, 512><<<dim3((unsigned)grid), dim3(512), 0, s>>>(st, ngroups, ins, off, dmat);
+  else
+    k_multi_reg<R, K><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, ngroups, ins, off, dmat);
 }
 
 template <typename R>
