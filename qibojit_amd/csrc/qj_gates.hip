@@ -38,17 +38,17 @@ struct M2 {
 constexpr int kIlp = 1;
 
 // --------------------------------------------------------------- 1 qubit --
-template <typename R, int KIND, int V>
-__global__ void __launch_bounds__(kBlock)
+template <typename R, int KIND, int V, int NT = kBlock>
+__global__ void __launch_bounds__(NT)
     k_one_qubit(cplx<R>* __restrict__ st, u64 ngroups, u64 tmask, Ins ins, M1<R> g) {
-  const u64 stride = (u64)gridDim.x * (kBlock * kIlp);
-  for (u64 b = (u64)blockIdx.x * (kBlock * kIlp) + threadIdx.x; b < ngroups; b += stride) {
+  const u64 stride = (u64)gridDim.x * (NT * kIlp);
+  for (u64 b = (u64)blockIdx.x * (NT * kIlp) + threadIdx.x; b < ngroups; b += stride) {
     u64 i0[kIlp];
     bool ok[kIlp];
     cvec<R, V> a[kIlp], c[kIlp];
 #pragma unroll
     for (int u = 0; u < kIlp; ++u) {
-      const u64 gv = b + (u64)u * kBlock;
+      const u64 gv = b + (u64)u * NT;
       ok[u] = gv < ngroups;
       i0[u] = ok[u] ? insert_bits(gv * V, ins) : 0;
     }
@@ -89,17 +89,17 @@ __global__ void __launch_bounds__(kBlock)
 // --------------------------------------------------------------- 2 qubits --
 // Amplitude r of a quad sits at base + (r>>1)*q1 + (r&1)*q2, q1/q2 = the
 // strides of the first/second listed target (matrix index 2*t1 + t2).
-template <typename R, int KIND, int V>
-__global__ void __launch_bounds__(kBlock)
+template <typename R, int KIND, int V, int NT = kBlock>
+__global__ void __launch_bounds__(NT)
     k_two_qubit(cplx<R>* __restrict__ st, u64 ngroups, u64 q1, u64 q2, Ins ins, M2<R> g) {
-  const u64 stride = (u64)gridDim.x * (kBlock * kIlp);
-  for (u64 b = (u64)blockIdx.x * (kBlock * kIlp) + threadIdx.x; b < ngroups; b += stride) {
+  const u64 stride = (u64)gridDim.x * (NT * kIlp);
+  for (u64 b = (u64)blockIdx.x * (NT * kIlp) + threadIdx.x; b < ngroups; b += stride) {
     u64 i0[kIlp];
     bool ok[kIlp];
     cvec<R, V> a0[kIlp], a1[kIlp], a2[kIlp], a3[kIlp];
 #pragma unroll
     for (int u = 0; u < kIlp; ++u) {
-      const u64 gv = b + (u64)u * kBlock;
+      const u64 gv = b + (u64)u * NT;
       ok[u] = gv < ngroups;
       i0[u] = ok[u] ? insert_bits(gv * V, ins) : 0;
     }
@@ -550,6 +550,18 @@ static int build_spec(int nqubits, const int* targets, int ntargets, const int32
 
 // complex64: amplitudes per lane access of the one- and two-qubit kernels;
 // QJ_C64_LANES=1 or 2 forces one for every target position.
+// QJ_GATE_BLOCK=128|256|512: workgroup size of the one- and two-target
+// strided kernels (default 256; 128 for complex64 one-target gates with one
+// amplitude per lane and the target at bit >= 22).
+static int gate_block_env() {
+  static const int v = [] {
+    const char* e = std::getenv("QJ_GATE_BLOCK");
+    const int x = e ? std::atoi(e) : 0;
+    return (x == 128 || x == 512) ? x : kBlock;
+  }();
+  return v;
+}
+
 static int c64_lanes_env() {
   static const int v = [] {
     const char* e = std::getenv("QJ_C64_LANES");
@@ -561,9 +573,14 @@ static int c64_lanes_env() {
 
 template <typename R, int KIND, int V>
 static void launch_one(cplx<R>* st, u64 ngroups, u64 tmask, const Ins& ins, const M1<R>& g,
-                       hipStream_t s) {
-  const u64 grid = grid_for(ngroups, (u64)kBlock * kIlp);
-  k_one_qubit<R, KIND, V><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, ngroups, tmask, ins, g);
+                       hipStream_t s, int nt) {
+  const u64 grid = grid_for(ngroups, (u64)nt * kIlp);
+  if (nt == 128)
+    k_one_qubit<R, KIND, V, 128><<<dim3((unsigned)grid), dim3(128), 0, s>>>(st, ngroups, tmask, ins, g);
+  else if (nt == 512)
+    k_one_qubit<R, KIND, V, 512><<<dim3((unsigned)grid), dim3(512), 0, s>>>(st, ngroups, tmask, ins, g);
+  else
+    k_one_qubit<R, KIND, V><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, ngroups, tmask, ins, g);
 }
 
 template <typename R>
@@ -599,9 +616,15 @@ static int one_qubit_impl(void* state, int nqubits, int kind, int m, const int32
   const u64 ngroups = vec ? npairs / 2 : npairs;
   const u64 tmask = u64(1) << m;
   cplx<R>* st = static_cast<cplx<R>*>(state);
+  // workgroup size: 128 threads for complex64 one-amplitude lanes with the
+  // target at bit >= 22 (H at bits 24 / 27 / 29 of 30 qubits: +7-14% over
+  // 256), else 256 (128 or 512 lost up to 17% elsewhere); QJ_GATE_BLOCK
+  // overrides
+  static const bool nt_forced = std::getenv("QJ_GATE_BLOCK") != nullptr;
+  const int nt = nt_forced ? gate_block_env() : ((sizeof(R) == 4 && !vec && m >= 22) ? 128 : kBlock);
 #define QJ_ONE(K)                                                  \
-  if (vec) launch_one<R, K, 2>(st, ngroups, tmask, ins, g, s);    \
-  else launch_one<R, K, 1>(st, ngroups, tmask, ins, g, s);
+  if (vec) launch_one<R, K, 2>(st, ngroups, tmask, ins, g, s, nt); \
+  else launch_one<R, K, 1>(st, ngroups, tmask, ins, g, s, nt);
   switch (kind) {
     case QJ_APPLY_GATE: QJ_ONE(QJ_APPLY_GATE); break;
     case QJ_APPLY_X: QJ_ONE(QJ_APPLY_X); break;
@@ -616,8 +639,14 @@ static int one_qubit_impl(void* state, int nqubits, int kind, int m, const int32
 template <typename R, int KIND, int V>
 static void launch_two(cplx<R>* st, u64 ngroups, u64 q1, u64 q2, const Ins& ins, const M2<R>& g,
                        hipStream_t s) {
-  const u64 grid = grid_for(ngroups, (u64)kBlock * kIlp);
-  k_two_qubit<R, KIND, V><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, ngroups, q1, q2, ins, g);
+  const int nt = gate_block_env();
+  const u64 grid = grid_for(ngroups, (u64)nt * kIlp);
+  if (nt == 128)
+    k_two_qubit<R, KIND, V, 128><<<dim3((unsigned)grid), dim3(128), 0, s>>>(st, ngroups, q1, q2, ins, g);
+  else if (nt == 512)
+    k_two_qubit<R, KIND, V, 512><<<dim3((unsigned)grid), dim3(512), 0, s>>>(st, ngroups, q1, q2, ins, g);
+  else
+    k_two_qubit<R, KIND, V><<<dim3((unsigned)grid), dim3(kBlock), 0, s>>>(st, ngroups, q1, q2, ins, g);
 }
 
 template <typename R>
