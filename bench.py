@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale from a plain 4 GiB copy):
 # profiles/sweep_30q_c128_hbm_traffic.md.  Keyed by (kernel, nqubits).
-PROFILED_TRAFFIC = {("k_wave<double,1,1>", 30): 34.360e9, ("k_one_qubit<double,0,1>", 30): 28.633e9}
+PROFILED_TRAFFIC = {("k_wave<double,1,1>", 30): 34.360e9, ("k_one_qubit<double,0,1,256>", 30): 28.633e9}
 
 
 def parse():
@@ -134,13 +134,21 @@ def kernel_name(backend, gate, n, dtype):
         m = n - 1 - gate.target_qubits[0]
         if v == 2 and kind in (0, 1, 2) and (m >= 22 or m <= 10):
             v = 1  # one complex64 amplitude per lane at these target bits (qj_gates.hip)
-        return f"k_one_qubit<{r},{kind},{v}>"
+        nt = 128 if dtype == "complex64" and v == 1 and m >= 22 else 256  # workgroup size
+        return f"k_one_qubit<{r},{kind},{v},{nt}>"
     if k == 2:
         kind = {"apply_swap": 1, "apply_fsim": 2}.get(GATE_OPS.get(name), 0)
-        return f"k_two_qubit<{r},{kind},{v}>"
+        return f"k_two_qubit<{r},{kind},{v},256>"
     if k == 3 and len(gate.control_qubits) == 0 and n >= 11 and all(n - 1 - t < 3 for t in gate.target_qubits):
         chunk = 1024 if dtype == "complex128" else 2048
-        return f"k_multi_low3<{r}, {chunk}>"  # the three lowest bits: LDS-staged chunks
+        return f"k_multi_low3<{r},{chunk}>"  # the three lowest bits: LDS-staged chunks
+    if k == 3:
+        m = min(n - 1 - t for t in gate.target_qubits)
+        if dtype == "complex128":
+            nt = 256 if 9 <= m <= 21 else 128
+        else:
+            nt = 128 if m >= 9 else 256
+        return f"k_multi_reg<{r},3,{nt}>"
     return f"k_multi_reg<{r},{k}>"
 
 
